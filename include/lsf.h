@@ -1,0 +1,176 @@
+/*
+ * lsf.h -- C ABI of the MI355X (gfx950) FDTD time-stepping library (lightsmith_amd/_lsf.so).
+ *
+ * The library replaces the per-step arrays operations of lightsmith's FDTD Engine
+ * (src/lightsmith/backends/_fdtd/engine.py) and of the per-step source hooks. The reference
+ * is pure Python: its "FFI" is the Engine constructor plus the source hook protocol
+ * bind/correct_H/advance_H/correct_E/advance_E (engine.py:12-13, 665-678). Each entry point
+ * below names the reference interface it stands in for.
+ *
+ * Conventions
+ *   - Every pointer argument named dev_* is device memory owned by the caller (the Python host
+ *     allocates it with torch); the library never allocates or frees field, state or table memory.
+ *   - Field layout: component-major, then z, y, x with x fastest: element (c, i, j, k) of the
+ *     reference's E[3, N0, N1, N2] sits at c*N + (k*N1 + j)*N0 + i, N = N0*N1*N2.
+ *     Field type: float (real fields) or complex64 {float re, im} (fields under a Bloch phase).
+ *   - Flat indices passed to lsf_add_* are in this native layout (the Python host remaps the
+ *     reference's z-fastest indices, engine.py:103-105).
+ *   - Every function returns 0 on success, or a nonzero LSF_E* code; lsf_last_error() gives the
+ *     message (thread-local). Work is enqueued on the caller's HIP stream; nothing synchronises
+ *     the host except lsf_graph_reset.
+ *   - Step counter: dev_counter[0] = completed steps (engine.py:271 "n"); dev_counter[1] = the step
+ *     in progress, valid for work after the E update (the E kernel copies and increments).
+ */
+#ifndef LSF_H
+#define LSF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSF_OK 0
+#define LSF_EINVAL 1   /* bad configuration or argument -> ValueError */
+#define LSF_EHIP 2     /* HIP runtime error -> RuntimeError */
+#define LSF_ESTATE 3   /* call out of order -> RuntimeError */
+
+typedef struct lsf_engine lsf_engine;
+
+/* Grid description. Replaces Engine.__init__'s scalar arguments (engine.py:231-300). */
+typedef struct lsf_grid {
+  int32_t shape[3];          /* N0, N1, N2: the reference's axis order (x, y, z) */
+  int32_t complex_fields;    /* 1: complex64 fields (any Bloch k != 0, engine.py:257), else float */
+  int32_t periodic[3];       /* periodic axes carry ghost planes 0 and n-1 (engine.py:47-66) */
+  double spacing[3];         /* cell size per axis (nm) */
+  double dt;                 /* time step (nm/c) */
+  double bloch_phase[3][2];  /* exp(+i k_a (n_a - 2) d_a) as (cos, sin) per periodic axis (engine.py:595-611) */
+  void* dev_E;               /* 3*N field elements */
+  void* dev_H;
+  int64_t* dev_counter;      /* int64[2], see above */
+  int32_t tick;              /* 1: the E kernel advances dev_counter (the top-level engine); 0: a sub-engine */
+} lsf_grid;
+
+/* Create / destroy an engine handle (engine.py:230-315; release engine.py:711-719). */
+int lsf_create(const lsf_grid* grid, lsf_engine** out);
+void lsf_destroy(lsf_engine* eng);
+
+/* Material, dense form: dev_cb[3][N] float = (dt/dx) * inv_eps per E component (engine.py:299), or
+ * -(m + 1) for an E node integrated in D-form by pure ADE medium m (engine.py:722-793). */
+int lsf_set_material_dense(lsf_engine* eng, const float* dev_cb);
+
+/* Material, indexed form: dev_ids[N] uint32 holding one 8-bit table index per component in bytes 0..2,
+ * dev_table[256] float in the dense form's encoding. Same arithmetic as the dense form, 4 B per cell. */
+int lsf_set_material_indexed(lsf_engine* eng, const uint32_t* dev_ids, const float* dev_table);
+
+/* CPML on one axis for one field (field 0 = E, 1 = H): replaces Engine._build_pml and the per-slab terms
+ * applied in _update_H/_update_E (engine.py:344-418, 636-643, 656-663).
+ * dev_profile[shape[axis]] float4 = {b, c, 1/kappa, slab index (or -1)} at each node position along the axis;
+ * dev_psi[c] (c != axis) = the psi array of component c, slab extent n_slab along the axis, full extent
+ * along the other two (field type). has_kappa = 0 skips the kappa term as the reference does. */
+int lsf_set_pml(lsf_engine* eng, int field, int axis, const void* dev_profile, int n_slab, void* dev_psi0,
+                void* dev_psi1, void* dev_psi2, int has_kappa);
+
+/* Pure ADE groups in D-form (engine.py:722-742, 759-793). n_nodes D-form nodes in native flat order;
+ * dev_chunk_base[(c*N2 + k)*N1 + j][x/64] = index of the chunk's first D-form node.
+ * dev_D[n_nodes]; dev_P[max_poles][n_nodes] pairs {P^n, P^{n-1}} (field type x2).
+ * dev_media[n_media][2 + 3*max_poles] float = {1/eps_inf, n_poles, (c1, c2, c3) per pole}
+ * (pole_coefficients, engine.py:108-111). */
+int lsf_set_ade(lsf_engine* eng, const int32_t* dev_chunk_base, void* dev_D, void* dev_P, int64_t n_nodes,
+                int max_poles, const float* dev_media, int n_media);
+
+/* Phases of a step at which sparse work runs, in this order (engine.py:665-678). */
+#define LSF_PH_CORRECT_H 0  /* after the H update: src.correct_H */
+#define LSF_PH_ADVANCE_H 1  /* src.advance_H */
+#define LSF_PH_OFFDIAG 2    /* inside _update_E: the off-diagonal gather (engine.py:653-655) */
+#define LSF_PH_CORRECT_E 3  /* src.correct_E */
+#define LSF_PH_GROUPS 4     /* groups.step for interface (mixed) groups */
+#define LSF_PH_ADVANCE_E 5  /* src.advance_E */
+
+/* Gather-add rows: F[target[r]] += sum_k w[r, k] * src[src_idx[r, k]] (field 0 = E, 1 = H).
+ * Replaces the off-diagonal rows (engine.py:653-655), the 1D-fed TFSF corrections (tfsf.py:195-204)
+ * and the Bloch plane corrections (tfsf.py:274-286). dev_ade_idx (or NULL) redirects E targets that are
+ * D-form nodes: D[a] += v, E += v * dev_ade_scale[r] (the correction enters before the group update). */
+int lsf_add_gather(lsf_engine* eng, int phase, int field, int64_t n_rows, int width, const int64_t* dev_target,
+                   const void* dev_src, const int64_t* dev_src_idx, const void* dev_w, const int32_t* dev_ade_idx,
+                   const float* dev_ade_scale);
+
+/* Modal injection: F[target[r]] += sum_q m[r, q] * table[t * Q + q], t = the step counter, 0 past table_len.
+ * Replaces PointDipole.correct_E (emission.py:139-140), ModeSource.correct_H/E (ports.py:368-372) and the
+ * Bloch column's current sheet (tfsf.py:291-296). dev_m has the field type; dev_table is float. */
+int lsf_add_modal(lsf_engine* eng, int phase, int field, int64_t n_rows, int n_modes, const int64_t* dev_target,
+                  const void* dev_m, const float* dev_table, int64_t table_len, const int32_t* dev_ade_idx,
+                  const float* dev_ade_scale);
+
+/* Table-fed TFSF corrections (layered.py:169-187, 231-242): for target row r, sum over its points p in
+ * [row_ptr[r], row_ptr[r+1]) of coef[p] * quintic(T, t + i0[p], u0[p], base[p]) with tables sampled every
+ * m steps, read index clamped to cmax. */
+int lsf_add_table(lsf_engine* eng, int phase, int field, int64_t n_rows, const int64_t* dev_target,
+                  const int32_t* dev_row_ptr, const float* dev_T, int m, int64_t cmax, const int64_t* dev_base,
+                  const int64_t* dev_i0, const float* dev_u0, const float* dev_coef, const int32_t* dev_ade_idx,
+                  const float* dev_ade_scale);
+
+/* Incident-wave view of a table source (layered.py:481-482): out[r] = quintic(T, t + i0[r], ...), r < n. */
+int lsf_add_table_view(lsf_engine* eng, int phase, int n, float* dev_out, const float* dev_T, int m, int64_t cmax,
+                       const int64_t* dev_base, const int64_t* dev_i0, const float* dev_u0);
+
+/* Auxiliary 1D Yee grid of a TFSF plane wave (tfsf.py:200-211), float arrays of length M:
+ * advance_H: H[m] = da[m] H[m] - db[m] (E[m+1] - E[m]), m < M-1;
+ * advance_E: E[m] = ca[m] E[m] - cb[m] (H[m] - H[m-1]), 0 < m < M-1; E[m_src] += wave[t]. */
+int lsf_add_line1d(lsf_engine* eng, int M, float* dev_E1, float* dev_H1, const float* dev_ca, const float* dev_cb,
+                   const float* dev_da, const float* dev_db, int m_src, const float* dev_wave, int64_t wave_len);
+
+/* A sub-engine (the Bloch plane source's auxiliary column, tfsf.py:256-262, 288-293) whose H and E
+ * updates run at LSF_PH_ADVANCE_H and LSF_PH_ADVANCE_E of this engine, on its step counter. */
+int lsf_add_subengine(lsf_engine* eng, lsf_engine* sub);
+
+/* Mixed (subpixel two-path) ADE group (engine.py:743-791), at LSF_PH_GROUPS. n nodes, S media slots.
+ * dev_idx[n] native flat E indices; dev_f[S][n] fractions; dev_w[n]; dev_h0[n]; dev_inv_a[n];
+ * dev_state: field-type state of (3 + S + 4*S*max_poles) * n elements, zeroed by the caller;
+ * dev_slot_media[S][2 + 3*max_poles] in lsf_set_ade's media encoding (n_poles = 0: non-dispersive). */
+int lsf_add_mixed_group(lsf_engine* eng, int64_t n, int S, int max_poles, const int64_t* dev_idx, const float* dev_f,
+                        const float* dev_w, const float* dev_h0, const float* dev_inv_a, const float* dev_slot_media,
+                        void* dev_state, int has_h, int has_a);
+
+/* Running DFT monitor (engine.py:556-592): acc_e[f, p] += exp(i w_f t) E_src[e_idx[p]],
+ * acc_h[f, p] += exp(i w_f (t - dt/2)) (H_src[h1[p]] + H_src[h2[p]]) / 2, t = dev_counter[0] * dt, phases in
+ * double and cast to complex64. energy_group 1 or 2 adds sum |E|^2 + |H|^2 into that energy slot
+ * (engine.py:580-589); n_freq <= 2048. src pointers default to this engine's fields when NULL (a source's own arrays
+ * otherwise: src_complex gives their type). */
+int lsf_add_dft_monitor(lsf_engine* eng, int n_freq, int64_t n_pts, const double* dev_omega,
+                        const int64_t* dev_e_idx, const int64_t* dev_h1, const int64_t* dev_h2, void* dev_acc_e,
+                        void* dev_acc_h, const void* dev_e_src, const void* dev_h_src, int src_complex,
+                        int energy_group);
+
+/* Time monitor (engine.py:213-224, 590-592): buf[pos, p] = E[e_idx[p]], pos += 1 per sample, pos < n_max. */
+int lsf_add_time_monitor(lsf_engine* eng, int64_t n_pts, const int64_t* dev_e_idx, void* dev_buf, int64_t n_max,
+                         int64_t* dev_pos);
+
+/* Energy proxy slots (engine.py:315 "energy"), float[2] (or a field-precision pair). */
+int lsf_set_energy(lsf_engine* eng, float* dev_energy);
+
+/* Launch tuning: rows per lane (1, 2 or 4), waves per block (1..16), z planes per block (>= 1). */
+int lsf_set_tiling(lsf_engine* eng, int rows_per_lane, int waves, int z_chunk);
+
+/* Run n_steps steps (Engine.step, engine.py:665-678) on `stream` (a hipStream_t; NULL = default). */
+int lsf_step(lsf_engine* eng, int n_steps, void* stream);
+/* One DFT/time-monitor sample (Engine._sample, engine.py:568-592). */
+int lsf_sample(lsf_engine* eng, void* stream);
+/* n_chunks x (dft_stride steps + one sample) (Engine.chunk, engine.py:680-683). */
+int lsf_chunk(lsf_engine* eng, int n_chunks, int stride, void* stream);
+
+/* HIP graph of one chunk (make_runner, engine.py:685-709): capture on `stream` (not the legacy default
+ * stream), replay n times, drop. */
+int lsf_graph_capture(lsf_engine* eng, int stride, void* stream);
+int lsf_graph_replay(lsf_engine* eng, int n, void* stream);
+int lsf_graph_reset(lsf_engine* eng);
+
+/* Message of the last failing call on this thread. */
+const char* lsf_last_error(void);
+/* Library version string. */
+const char* lsf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LSF_H */

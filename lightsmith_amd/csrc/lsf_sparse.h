@@ -1,0 +1,267 @@
+// Sparse per-step work: source corrections, auxiliary grids, interface ADE groups and monitors.
+// Each kernel is one thread per output row; rows that write the same node never coexist in one
+// launch (the host merges them), so no atomics are needed and results are deterministic.
+#pragma once
+
+#include "lsf_common.h"
+
+namespace lsf {
+
+// A correction v for E (or H) node t. E nodes integrated in D-form take it through D (the reference adds
+// it to E before the group update: D += E_now - E_prev picks it up, and the new E is (D - sum P) / eps_inf).
+template <class T>
+__device__ __forceinline__ void apply_corr(T* F, long long t, T v, const int* ade_idx, const float* ade_scale,
+                                           T* D, long long r) {
+  if (ade_idx) {
+    const int q = ade_idx[r];
+    if (q >= 0) {
+      D[q] = add(D[q], v);
+      F[t] = add(F[t], scl(ade_scale[r], v));
+      return;
+    }
+  }
+  F[t] = add(F[t], v);
+}
+
+template <class T>
+__global__ void k_gather(long long n_rows, int width, const long long* __restrict__ target, T* F,
+                         const T* __restrict__ src, const long long* __restrict__ sidx, const T* __restrict__ w,
+                         const int* ade_idx, const float* ade_scale, T* D) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rows) return;
+  T acc = zero<T>();
+  for (int q = 0; q < width; ++q) {
+    const long long e = r * width + q;
+    acc = add(acc, mul(w[e], src[sidx[e]]));
+  }
+  apply_corr(F, target[r], acc, ade_idx, ade_scale, D, r);
+}
+
+template <class T>
+__global__ void k_modal(long long n_rows, int nm, const long long* __restrict__ target, T* F,
+                        const T* __restrict__ m, const float* __restrict__ table, long long table_len,
+                        const long long* counter, int slot, const int* ade_idx, const float* ade_scale, T* D) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rows) return;
+  const long long t = counter[slot];
+  if (t < 0 || t >= table_len) return;
+  T acc = zero<T>();
+  for (int q = 0; q < nm; ++q) acc = add(acc, scl(table[t * nm + q], m[r * nm + q]));
+  apply_corr(F, target[r], acc, ade_idx, ade_scale, D, r);
+}
+
+// Quintic interpolation of tables sampled every m steps (layered.py:164-187), float32 as on the GPU
+// path of the reference: (t + i0) / m = c + u with c clamped at cmax.
+__device__ __forceinline__ float quintic(const float* __restrict__ T, long long t, long long i0, float u0, long long base,
+                                         int m, long long cmax) {
+  const long long q = t + i0;
+  long long r = q % m;
+  if (r < 0) r += m;
+  long long c = (q - r) / m;
+  if (c > cmax) c = cmax;
+  const float u = ((float)r + u0) / (float)m;
+  const float d0 = u + 2.f, d1 = u + 1.f, d2 = u, d3 = u - 1.f, d4 = u - 2.f, d5 = u - 3.f;
+  const long long ix = base + c;
+  // weights prod_{k != j} d_k / prod_{k != j} (x_j - x_k), stencil x = -2 .. 3
+  const float w0 = (d1 * d2 * d3 * d4 * d5) / -120.f;
+  const float w1 = (d0 * d2 * d3 * d4 * d5) / 24.f;
+  const float w2 = (d0 * d1 * d3 * d4 * d5) / -12.f;
+  const float w3 = (d0 * d1 * d2 * d4 * d5) / 12.f;
+  const float w4 = (d0 * d1 * d2 * d3 * d5) / -24.f;
+  const float w5 = (d0 * d1 * d2 * d3 * d4) / 120.f;
+  return w0 * T[ix] + w1 * T[ix + 1] + w2 * T[ix + 2] + w3 * T[ix + 3] + w4 * T[ix + 4] + w5 * T[ix + 5];
+}
+
+template <class T>
+__global__ void k_table(long long n_rows, const long long* __restrict__ target, const int* __restrict__ row_ptr, T* F,
+                        const float* __restrict__ tab, int m, long long cmax, const long long* __restrict__ base,
+                        const long long* __restrict__ i0, const float* __restrict__ u0, const float* __restrict__ coef,
+                        const long long* counter, int slot, const int* ade_idx, const float* ade_scale, T* D) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rows) return;
+  const long long t = counter[slot];
+  float acc = 0.f;
+  for (int p = row_ptr[r]; p < row_ptr[r + 1]; ++p) acc += coef[p] * quintic(tab, t, i0[p], u0[p], base[p], m, cmax);
+  T v = zero<T>();
+  if constexpr (sizeof(T) == sizeof(float)) v = acc;
+  else v = mk(acc, 0.f);
+  apply_corr(F, target[r], v, ade_idx, ade_scale, D, r);
+}
+
+__global__ void k_table_view(int n, float* out, const float* __restrict__ tab, int m, long long cmax,
+                             const long long* __restrict__ base, const long long* __restrict__ i0,
+                             const float* __restrict__ u0, const long long* counter, int slot) {
+  const int r = threadIdx.x;
+  if (r >= n) return;
+  out[r] = quintic(tab, counter[slot], i0[r], u0[r], base[r], m, cmax);
+}
+
+// Auxiliary 1D grid of a TFSF plane wave (tfsf.py:200-211). One block strides over the line.
+__global__ void k_line_h(int M, const float* __restrict__ E1, float* H1, const float* __restrict__ da,
+                         const float* __restrict__ db) {
+  for (int m = threadIdx.x; m < M - 1; m += blockDim.x) H1[m] = da[m] * H1[m] - db[m] * (E1[m + 1] - E1[m]);
+}
+
+__global__ void k_line_e(int M, float* E1, const float* __restrict__ H1, const float* __restrict__ ca,
+                         const float* __restrict__ cb, int m_src, const float* __restrict__ wave, long long wave_len,
+                         const long long* counter, int slot) {
+  const long long t = counter[slot];
+  const float w = (t >= 0 && t < wave_len) ? wave[t] : 0.f;
+  for (int m = 1 + threadIdx.x; m < M - 1; m += blockDim.x) {
+    float e = ca[m] * E1[m] - cb[m] * (H1[m] - H1[m - 1]);
+    if (m == m_src) e += w;
+    E1[m] = e;
+  }
+}
+
+// Mixed (interface) ADE group, engine.py:743-791. State per node: D, E_prev, E_a, E_h[S], then per slot s and
+// pole p the pairs (P, P_prev) of the harmonic path and of the arithmetic path.
+template <class T>
+__global__ void k_mixed(long long n, int S, int maxp, const long long* __restrict__ idx, const float* __restrict__ f,
+                        const float* __restrict__ w, const float* __restrict__ h0, const float* __restrict__ inv_a,
+                        const float* __restrict__ sm, T* st, int has_h, int has_a, T* E) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int ms = 2 + 3 * maxp;
+  T* D = st;
+  T* Ep = st + n;
+  T* Ea = st + 2 * n;
+  T* Eh = st + 3 * n;                              // [S][n]
+  T* Ph = st + (3 + S) * n;                        // [S][maxp][2][n]
+  T* Pa = st + (3 + S + 2 * S * maxp) * n;         // [S][maxp][2][n]
+  const T e_now = E[idx[r]];
+  const T d = add(D[r], sub(e_now, Ep[r]));
+  D[r] = d;
+  T out = zero<T>();
+  bool first = true;
+  if (has_h) {
+    T harm = scl(h0[r], d);
+    for (int s = 0; s < S; ++s) {
+      const float* mp = sm + s * ms;
+      const int np = (int)mp[1];
+      if (np == 0) continue;
+      T tot = zero<T>();
+      const T eh = Eh[s * n + r];
+      for (int p = 0; p < np; ++p) {
+        T* P0 = Ph + ((long long)(s * maxp + p) * 2) * n;
+        T* P1 = P0 + n;
+        const T pn = add(add(scl(mp[2 + 3 * p], P0[r]), scl(mp[3 + 3 * p], P1[r])), scl(mp[4 + 3 * p], eh));
+        P1[r] = P0[r];
+        P0[r] = pn;
+        tot = p == 0 ? pn : add(tot, pn);
+      }
+      const T ehn = scl(mp[0], sub(d, tot));
+      Eh[s * n + r] = ehn;
+      harm = add(harm, scl(f[s * n + r], ehn));
+    }
+    out = scl(w[r], harm);
+    first = false;
+  }
+  if (has_a) {
+    T acc = d;
+    const T ea = Ea[r];
+    for (int s = 0; s < S; ++s) {
+      const float* mp = sm + s * ms;
+      const int np = (int)mp[1];
+      if (np == 0) continue;
+      T tot = zero<T>();
+      for (int p = 0; p < np; ++p) {
+        T* P0 = Pa + ((long long)(s * maxp + p) * 2) * n;
+        T* P1 = P0 + n;
+        const T pn = add(add(scl(mp[2 + 3 * p], P0[r]), scl(mp[3 + 3 * p], P1[r])), scl(mp[4 + 3 * p], ea));
+        P1[r] = P0[r];
+        P0[r] = pn;
+        tot = p == 0 ? pn : add(tot, pn);
+      }
+      acc = sub(acc, scl(f[s * n + r], tot));
+    }
+    const T ean = scl(inv_a[r], acc);
+    Ea[r] = ean;
+    const T part = scl(1.f - w[r], ean);
+    out = first ? part : add(out, part);
+  }
+  E[idx[r]] = out;
+  Ep[r] = out;
+}
+
+// Running DFT monitor (engine.py:564-592). Block size 256; phases for all frequencies in LDS, computed in
+// double from the step counter and cast to complex64 as the reference does.
+template <class S>
+__global__ void k_dft(int nf, long long np, const double* __restrict__ omega, double dt,
+                      const long long* counter, const S* __restrict__ esrc, const S* __restrict__ hsrc,
+                      const long long* __restrict__ eidx, const long long* __restrict__ h1,
+                      const long long* __restrict__ h2, c32* acc_e, c32* acc_h, float* partial) {
+  extern __shared__ c32 ph[];  // [2 * nf]
+  __shared__ float red[256];
+  const double te = (double)counter[0] * dt;
+  const double th = te - 0.5 * dt;
+  for (int q = threadIdx.x; q < nf; q += blockDim.x) {
+    const double we = omega[q] * te, wh = omega[q] * th;
+    ph[q] = mk((float)cos(we), (float)sin(we));
+    ph[nf + q] = mk((float)cos(wh), (float)sin(wh));
+  }
+  __syncthreads();
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  float e2 = 0.f;
+  if (p < np) {
+    const S se = esrc[eidx[p]];
+    e2 += norm2(se);
+    for (int q = 0; q < nf; ++q) {
+      const c32 v = mul(ph[q], to_c(se));
+      c32 a = acc_e[q * np + p];
+      acc_e[q * np + p] = add(a, v);
+    }
+    if (acc_h) {
+      const S sh = scl(0.5f, add(hsrc[h1[p]], hsrc[h2[p]]));
+      e2 += norm2(sh);
+      for (int q = 0; q < nf; ++q) {
+        const c32 v = mul(ph[nf + q], to_c(sh));
+        c32 a = acc_h[q * np + p];
+        acc_h[q * np + p] = add(a, v);
+      }
+    }
+  }
+  if (partial) {
+    red[threadIdx.x] = e2;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  }
+}
+
+// Sum each monitor's block partials in a fixed order and set the energy slots (engine.py:580-589).
+// mon_off[m] .. mon_off[m+1] are monitor m's partials, mon_group[m] its slot (0, 1), -1 for none.
+__global__ void k_energy(int n_mon, const int* __restrict__ mon_off, const int* __restrict__ mon_group,
+                         const float* __restrict__ partial, float* energy) {
+  if (threadIdx.x != 0) return;
+  double tot[2] = {0.0, 0.0};
+  int have[2] = {0, 0};
+  for (int m = 0; m < n_mon; ++m) {
+    const int g = mon_group[m];
+    if (g < 0) continue;
+    double s = 0.0;
+    for (int q = mon_off[m]; q < mon_off[m + 1]; ++q) s += partial[q];
+    tot[g] += s;
+    have[g] = 1;
+  }
+  for (int g = 0; g < 2; ++g)
+    if (have[g]) energy[g] = (float)tot[g];
+}
+
+template <class T>
+__global__ void k_timemon(long long np, const long long* __restrict__ eidx, const T* __restrict__ E, T* buf,
+                          long long n_max, const long long* pos) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long s = pos[0];
+  if (p >= np || s < 0 || s >= n_max) return;
+  buf[s * np + p] = E[eidx[p]];
+}
+
+__global__ void k_bump(long long* pos) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) pos[0] += 1;
+}
+
+}  // namespace lsf

@@ -1,0 +1,728 @@
+"""The FDTD engine on MI355X: a drop-in for lightsmith's ``Engine`` (engine.py:230-719).
+
+The constructor takes the reference's arguments (shape, dx, dt, single, inv_eps, groups, pml, sources,
+monitors, dft_stride, periodic, bloch, offdiag, time_monitors, mirror, widths) and hands the time stepping
+to the HIP library ``_lsf.so``: one H kernel and one E kernel per step (curl + CPML fused, the pure-ADE
+update fused into the E kernel), plus sparse kernels for sources, interface groups and monitors.
+Host set-up runs once in NumPy (float64) and uploads float32 tables.
+
+Layout: each field component is stored z-major with x fastest (element (c, i, j, k) of the reference's
+E[3, N0, N1, N2] at c*N + (k*N1 + j)*N0 + i). ``E``/``H`` are views in the reference's logical shape
+(3, N0, N1, N2); ``E_flat``/``H_flat`` are the native flat arrays; ``native_index`` maps the reference's
+flat indices (engine.py:103-105) to native ones. Periodic ghost planes are read through the Bloch phase
+inside the kernels and are not materialised in the arrays.
+
+Precision: float32 fields (complex64 under a Bloch phase), as the reference's GPU path (single=True).
+Not supported here: mirror planes and graded grids (the reference's symmetry.py and grid.py paths), and
+the vertex-tensor interface groups (TensorMesh).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .._lib import (PH_ADVANCE_E, PH_ADVANCE_H, PH_CORRECT_E, PH_CORRECT_H, PH_GROUPS, PH_OFFDIAG, check, ptr)
+from .media import Medium, pole_coefficients
+
+__all__ = ["Engine", "PMLSpec", "DFTMonitor", "TimeMonitor", "DispersiveGroup", "OffDiagonal", "flat_index",
+           "e_region", "h_region"]
+
+
+def flat_index(c, i, j, k, shape):
+    """The reference's flat index into E[3, N0, N1, N2] (engine.py:103-105)."""
+    n0, n1, n2 = shape
+    return ((np.asarray(c) * n0 + np.asarray(i)) * n1 + np.asarray(j)) * n2 + np.asarray(k)
+
+
+def e_region(c: int, shape, periodic=(False, False, False)):
+    """Stepped index box of E_c (engine.py:47-60, no mirror planes)."""
+    return tuple(slice(1, n - 1) if (periodic[a] or a != c) else slice(0, n - 1) for a, n in enumerate(shape))
+
+
+def h_region(c: int, shape, periodic=(False, False, False)):
+    """Stepped index box of H_c (engine.py:63-66, no mirror planes)."""
+    return tuple(slice(1, n - 1) if periodic[a] else (slice(0, n) if a == c else slice(0, n - 1))
+                 for a, n in enumerate(shape))
+
+
+@dataclass
+class PMLSpec:
+    """engine.py:184-191"""
+
+    cells: tuple
+    m: float = 3.0
+    kappa_max: float = 1.0
+    alpha_max: float = 0.0
+    eps_ref: float = 1.0
+    eps_ref_high: float | None = None
+
+
+@dataclass
+class DispersiveGroup:
+    """engine.py:129-137: E nodes in D-form; pure when ``w`` is None (one medium)."""
+
+    idx: np.ndarray
+    media: list
+    frac: np.ndarray
+    w: np.ndarray | None
+
+
+@dataclass
+class OffDiagonal:
+    """engine.py:117-126"""
+
+    idx3: np.ndarray
+    comp: np.ndarray
+    coef: np.ndarray
+
+
+@dataclass
+class DFTMonitor:
+    """engine.py:194-210: running DFT of E at ``e_idx`` and of (H[h_idx1] + H[h_idx2]) / 2."""
+
+    omega: np.ndarray
+    e_idx: np.ndarray
+    h_idx1: np.ndarray | None = None
+    h_idx2: np.ndarray | None = None
+    energy: int = 0
+    source: object | None = None
+    acc_e: object = field(default=None, repr=False)
+    acc_h: object = field(default=None, repr=False)
+
+    def results(self):
+        e = self.acc_e.cpu().numpy().astype(complex)
+        h = None if self.acc_h is None else self.acc_h.cpu().numpy().astype(complex)
+        return e, h
+
+
+@dataclass
+class TimeMonitor:
+    """engine.py:213-224"""
+
+    e_idx: np.ndarray
+    n_max: int
+    buf: object = field(default=None, repr=False)
+    pos: object = field(default=None, repr=False)
+
+    def results(self, n: int) -> np.ndarray:
+        return self.buf.cpu().numpy()[:n].T
+
+
+def cpml_profile(spec: PMLSpec, n_lo: int, n_hi: int, N: int, h: float, dt: float, u: np.ndarray, high_eps=None):
+    """(b, c, kappa) of the CPML grading at positions u (engine.py:358-371): polynomial order m,
+    sigma_max = 0.8 (m + 1) / (sqrt(eps_ref) h), CFS alpha falling linearly into the layer."""
+    s_lo = 0.8 * (spec.m + 1) / (np.sqrt(spec.eps_ref) * h)
+    s_hi = 0.8 * (spec.m + 1) / (np.sqrt(spec.eps_ref_high or spec.eps_ref) * h)
+    r_lo = np.clip((n_lo - u) / n_lo, 0, 1) if n_lo else np.zeros_like(u)
+    r_hi = np.clip((u - (N - 1 - n_hi)) / n_hi, 0, 1) if n_hi else np.zeros_like(u)
+    rho = np.maximum(r_lo, r_hi)
+    sig = np.where(r_lo >= r_hi, s_lo, s_hi) * rho ** spec.m
+    kap = 1 + (spec.kappa_max - 1) * rho ** spec.m
+    alp = spec.alpha_max * (1 - rho) * (rho > 0)
+    b = np.exp(-(sig / kap + alp) * dt)
+    den = sig * kap + kap ** 2 * alp
+    c = np.where(sig > 0, sig * (b - 1) / np.where(den > 0, den, 1), 0.0)
+    return b, c, kap
+
+
+def _device_of(device) -> torch.device:
+    if hasattr(device, "native_device"):
+        device = device.native_device()
+    elif hasattr(device, "index") and hasattr(device, "framework"):
+        device = getattr(device, "name", "cuda:0")
+    dev = torch.device(device if device is not None else "cuda:0")
+    if dev.type != "cuda":
+        raise ValueError(f"lightsmith_amd runs on a GPU (torch 'cuda' = HIP); got device {dev}")
+    return dev
+
+
+class Engine:
+    """FDTD engine whose stepping runs in the HIP library (engine.py:230-315 constructor contract)."""
+
+    # launch tiling (rows per lane, waves per block, z planes per block); see lsf_set_tiling
+    tiling = (2, 4, 16)
+
+    def __init__(self, device=None, *, shape, dx, dt: float, single: bool = True, inv_eps: np.ndarray,
+                 groups=(), pml: PMLSpec | None = None, sources=(), monitors=(), dft_stride: int = 1,
+                 periodic=(False, False, False), bloch=(0.0, 0.0, 0.0), offdiag: OffDiagonal | None = None,
+                 time_monitors=(), mirror=(0, 0, 0), widths=None, material: str = "auto", _sub: bool = False,
+                 _counter=None):
+        if not single:
+            raise ValueError("the HIP engine steps float32 fields (single=True); double precision runs on the CPU")
+        if any(mirror):
+            raise ValueError("mirror planes are not supported by the HIP engine (run symmetry='off')")
+        if widths is not None and any(w is not None for w in widths):
+            raise ValueError("graded grids are not supported by the HIP engine (grid='uniform')")
+        self.lib = _lib.load()
+        self.device = device
+        self._dev = _device_of(device)
+        self.xp = torch
+        self.shape = tuple(int(n) for n in shape)
+        self.dt, self.single = float(dt), True
+        self.spacing = tuple(float(v) for v in np.broadcast_to(np.asarray(dx, dtype=float), (3,)))
+        self.uniform = len(set(self.spacing)) == 1
+        self.dx = self.spacing[0]
+        self.periodic = tuple(bool(p) for p in periodic)
+        self.bloch = tuple(float(k) for k in bloch)
+        if any(k and not p for k, p in zip(self.bloch, self.periodic)):
+            raise ValueError("a Bloch wavevector needs a periodic axis")
+        self.complex = any(self.bloch)
+        self.mirror = (0, 0, 0)
+        self.stride = int(dft_stride)
+        self.rdt = torch.float32
+        self.cdt = torch.complex64
+        self.fdt = self.cdt if self.complex else self.rdt
+        n0, n1, n2 = self.shape
+        self.N = n0 * n1 * n2
+        self._keep = []  # device tables whose lifetime is the engine's
+        self.E_flat = torch.zeros(3 * self.N, dtype=self.fdt, device=self._dev)
+        self.H_flat = torch.zeros(3 * self.N, dtype=self.fdt, device=self._dev)
+        self.E = self.E_flat.view(3, n2, n1, n0).permute(0, 3, 2, 1)
+        self.H = self.H_flat.view(3, n2, n1, n0).permute(0, 3, 2, 1)
+        self._counter = _counter if _counter is not None else torch.zeros(2, dtype=torch.int64, device=self._dev)
+        self.n = self._counter[0:1]
+        self.n_host = 0
+        self.RE = [e_region(c, self.shape, self.periodic) for c in range(3)]
+        self.RH = [h_region(c, self.shape, self.periodic) for c in range(3)]
+        self.cdx = self.dt / self.dx if self.uniform else self.dt  # engine.py:275
+        self.cH = self.cdx
+
+        g = _lib.LsfGrid()
+        g.shape[:] = [n0, n1, n2]
+        g.complex_fields = int(self.complex)
+        g.periodic[:] = [int(p) for p in self.periodic]
+        g.spacing[:] = list(self.spacing)
+        g.dt = self.dt
+        for a in range(3):
+            kl = self.bloch[a] * (self.shape[a] - 2) * self.spacing[a] if self.periodic[a] else 0.0
+            g.bloch_phase[a][0], g.bloch_phase[a][1] = np.cos(kl), np.sin(kl)
+        g.dev_E, g.dev_H, g.dev_counter = ptr(self.E_flat), ptr(self.H_flat), ptr(self._counter)
+        g.tick = 0 if _sub else 1
+        handle = ctypes.c_void_p()
+        check(self.lib.lsf_create(ctypes.byref(g), ctypes.byref(handle)), "lsf_create")
+        self._h = handle
+        ry, waves, zc = self.tiling
+        check(self.lib.lsf_set_tiling(self._h, ry, waves, zc), "lsf_set_tiling")
+
+        inv_eps = np.asarray(inv_eps, dtype=float)
+        if inv_eps.shape != (3,) + self.shape:
+            raise ValueError(f"inv_eps must have shape {(3,) + self.shape}, got {inv_eps.shape}")
+        groups = list(groups)
+        for grp in groups:
+            if not isinstance(grp, DispersiveGroup):
+                raise ValueError(f"unsupported dispersive group type {type(grp).__name__} (vertex tensors: "
+                                 "mesh_refinement other than 'conformal')")
+        self._build_ade(groups)
+        self._set_material(inv_eps, material)
+        self._build_pml(pml or PMLSpec(cells=((0, 0),) * 3), inv_eps)
+        self._build_mixed(groups)
+        self._build_offdiag(offdiag, groups)
+        self.energy = torch.zeros(2, dtype=torch.float32, device=self._dev)
+        check(self.lib.lsf_set_energy(self._h, ptr(self.energy)))
+        self.sources = list(sources)
+        for s in self.sources:
+            if not hasattr(s, "bind_native"):
+                raise TypeError(f"source {type(s).__name__} has no HIP binding (bind_native)")
+            s.bind(self)
+        self.monitors = list(monitors)
+        for m in self.monitors:
+            self._bind_monitor(m)
+        self.time_monitors = list(time_monitors)
+        for m in self.time_monitors:
+            self._bind_time_monitor(m)
+        self._graph_stream = None
+        self._graphed = False
+
+    # --- helpers ---------------------------------------------------------------------------
+    def native_index(self, flat) -> np.ndarray:
+        """Native flat index of reference flat indices into the stacked (3, N0, N1, N2) arrays."""
+        n0, n1, n2 = self.shape
+        flat = np.asarray(flat, dtype=np.int64)
+        k = flat % n2
+        rest = flat // n2
+        j = rest % n1
+        rest //= n1
+        i = rest % n0
+        c = rest // n0
+        return c * self.N + (k * n1 + j) * n0 + i
+
+    def reference_index(self, native) -> np.ndarray:
+        """Inverse of native_index."""
+        n0, n1, n2 = self.shape
+        native = np.asarray(native, dtype=np.int64)
+        c, cell = np.divmod(native, self.N)
+        i = cell % n0
+        rest = cell // n0
+        j = rest % n1
+        k = rest // n1
+        return ((c * n0 + i) * n1 + j) * n2 + k
+
+    def to_device(self, a, dtype=None):
+        """Upload a NumPy array (kept alive for the engine's lifetime)."""
+        a = np.asarray(a)
+        if dtype is None:
+            dtype = {np.dtype(np.int64): torch.int64, np.dtype(np.int32): torch.int32,
+                     np.dtype(np.float64): torch.float32, np.dtype(np.float32): torch.float32,
+                     np.dtype(np.complex128): torch.complex64, np.dtype(np.complex64): torch.complex64,
+                     np.dtype(np.uint32): torch.int32}[a.dtype]
+        if a.dtype == np.uint32:
+            a = a.view(np.int32)
+        t = torch.as_tensor(np.ascontiguousarray(a)).to(device=self._dev, dtype=dtype)
+        self._keep.append(t)
+        return t
+
+    def zeros(self, n, complex_=None):
+        t = torch.zeros(int(n), dtype=self.fdt if complex_ is None else (self.cdt if complex_ else self.rdt),
+                        device=self._dev)
+        self._keep.append(t)
+        return t
+
+    def field_values(self, vals) -> torch.Tensor:
+        """Upload coefficients of the field type (complex under a Bloch phase)."""
+        vals = np.asarray(vals)
+        if self.complex:
+            return self.to_device(vals.astype(np.complex128), torch.complex64)
+        if np.iscomplexobj(vals):
+            vals = vals.real
+        return self.to_device(vals.astype(np.float64), torch.float32)
+
+    @staticmethod
+    def check_index(idx, n: int, what: str):
+        """Every index the kernels dereference is validated here, on the host, before upload."""
+        idx = np.asarray(idx)
+        if idx.size and (idx.min() < 0 or idx.max() >= n):
+            raise ValueError(f"{what}: index out of range [0, {n})")
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self._dev).cuda_stream)
+
+    # --- material and ADE --------------------------------------------------------------------
+    def _build_ade(self, groups):
+        """Pure groups: one D-form state slot per node, ordered by native index; chunk bases for the
+        E kernel's ballot indexing; per-medium pole coefficients (engine.py:722-742)."""
+        pure = [g for g in groups if g.w is None]
+        self.ade_media: list[Medium] = []
+        nodes, meds = [], []
+        for g in pure:
+            (m,) = g.media
+            if not m.dispersive:
+                raise ValueError("a pure D-form group needs a dispersive medium")
+            if m not in self.ade_media:
+                self.ade_media.append(m)
+            mi = self.ade_media.index(m)
+            nat = self.native_index(g.idx)
+            c, cell = np.divmod(nat, self.N)
+            i = cell % self.shape[0]
+            j = (cell // self.shape[0]) % self.shape[1]
+            k = cell // (self.shape[0] * self.shape[1])
+            st = np.zeros(len(nat), bool)
+            for cc in range(3):
+                R = self.RE[cc]
+                sel = c == cc
+                st[sel] = ((i[sel] >= R[0].start) & (i[sel] < R[0].stop) & (j[sel] >= R[1].start) &
+                           (j[sel] < R[1].stop) & (k[sel] >= R[2].start) & (k[sel] < R[2].stop))
+            nodes.append(nat[st])
+            meds.append(np.full(int(st.sum()), mi, dtype=np.int32))
+        self.ade_nodes = np.concatenate(nodes) if nodes else np.zeros(0, np.int64)
+        self.ade_medium = np.concatenate(meds) if meds else np.zeros(0, np.int32)
+        order = np.argsort(self.ade_nodes, kind="stable")
+        self.ade_nodes, self.ade_medium = self.ade_nodes[order], self.ade_medium[order]
+        if len(self.ade_nodes) and np.any(np.diff(self.ade_nodes) == 0):
+            raise ValueError("an E node belongs to two dispersive groups")
+        n = len(self.ade_nodes)
+        self.n_ade = n
+        if not n:
+            return
+        max_p = max(len(m.poles) for m in self.ade_media)
+        self.ade_max_poles = max_p
+        tab = np.zeros((len(self.ade_media), 2 + 3 * max_p))
+        for q, m in enumerate(self.ade_media):
+            tab[q, 0], tab[q, 1] = 1.0 / m.eps_inf, len(m.poles)
+            for p, pole in enumerate(m.poles):
+                tab[q, 2 + 3 * p:5 + 3 * p] = pole_coefficients(pole, self.dt)
+        n0, n1, n2 = self.shape
+        nchunk = (n0 + 63) // 64
+        c, cell = np.divmod(self.ade_nodes, self.N)
+        i = cell % n0
+        row = cell // n0  # k * n1 + j
+        keys = (c * (n1 * n2) + row) * nchunk + i // 64
+        chunk_base = np.searchsorted(keys, np.arange(3 * n1 * n2 * nchunk), side="left").astype(np.int32)
+        self._ade_D = self.zeros(n)
+        self._ade_P = torch.zeros(max_p * n * 2, dtype=self.fdt, device=self._dev)
+        self._keep.append(self._ade_P)
+        self._ade_tab = self.to_device(tab.astype(np.float32))
+        self._ade_chunk = self.to_device(chunk_base)
+        self._ade_inv_eps_inf = np.array([1.0 / m.eps_inf for m in self.ade_media])
+        check(self.lib.lsf_set_ade(self._h, ptr(self._ade_chunk), ptr(self._ade_D), ptr(self._ade_P), n, max_p,
+                                   ptr(self._ade_tab), len(self.ade_media)), "lsf_set_ade")
+
+    def ade_redirect(self, native_targets):
+        """(ade_idx, ade_scale) device arrays for E corrections at native_targets (None if no target is a
+        pure D-form node)."""
+        if not self.n_ade:
+            return None, None
+        t = np.asarray(native_targets, dtype=np.int64)
+        pos = np.searchsorted(self.ade_nodes, t)
+        pos_c = np.minimum(pos, self.n_ade - 1)
+        hit = (pos < self.n_ade) & (self.ade_nodes[pos_c] == t)
+        if not hit.any():
+            return None, None
+        idx = np.where(hit, pos_c, -1).astype(np.int32)
+        scale = np.where(hit, self._ade_inv_eps_inf[self.ade_medium[pos_c]], 0.0).astype(np.float32)
+        return self.to_device(idx), self.to_device(scale)
+
+    def _set_material(self, inv_eps, mode: str):
+        """E coefficients dt/dx * inv_eps per node (engine.py:299), D-form nodes encoded -(medium + 1).
+        Stored as 8-bit indices into a 256-entry table when the grid has at most 256 distinct values."""
+        n0, n1, n2 = self.shape
+        cb = (self.cdx * inv_eps).astype(np.float32).transpose(0, 3, 2, 1).reshape(3, -1)  # native order
+        flat = cb.reshape(-1)
+        if not np.all(np.isfinite(flat)) or flat.min() < 0:
+            raise ValueError("inv_eps must be finite and non-negative (dispersive cells go in groups)")
+        if self.n_ade:
+            flat[self.ade_nodes] = -(self.ade_medium.astype(np.float32) + 1.0)
+        self.material_mode = "dense"
+        if mode in ("auto", "indexed"):
+            try:
+                import pandas as pd  # hash-based unique: O(N)
+                uniq = pd.unique(flat)
+            except ImportError:  # pragma: no cover
+                uniq = np.unique(flat)
+            if len(uniq) <= 256:
+                table = np.sort(np.asarray(uniq, dtype=np.float32))
+                ids = np.searchsorted(table, cb).astype(np.uint32)  # (3, N)
+                packed = ids[0] | (ids[1] << 8) | (ids[2] << 16)
+                tab = np.zeros(256, np.float32)
+                tab[:len(table)] = table
+                self._mat_ids = self.to_device(packed.astype(np.uint32))
+                self._mat_tab = self.to_device(tab)
+                check(self.lib.lsf_set_material_indexed(self._h, ptr(self._mat_ids), ptr(self._mat_tab)))
+                self.material_mode = "indexed"
+                return
+            if mode == "indexed":
+                raise ValueError(f"{len(uniq)} distinct material coefficients: more than an 8-bit table holds")
+        self._mat = self.to_device(flat.astype(np.float32))
+        check(self.lib.lsf_set_material_dense(self._h, ptr(self._mat)))
+
+    # --- CPML (engine.py:344-418) --------------------------------------------------------------
+    def _build_pml(self, spec: PMLSpec, inv_eps):
+        self.pml_cells = tuple(tuple(int(v) for v in c) for c in spec.cells)
+        self.n_psi = 0
+        for d in range(3):
+            n_lo, n_hi = self.pml_cells[d]
+            if not (n_lo or n_hi):
+                continue
+            if self.periodic[d]:
+                raise ValueError("a periodic axis cannot have a PML")
+            N, h = self.shape[d], self.spacing[d]
+            for fld, (start, stop), half in ((0, (1, N - 1), 0.0), (1, (0, N - 1), 0.5)):
+                ranges = []
+                if n_lo:
+                    ranges.append((start, min(stop, n_lo + 1)))
+                if n_hi:
+                    ranges.append((max(start, N - 2 - n_hi), stop))
+                ranges = [(a, b) for a, b in ranges if b > a]
+                prof = np.zeros((N, 4), np.float64)
+                prof[:, 0], prof[:, 2], prof[:, 3] = 1.0, 1.0, -1.0
+                ns, has_k = 0, False
+                for a, b in ranges:
+                    u = np.arange(a, b) + half
+                    bb, cc, kk = cpml_profile(spec, n_lo, n_hi, N, h, self.dt, u)
+                    prof[a:b, 0], prof[a:b, 1], prof[a:b, 2] = bb, cc, 1.0 / kk
+                    prof[a:b, 3] = ns + np.arange(b - a)
+                    has_k |= bool(np.any(kk > 1))
+                    ns += b - a
+                if not ns:
+                    continue
+                others = [self.shape[a] for a in range(3) if a != d]
+                size = ns * others[0] * others[1]
+                psis = [None, None, None]
+                for c in range(3):
+                    if c != d:
+                        psis[c] = self.zeros(size)
+                        self.n_psi += size
+                tprof = self.to_device(prof.astype(np.float32))
+                check(self.lib.lsf_set_pml(self._h, fld, d, ptr(tprof), ns, ptr(psis[0]), ptr(psis[1]),
+                                           ptr(psis[2]), int(has_k)), "lsf_set_pml")
+
+    # --- mixed (interface) groups (engine.py:743-791) --------------------------------------------
+    def _build_mixed(self, groups):
+        for g in groups:
+            if g.w is None:
+                continue
+            media = list(g.media)
+            f = np.asarray(g.frac, dtype=float)
+            S = len(media)
+            maxp = max([len(m.poles) for m in media] + [1])
+            nat = self.native_index(g.idx)
+            self.check_index(nat, 3 * self.N, "mixed group index")
+            h0 = sum(f[s] / m.eps_inf for s, m in enumerate(media) if not m.dispersive) + 0 * f[0]
+            inv_a = 1.0 / sum(f[s] * m.eps_inf for s, m in enumerate(media))
+            w = np.asarray(g.w, dtype=float)
+            sm = np.zeros((S, 2 + 3 * maxp))
+            for s, m in enumerate(media):
+                sm[s, 0], sm[s, 1] = 1.0 / m.eps_inf, len(m.poles)
+                for p, pole in enumerate(m.poles):
+                    sm[s, 2 + 3 * p:5 + 3 * p] = pole_coefficients(pole, self.dt)
+            n = len(nat)
+            state = self.zeros((3 + S + 4 * S * maxp) * n)
+            arrs = [self.to_device(nat), self.to_device(f.astype(np.float32)), self.to_device(w.astype(np.float32)),
+                    self.to_device(np.asarray(h0, np.float32)), self.to_device(np.asarray(inv_a, np.float32)),
+                    self.to_device(sm.astype(np.float32))]
+            check(self.lib.lsf_add_mixed_group(self._h, n, S, maxp, *(ptr(a) for a in arrs), ptr(state),
+                                               int(np.any(w > 0)), int(np.any(w < 1))), "lsf_add_mixed_group")
+
+    # --- off-diagonal tensor terms (engine.py:421-518, uniform grid) ------------------------------
+    def _build_offdiag(self, od, groups):
+        self.od = None
+        if od is None or not len(od.comp):
+            return
+        shape = self.shape
+        nflat = 3 * self.N
+        idx3 = np.asarray(od.idx3, dtype=np.int64)
+        comp = np.asarray(od.comp, dtype=np.int64)
+        coef = np.asarray(od.coef, dtype=float)
+        eye = np.eye(3, dtype=np.int64)
+        tc, ti, sc, si, fwd, bwd = [], [], [], [], [], []
+        for d in range(3):
+            for corner in range(4):
+                q = idx3 + eye[comp] * (corner & 1)
+                q[:, d] -= corner >> 1
+                phase = np.ones(len(q), dtype=complex)
+                ok = (comp != d) & (coef[:, d] != 0)
+                for a in range(3):
+                    n = shape[a]
+                    if self.periodic[a]:
+                        kl = self.bloch[a] * (n - 2) * self.spacing[a]
+                        lo, hi = q[:, a] == 0, q[:, a] == n - 1
+                        q[lo, a], q[hi, a] = n - 2, 1
+                        phase[lo] *= np.exp(-1j * kl)
+                        phase[hi] *= np.exp(1j * kl)
+                    else:
+                        rd = e_region(d, shape)[a]
+                        ok &= (q[:, a] < rd.stop) & (q[:, a] >= rd.start)
+                tc.append(comp[ok])
+                ti.append(idx3[ok])
+                sc.append(np.full(int(ok.sum()), d))
+                si.append(q[ok])
+                fwd.append(coef[ok, d] / 8 * phase[ok])
+                bwd.append(np.conj(coef[ok, d] * phase[ok]) / 8)
+        tc, ti, sc, si = (np.concatenate(x) for x in (tc, ti, sc, si))
+        v = np.concatenate([np.concatenate(fwd), np.concatenate(bwd)])
+        tc, sc = np.concatenate([tc, sc]), np.concatenate([sc, tc])
+        ti, si = np.concatenate([ti, si]), np.concatenate([si, ti])
+        t = flat_index(tc, ti[:, 0], ti[:, 1], ti[:, 2], shape)
+        s_ = flat_index(sc, si[:, 0], si[:, 1], si[:, 2], shape)
+        if groups:
+            disp = np.concatenate([np.asarray(g.idx) for g in groups])
+            keep = ~(np.isin(t, disp) | np.isin(s_, disp))
+            t, s_, v = t[keep], s_[keep], v[keep]
+        if not len(t):
+            return
+        key, inv = np.unique(t * nflat + s_, return_inverse=True)
+        v = np.bincount(inv, v.real, len(key)) + 1j * np.bincount(inv, v.imag, len(key))
+        t, s_ = key // nflat, key % nflat
+        targets, start, count = np.unique(t, return_index=True, return_counts=True)
+        slot = np.arange(len(t)) - np.repeat(start, count)
+        row = np.repeat(np.arange(len(targets)), count)
+        width = int(count.max())
+        hh = np.zeros((len(targets), 4 * width), dtype=np.int64)
+        ww = np.zeros((len(targets), 4 * width), dtype=complex)
+        dd, rest = np.divmod(s_, self.N)
+        q = np.stack(np.unravel_index(rest, shape), axis=1)
+        d1, d2 = (dd + 1) % 3, (dd + 2) % 3
+        inv_d = 1.0 / np.asarray(self.spacing)
+        # curl_d H at q = (H_d2[q] - H_d2[q - e_d1]) / dx_d1 - (H_d1[q] - H_d1[q - e_d2]) / dx_d2
+        for k, (cmp, back, sgn, ax) in enumerate(((d2, None, 1.0, d1), (d2, d1, -1.0, d1),
+                                                  (d1, None, -1.0, d2), (d1, d2, 1.0, d2))):
+            p = q if back is None else q - eye[back]
+            hh[row, 4 * slot + k] = flat_index(cmp, p[:, 0], p[:, 1], p[:, 2], shape)
+            ww[row, 4 * slot + k] = self.dt * sgn * inv_d[ax] * v
+        self.od = (targets, hh, ww)
+        tgt = self.native_index(targets)
+        src = self.native_index(hh.reshape(-1))
+        self._od = [self.to_device(tgt), self.to_device(src), self.field_values(ww.reshape(-1))]
+        check(self.lib.lsf_add_gather(self._h, PH_OFFDIAG, 0, len(tgt), 4 * width, ptr(self._od[0]),
+                                      ptr(self.H_flat), ptr(self._od[1]), ptr(self._od[2]), None, None),
+              "lsf_add_gather (off-diagonal)")
+
+    # --- sparse work used by sources ---------------------------------------------------------------
+    def add_gather(self, phase: int, field_: str, targets, src_tensor, src_idx, weights):
+        """Rows F[target] += sum_k w[k] * src[src_idx[k]]; duplicate targets are merged into one row."""
+        targets = np.asarray(targets, dtype=np.int64)
+        src_idx = np.asarray(src_idx, dtype=np.int64).reshape(len(targets), -1)
+        weights = np.asarray(weights).reshape(len(targets), -1)
+        uniq, inv, counts = np.unique(targets, return_inverse=True, return_counts=True)
+        taps = src_idx.shape[1]
+        width = int(counts.max()) * taps
+        order = np.argsort(inv, kind="stable")
+        slot = np.arange(len(targets)) - np.repeat(np.cumsum(counts) - counts, counts)
+        S = np.zeros((len(uniq), width), dtype=np.int64)
+        W = np.zeros((len(uniq), width), dtype=weights.dtype)
+        rows = inv[order]
+        for t in range(taps):
+            S[rows, slot * taps + t] = src_idx[order, t]
+            W[rows, slot * taps + t] = weights[order, t]
+        fld = 0 if field_ == "E" else 1
+        self.check_index(uniq, 3 * self.N, "gather target")
+        self.check_index(S, src_tensor.numel(), "gather source")
+        ai, asc = self.ade_redirect(uniq) if fld == 0 else (None, None)
+        arrs = [self.to_device(uniq), self.to_device(S.reshape(-1)), self.field_values(W.reshape(-1))]
+        check(self.lib.lsf_add_gather(self._h, phase, fld, len(uniq), width, ptr(arrs[0]), ptr(src_tensor),
+                                      ptr(arrs[1]), ptr(arrs[2]), ptr(ai), ptr(asc)), "lsf_add_gather")
+
+    def add_modal(self, phase: int, field_: str, targets, m, table):
+        """F[target[r]] += sum_q m[r, q] table[n, q] (targets must be distinct)."""
+        targets = np.asarray(targets, dtype=np.int64)
+        if len(np.unique(targets)) != len(targets):
+            raise ValueError("modal source targets must be distinct")
+        m = np.asarray(m).reshape(len(targets), -1)
+        table = np.asarray(table, dtype=np.float64).reshape(-1, m.shape[1])
+        self.check_index(targets, 3 * self.N, "modal target")
+        fld = 0 if field_ == "E" else 1
+        ai, asc = self.ade_redirect(targets) if fld == 0 else (None, None)
+        arrs = [self.to_device(targets), self.field_values(m.reshape(-1)), self.to_device(table.astype(np.float32))]
+        check(self.lib.lsf_add_modal(self._h, phase, fld, len(targets), m.shape[1], ptr(arrs[0]), ptr(arrs[1]),
+                                     ptr(arrs[2]), table.shape[0], ptr(ai), ptr(asc)), "lsf_add_modal")
+
+    def add_modal_to(self, phase: int, field_: str, targets, m, table):
+        self.add_modal(phase, field_, targets, m, table)
+
+    def add_table(self, phase: int, field_: str, nodes, T, m: int, cmax: int, base, i0, u0, coef):
+        """Table-fed corrections, points grouped by target node (CSR)."""
+        nodes = np.asarray(nodes, dtype=np.int64)
+        order = np.argsort(nodes, kind="stable")
+        uniq, counts = np.unique(nodes[order], return_counts=True)
+        row_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+        self.check_index(uniq, 3 * self.N, "table target")
+        base, i0 = np.asarray(base, np.int64), np.asarray(i0, np.int64)
+        if base.size and (base.min() < 0 or i0.min() < 0 or base.max() + int(cmax) + 6 > T.numel()):
+            raise ValueError("table source: reads outside the tables")
+        fld = 0 if field_ == "E" else 1
+        ai, asc = self.ade_redirect(uniq) if fld == 0 else (None, None)
+        arrs = [self.to_device(uniq), self.to_device(row_ptr), T,
+                self.to_device(np.asarray(base, np.int64)[order]), self.to_device(np.asarray(i0, np.int64)[order]),
+                self.to_device(np.asarray(u0, np.float32)[order]), self.to_device(np.asarray(coef, np.float32)[order])]
+        check(self.lib.lsf_add_table(self._h, phase, fld, len(uniq), ptr(arrs[0]), ptr(arrs[1]), ptr(arrs[2]), int(m),
+                                     int(cmax), ptr(arrs[3]), ptr(arrs[4]), ptr(arrs[5]), ptr(arrs[6]), ptr(ai),
+                                     ptr(asc)), "lsf_add_table")
+
+    # --- monitors (engine.py:556-592) ----------------------------------------------------------
+    def _bind_monitor(self, m: DFTMonitor):
+        omega = np.asarray(m.omega, dtype=float)
+        src = m.source
+        if src is None:
+            e_idx = self.native_index(m.e_idx)
+            h1 = None if m.h_idx1 is None else self.native_index(m.h_idx1)
+            h2 = None if m.h_idx2 is None else self.native_index(m.h_idx2)
+            esrc = hsrc = None
+            cx = int(self.complex)
+            n_e = n_h = 3 * self.N
+        else:
+            esrc, hsrc, cx, remap = src.monitor_arrays()
+            e_idx = remap(m.e_idx)
+            h1 = None if m.h_idx1 is None else remap(m.h_idx1)
+            h2 = None if m.h_idx2 is None else remap(m.h_idx2)
+            n_e, n_h = esrc.numel(), hsrc.numel()
+        self.check_index(e_idx, n_e, "monitor E index")
+        if h1 is not None:
+            self.check_index(h1, n_h, "monitor H index")
+            self.check_index(h2, n_h, "monitor H index")
+        m._omega = self.to_device(omega, torch.float64)
+        m._e = self.to_device(e_idx)
+        m.acc_e = torch.zeros((len(omega), len(e_idx)), dtype=torch.complex64, device=self._dev)
+        m.acc_h = None
+        if h1 is not None:
+            m._h1, m._h2 = self.to_device(h1), self.to_device(h2)
+            m.acc_h = torch.zeros((len(omega), len(h1)), dtype=torch.complex64, device=self._dev)
+        else:
+            m._h1 = m._h2 = None
+        check(self.lib.lsf_add_dft_monitor(self._h, len(omega), len(e_idx), ptr(m._omega), ptr(m._e), ptr(m._h1),
+                                           ptr(m._h2), ptr(m.acc_e), ptr(m.acc_h), ptr(esrc), ptr(hsrc), int(cx),
+                                           int(m.energy) if m.acc_h is not None else 0), "lsf_add_dft_monitor")
+
+    def _bind_time_monitor(self, m: TimeMonitor):
+        e_idx = self.native_index(m.e_idx)
+        self.check_index(e_idx, 3 * self.N, "time monitor index")
+        m._e = self.to_device(e_idx)
+        m.buf = torch.zeros((int(m.n_max), len(e_idx)), dtype=self.fdt, device=self._dev)
+        m.pos = torch.zeros(1, dtype=torch.int64, device=self._dev)
+        check(self.lib.lsf_add_time_monitor(self._h, len(e_idx), ptr(m._e), ptr(m.buf), int(m.n_max), ptr(m.pos)))
+
+    # --- stepping (engine.py:665-719) ------------------------------------------------------------
+    def step(self):
+        check(self.lib.lsf_step(self._h, 1, self._stream()), "lsf_step")
+        self.n_host += 1
+
+    def steps(self, n: int):
+        check(self.lib.lsf_step(self._h, int(n), self._stream()), "lsf_step")
+        self.n_host += int(n)
+
+    def sample(self):
+        check(self.lib.lsf_sample(self._h, self._stream()), "lsf_sample")
+
+    def chunk(self):
+        check(self.lib.lsf_chunk(self._h, 1, self.stride, self._stream()), "lsf_chunk")
+        self.n_host += self.stride
+
+    def make_runner(self, use_graphs: bool):
+        """(run_one_chunk, used_graphs): with graphs, one chunk is captured as a HIP graph on a side stream
+        and replayed (engine.py:685-709; the warm-up chunk runs eagerly)."""
+        if not use_graphs:
+            return self.chunk, False
+        stream = torch.cuda.Stream(device=self._dev)
+        stream.wait_stream(torch.cuda.current_stream(self._dev))
+        state = {"captured": False}
+        self._graph_stream = stream
+
+        def run():
+            if not state["captured"]:
+                self.chunk()  # the warm-up chunk: real steps, eager
+                torch.cuda.current_stream(self._dev).synchronize()
+                check(self.lib.lsf_graph_capture(self._h, self.stride, ctypes.c_void_p(stream.cuda_stream)),
+                      "lsf_graph_capture")
+                state["captured"] = True
+                self._graphed = True
+                return
+            stream.wait_stream(torch.cuda.current_stream(self._dev))
+            check(self.lib.lsf_graph_replay(self._h, 1, ctypes.c_void_p(stream.cuda_stream)), "lsf_graph_replay")
+            torch.cuda.current_stream(self._dev).wait_stream(stream)
+            self.n_host += self.stride
+
+        return run, True
+
+    def release(self):
+        """Drop a captured graph once the run is over (engine.py:711-719)."""
+        if self._graphed:
+            check(self.lib.lsf_graph_reset(self._h), "lsf_graph_reset")
+            self._graphed = False
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            torch.cuda.synchronize(self._dev)
+            self.lib.lsf_destroy(h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 (interpreter shutdown)
+            pass
+
+    # --- diagnostics -------------------------------------------------------------------------------
+    def bytes_per_step(self) -> dict:
+        """Algorithmic HBM bytes of one step: fields read + written, 3 float32 coefficients per cell,
+        psi read + written, D-form state (D, E_prev, P^n, P^{n-1} per pole) read + written."""
+        s = 8 if self.complex else 4
+        n_ade = self.n_ade
+        p = getattr(self, "ade_max_poles", 0)
+        fields = self.N * (12 * s + 12)
+        psi = self.n_psi * 2 * s
+        ade = n_ade * (2 + 2 * p) * 2 * s
+        return {"fields": fields, "psi": psi, "ade": ade, "total": fields + psi + ade}

@@ -1,0 +1,148 @@
+"""Matched configurations for the oracle (FP64 NumPy) and the HIP engine.
+
+Each builder takes a namespace ``ns`` -- the oracle modules or the product package -- so that the very
+same inputs (material arrays, index tables, source tables) reach both sides.
+"""
+
+from __future__ import annotations
+
+import types
+
+import numpy as np
+
+
+def oracle_ns():
+    from oracle import fdtd, media, sources
+    return types.SimpleNamespace(Engine=fdtd.Engine, PMLSpec=fdtd.PMLSpec, DFTMonitor=fdtd.DFTMonitor,
+                                 TimeMonitor=fdtd.TimeMonitor, DispersiveGroup=fdtd.DispersiveGroup,
+                                 OffDiagonal=fdtd.OffDiagonal, PointDipole=sources.PointDipole,
+                                 TFSFPlaneWave=sources.TFSFPlaneWave, TableSource=sources.TableSource,
+                                 BlochPlaneSource=sources.BlochPlaneSource, ModeInjection=sources.ModeInjection,
+                                 Medium=media.Medium, Pole=media.Pole, flat_index=fdtd.flat_index,
+                                 e_region=fdtd.e_region, h_region=fdtd.h_region, oracle=True)
+
+
+def product_ns():
+    from lightsmith_amd import fdtd
+    from lightsmith_amd.fdtd import sources
+    return types.SimpleNamespace(Engine=fdtd.Engine, PMLSpec=fdtd.PMLSpec, DFTMonitor=fdtd.DFTMonitor,
+                                 TimeMonitor=fdtd.TimeMonitor, DispersiveGroup=fdtd.DispersiveGroup,
+                                 OffDiagonal=fdtd.OffDiagonal, PointDipole=sources.PointDipole,
+                                 TFSFPlaneWave=sources.TFSFPlaneWave, TableSource=sources.TableSource,
+                                 BlochPlaneSource=sources.BlochPlaneSource, ModeInjection=sources.ModeInjection,
+                                 Medium=fdtd.Medium, Pole=fdtd.Pole, flat_index=fdtd.flat_index,
+                                 e_region=fdtd.e_region, h_region=fdtd.h_region, oracle=False)
+
+
+def make(ns, device=None, **kw):
+    """Engine of namespace ns; the oracle takes no device, the HIP engine runs float32 on `device`."""
+    if ns.oracle:
+        kw.pop("single", None)
+        return ns.Engine(**kw)
+    kw.setdefault("single", True)
+    return ns.Engine(device, **kw)
+
+
+def fields(eng):
+    """(E, H) as complex128 NumPy arrays in the reference layout (3, N0, N1, N2)."""
+    E, H = eng.E, eng.H
+    if hasattr(E, "cpu"):
+        E, H = E.cpu().numpy(), H.cpu().numpy()
+    return np.asarray(E, dtype=complex), np.asarray(H, dtype=complex)
+
+
+def rel_l2(a, b, mask=None) -> float:
+    a, b = np.asarray(a), np.asarray(b)
+    if mask is not None:
+        a, b = a[mask], b[mask]
+    den = np.linalg.norm(b.ravel())
+    return float(np.linalg.norm((a - b).ravel()) / (den if den > 0 else 1.0))
+
+
+def real_mask(shape, periodic):
+    """(3, N0, N1, N2) mask of real (non-ghost) cells: ghost planes are not materialised by the HIP engine."""
+    m = np.ones((3,) + tuple(shape), bool)
+    for a, p in enumerate(periodic):
+        if p:
+            sl = [slice(None)] * 4
+            sl[a + 1] = 0
+            m[tuple(sl)] = False
+            sl[a + 1] = shape[a] - 1
+            m[tuple(sl)] = False
+    return m
+
+
+def random_fields(eng, seed: int, periodic=(False, False, False)):
+    """Seed E and H with random values on their stepped regions (same values on both sides)."""
+    rng = np.random.default_rng(seed)
+    shape = eng.shape
+    for c in range(3):
+        for F, R in ((eng.E, eng.RE[c]), (eng.H, eng.RH[c])):
+            sub = tuple(R)
+            vals = rng.standard_normal(tuple(r.stop - r.start for r in sub))
+            if hasattr(F, "cpu"):
+                import torch
+                F[c][sub] = torch.as_tensor(vals, dtype=F.dtype, device=F.device)
+            else:
+                F[c][sub] = vals
+    return shape
+
+
+# --- configurations -------------------------------------------------------------------------------
+
+def lorentz3(rng):
+    """A 3-pole Lorentz medium with seeded poles (omega_p, omega_0 in [1, 10] eV, gamma in [0.05, 0.5] eV)."""
+    from oracle.media import HBARC_EV_NM
+    poles = []
+    for _ in range(3):
+        wp, w0, g = rng.uniform(1, 10), rng.uniform(1, 10), rng.uniform(0.05, 0.5)
+        poles.append(((wp / HBARC_EV_NM) ** 2, w0 / HBARC_EV_NM, g / HBARC_EV_NM))
+    return poles
+
+
+def medium(ns, eps_inf, poles):
+    return ns.Medium(eps_inf, tuple(ns.Pole(*p) for p in poles))
+
+
+def dispersive_box(ns, device=None, shape=(40, 36, 32), dx=2.0, seed=0, pml_cells=6, n_steps_hint=400,
+                   with_dipole=True, frac=0.1, kappa_max=1.0, alpha=0.0, monitors=True, material="auto"):
+    """CPML box with random dielectric values, a fraction of D-form 3-pole Lorentz nodes, a z dipole and a
+    DFT plane: a small version of the 320^3 throughput configuration."""
+    rng = np.random.default_rng(seed)
+    eps = rng.choice([1.0, 2.25, 4.0, 12.0], size=shape)
+    inv_eps = np.broadcast_to(1.0 / eps, (3,) + shape).copy()
+    poles = lorentz3(np.random.default_rng(seed + 100))
+    med = medium(ns, 1.0, poles)
+    pml = tuple((pml_cells, pml_cells) for _ in range(3))
+    inner = np.zeros(shape, bool)
+    inner[pml_cells:-pml_cells, pml_cells:-pml_cells, pml_cells:-pml_cells] = True
+    disp = inner & (rng.random(shape) < frac)
+    ii, jj, kk = np.nonzero(disp)
+    idx = np.concatenate([ns.flat_index(c, ii, jj, kk, shape) for c in range(3)])
+    inv_eps.reshape(3, -1)[:, np.ravel_multi_index((ii, jj, kk), shape)] = 1.0
+    groups = [ns.DispersiveGroup(idx=idx, media=[med], frac=np.ones((1, len(idx))), w=None)] if len(idx) else []
+    dt = 0.95 * dx / np.sqrt(3)
+    sources = []
+    if with_dipole:
+        ctr = np.asarray(shape) * dx / 2 + 0.3
+        sources.append(ns.PointDipole(shape=shape, origin=np.zeros(3), dx=dx, dt=dt, pos=ctr, pdir=(0.2, 0.1, 1.0),
+                                      inv_eps=inv_eps, omega_c=2 * np.pi / 60, omega_half=0.15 * 2 * np.pi / 60,
+                                      max_steps=n_steps_hint + 10))
+    mons = []
+    if monitors:
+        kz = shape[2] // 2 + 3
+        i = np.arange(pml_cells, shape[0] - pml_cells)
+        j = np.arange(pml_cells, shape[1] - pml_cells)
+        gi, gj = np.meshgrid(i, j, indexing="ij")
+        gi, gj = gi.ravel(), gj.ravel()
+        e = np.concatenate([ns.flat_index(0, gi, gj, kz, shape), ns.flat_index(1, gi, gj, kz, shape)])
+        h1 = np.concatenate([ns.flat_index(1, gi, gj, kz - 1, shape), ns.flat_index(0, gi, gj, kz - 1, shape)])
+        h2 = np.concatenate([ns.flat_index(1, gi, gj, kz, shape), ns.flat_index(0, gi, gj, kz, shape)])
+        omega = 2 * np.pi / np.linspace(50, 80, 5)
+        mons = [ns.DFTMonitor(omega, e, h1, h2, energy=1)]
+    spec = ns.PMLSpec(cells=pml, kappa_max=kappa_max, alpha_max=alpha, eps_ref=1.0)
+    kw = dict(shape=shape, dx=dx, dt=dt, inv_eps=inv_eps, groups=groups, pml=spec, sources=sources, monitors=mons,
+              dft_stride=10)
+    if not ns.oracle:
+        kw["material"] = material
+    return make(ns, device, **kw)
