@@ -1,0 +1,209 @@
+#!/usr/bin/env python
+"""Throughput benchmark of the FDTD time-stepping hot path on one MI355X.
+
+Workload: the plain-FDTD throughput configuration (lightsmith_amd/fdtd/workloads.py): 320^3 cells, 16-cell
+CPML, 200 seeded dielectric spheres, 10 % of the non-CPML cells D-form 3-pole Lorentz ADE, a point dipole,
+a 16-frequency DFT plane; float32 fields. A step is one FDTD time step over the whole grid (H update, E
+update with CPML and ADE, dipole, and one DFT sample every dft_stride steps), replayed as HIP graphs of one
+DFT chunk each, inputs resident in HBM.
+
+Output: one JSON line on stdout. value = cells x steps / wall time of the timed steps (Gcell-updates/s).
+roofline: the bulk update (H kernel + E kernel) against the HBM roofline: algorithmic bytes per step
+(fields, coefficients, CPML psi, ADE state, each read and written once) / the two kernels' mean durations,
+measured with HIP events around each launch on the launch stream over a second pass of the same steps.
+cpu_baseline: the FP64 NumPy oracle (one thread) timed on a bounded sample of the same scene.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s + % HBM roofline; batched and adjoint vs plain GPU FDTD"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_throughput_320.json")
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(eng, scene) -> dict:
+    """Bytes per step that the hot path must move at minimum (field, coefficient, psi, ADE terms of
+    the per-step formula; plus the dipole's and the DFT plane's share)."""
+    b = eng.bytes_per_step()
+    s = 8 if eng.complex else 4
+    n_src = 2 * 1  # dipole nodes: target read + write, one table tap
+    src = n_src * (4 + 2 * s)
+    e_idx = scene["monitor"][0]
+    mon = len(scene["omega"]) * len(e_idx) * 2 * 2 * 8 / eng.stride  # acc_e + acc_h complex64 r/w per sample
+    b["sources"] = src
+    b["monitors"] = mon
+    b["step_total"] = b["total"] + src + mon
+    return b
+
+
+def cpu_baseline(scene, steps: int) -> dict:
+    """The FP64 NumPy oracle on the same scene, one thread, `steps` timed steps after one warm-up step."""
+    from oracle import fdtd as ofd
+    from oracle import media as omed
+    from oracle import sources as osrc
+
+    m = scene["medium"]
+    med = omed.Medium(m.eps_inf, tuple(omed.Pole(p.sigma, p.omega0, p.gamma) for p in m.poles))
+    idx = scene["disp_idx"]
+    groups = [ofd.DispersiveGroup(idx=idx, media=[med], frac=np.ones((1, len(idx))), w=None)]
+    src = osrc.PointDipole(shape=scene["shape"], origin=np.zeros(3), dx=scene["dx"], dt=scene["dt"],
+                           pos=scene["dipole_pos"], pdir=scene["dipole_dir"], inv_eps=scene["inv_eps"],
+                           omega_c=scene["omega_c"], omega_half=scene["omega_half"], max_steps=steps + 10)
+    mon = ofd.DFTMonitor(scene["omega"], *scene["monitor"], energy=1)
+    p = scene["pml"]
+    pml = ofd.PMLSpec(cells=((p, p),) * 3, alpha_max=0.05 * float(np.min(scene["omega"])), eps_ref=1.0)
+    eng = ofd.Engine(shape=scene["shape"], dx=scene["dx"], dt=scene["dt"], inv_eps=scene["inv_eps"], groups=groups,
+                     pml=pml, sources=[src], monitors=[mon], dft_stride=scene["stride"])
+    eng.step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.step()
+    dt = time.perf_counter() - t0
+    return {"value": scene["n_cells"] * steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps of the same {scene['shape'][0]}^3 scene after 1 warm-up step; FP64 NumPy "
+                      "oracle (elementwise NumPy, one thread); DFT samples excluded (none fall in the sample)",
+            "seconds": dt}
+
+
+def read_traffic():
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        return json.load(f)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=320, help="grid size per axis (the workload is 320)")
+    ap.add_argument("--cpu-steps", type=int, default=4, help="timed steps of the CPU baseline (0: skip)")
+    ap.add_argument("--tiling", default=None, help="rows_per_lane,waves,z_chunk")
+    ap.add_argument("--no-roofline-pass", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 1 GiB (for scaling rocprof FETCH/WRITE counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (one process, one MI355X); --gpus above 1 is not "
+              "supported", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from lightsmith_amd import _lib
+    from lightsmith_amd.fdtd import workloads
+
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        return 1
+    dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 5 copies of 1 GiB")
+        return 0
+
+    t_setup = time.perf_counter()
+    scene = workloads.throughput_scene(n=args.n)
+    stride = scene["stride"]
+    K = workloads.steps_for(args.steps, stride)
+    W = workloads.steps_for(max(args.warmup, stride), stride)
+    tiling = tuple(int(v) for v in args.tiling.split(",")) if args.tiling else None
+    eng = workloads.throughput_engine(scene, dev, max_steps=W + 2 * K + 4 * stride, tiling=tiling)
+    torch.cuda.synchronize()
+    log(f"set-up {time.perf_counter() - t_setup:.1f} s: {scene['shape']} cells, dt {scene['dt']:.4f} nm/c, "
+        f"stride {stride}, {eng.n_ade} D-form nodes, {eng.n_psi} psi values, material {eng.material_mode}")
+
+    run, used = eng.make_runner(True)
+    for _ in range(W // stride):
+        run()
+    torch.cuda.synchronize()
+    log(f"warm-up {W} steps done")
+
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(K // stride):
+        run()
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_ev = start.elapsed_time(stop)
+    eng.release()
+    elapsed = max(wall, ms_ev / 1e3)
+    value = scene["n_cells"] * K / elapsed / 1e9
+    log(f"timed {K} steps: {elapsed * 1e3:.1f} ms ({value:.2f} Gcell-updates/s)")
+    energy = float(eng.energy[0])
+    if not np.isfinite(energy):
+        print("bench.py: fields diverged", file=sys.stderr)
+        return 1
+
+    bytes_ = algorithmic_bytes(eng, scene)
+    roofline = None
+    if not args.no_roofline_pass:
+        ms = (ctypes.c_double * 2)(0.0, 0.0)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        for _ in range(K // stride):
+            _lib.check(eng.lib.lsf_step_timed(eng._h, stride, stream, ms), "lsf_step_timed")
+            _lib.check(eng.lib.lsf_sample(eng._h, stream), "lsf_sample")
+        torch.cuda.synchronize()
+        h_ms, e_ms = ms[0] / K, ms[1] / K
+        kern_s = (h_ms + e_ms) / 1e3
+        achieved = bytes_["total"] / kern_s / 1e9
+        traffic = read_traffic()
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": None if traffic is None else traffic.get("bytes_per_step"),
+                    "kernel": "k_update_h + k_update_e (one time step's bulk update)",
+                    "algorithmic_bytes_per_step": int(bytes_["total"]),
+                    "kernel_us": {"h": round(h_ms * 1e3, 2), "e": round(e_ms * 1e3, 2)},
+                    "step_achieved_GBs": round(bytes_["step_total"] / elapsed * K / 1e9, 1)}
+        log(f"kernels: H {h_ms * 1e3:.1f} us, E {e_ms * 1e3:.1f} us per step; {achieved:.0f} GB/s algorithmic "
+            f"({achieved / HBM_PEAK_GBS:.1%} of {HBM_PEAK_GBS:.0f})")
+
+    cpu = None
+    if args.cpu_steps > 0:
+        log(f"CPU baseline: {args.cpu_steps} steps of the FP64 oracle ...")
+        cpu = cpu_baseline(scene, args.cpu_steps)
+        log(f"CPU baseline {cpu['value'] * 1e3:.2f} Mcell-updates/s ({cpu['seconds']:.1f} s)")
+
+    out = {"metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": K,
+           "warmup": W, "ms_per_step": round(elapsed * 1e3 / K, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"plain 3D FDTD throughput: {args.n}^3 cells, 16-cell CPML, 200 seeded spheres "
+                                  "(eps 1-12), 10% of non-CPML cells 3-pole Lorentz ADE, z point dipole at 1550 nm, "
+                                  "16-wavelength DFT plane, float32 fields, HIP graphs per DFT chunk",
+                      "cells": scene["n_cells"], "dx_nm": scene["dx"], "dt_nm_per_c": scene["dt"],
+                      "dft_stride": stride, "dispersive_nodes": int(eng.n_ade), "psi_values": int(eng.n_psi),
+                      "material": eng.material_mode, "tiling": list(eng.tiling if tiling is None else tiling),
+                      "parallelism": "single GPU"},
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

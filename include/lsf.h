@@ -51,7 +51,7 @@ typedef struct lsf_grid {
   int32_t tick;              /* 1: the E kernel advances dev_counter (the top-level engine); 0: a sub-engine */
 } lsf_grid;
 
-/* Create / destroy an engine handle (engine.py:230-315; release engine.py:711-719). */
+/* Create / destroy an engine handle (engine.py:230-315; release engine.py:711-719). 3 * cells < 2^31. */
 int lsf_create(const lsf_grid* grid, lsf_engine** out);
 void lsf_destroy(lsf_engine* eng);
 
@@ -73,11 +73,12 @@ int lsf_set_pml(lsf_engine* eng, int field, int axis, const void* dev_profile, i
 
 /* Pure ADE groups in D-form (engine.py:722-742, 759-793). n_nodes D-form nodes in native flat order;
  * dev_chunk_base[(c*N2 + k)*N1 + j][x/64] = index of the chunk's first D-form node.
- * dev_D[n_nodes]; dev_P[max_poles][n_nodes] pairs {P^n, P^{n-1}} (field type x2).
- * dev_media[n_media][2 + 3*max_poles] float = {1/eps_inf, n_poles, (c1, c2, c3) per pole}
- * (pole_coefficients, engine.py:108-111). */
-int lsf_set_ade(lsf_engine* eng, const int32_t* dev_chunk_base, void* dev_D, void* dev_P, int64_t n_nodes,
-                int max_poles, const float* dev_media, int n_media);
+ * 1 <= max_poles <= 4; dev_P[max_poles][n_nodes] pairs {P^n, P^{n-1}} (field type x2), zeroed by the caller.
+ * D is not stored: D^n = eps_inf E^n + sum P^n holds after every step, so the update advances E by
+ * (dE - sum (P^{n+1} - P^n)) / eps_inf. dev_media[n_media][2 + 3*max_poles] float =
+ * {1/eps_inf, n_poles, (c1, c2, c3) per pole} (pole_coefficients, engine.py:108-111); n_media <= 16. */
+int lsf_set_ade(lsf_engine* eng, const int32_t* dev_chunk_base, void* dev_P, int64_t n_nodes, int max_poles,
+                const float* dev_media, int n_media);
 
 /* Phases of a step at which sparse work runs, in this order (engine.py:665-678). */
 #define LSF_PH_CORRECT_H 0  /* after the H update: src.correct_H */
@@ -89,26 +90,23 @@ int lsf_set_ade(lsf_engine* eng, const int32_t* dev_chunk_base, void* dev_D, voi
 
 /* Gather-add rows: F[target[r]] += sum_k w[r, k] * src[src_idx[r, k]] (field 0 = E, 1 = H).
  * Replaces the off-diagonal rows (engine.py:653-655), the 1D-fed TFSF corrections (tfsf.py:195-204)
- * and the Bloch plane corrections (tfsf.py:274-286). dev_ade_idx (or NULL) redirects E targets that are
- * D-form nodes: D[a] += v, E += v * dev_ade_scale[r] (the correction enters before the group update). */
+ * and the Bloch plane corrections (tfsf.py:274-286). dev_scale (or NULL = 1) scales each row's sum: 1/eps_inf
+ * for a pure D-form E node, where the reference's correction enters through D before the group update. */
 int lsf_add_gather(lsf_engine* eng, int phase, int field, int64_t n_rows, int width, const int64_t* dev_target,
-                   const void* dev_src, const int64_t* dev_src_idx, const void* dev_w, const int32_t* dev_ade_idx,
-                   const float* dev_ade_scale);
+                   const void* dev_src, const int64_t* dev_src_idx, const void* dev_w, const float* dev_scale);
 
 /* Modal injection: F[target[r]] += sum_q m[r, q] * table[t * Q + q], t = the step counter, 0 past table_len.
  * Replaces PointDipole.correct_E (emission.py:139-140), ModeSource.correct_H/E (ports.py:368-372) and the
  * Bloch column's current sheet (tfsf.py:291-296). dev_m has the field type; dev_table is float. */
 int lsf_add_modal(lsf_engine* eng, int phase, int field, int64_t n_rows, int n_modes, const int64_t* dev_target,
-                  const void* dev_m, const float* dev_table, int64_t table_len, const int32_t* dev_ade_idx,
-                  const float* dev_ade_scale);
+                  const void* dev_m, const float* dev_table, int64_t table_len, const float* dev_scale);
 
 /* Table-fed TFSF corrections (layered.py:169-187, 231-242): for target row r, sum over its points p in
  * [row_ptr[r], row_ptr[r+1]) of coef[p] * quintic(T, t + i0[p], u0[p], base[p]) with tables sampled every
  * m steps, read index clamped to cmax. */
 int lsf_add_table(lsf_engine* eng, int phase, int field, int64_t n_rows, const int64_t* dev_target,
                   const int32_t* dev_row_ptr, const float* dev_T, int m, int64_t cmax, const int64_t* dev_base,
-                  const int64_t* dev_i0, const float* dev_u0, const float* dev_coef, const int32_t* dev_ade_idx,
-                  const float* dev_ade_scale);
+                  const int64_t* dev_i0, const float* dev_u0, const float* dev_coef, const float* dev_scale);
 
 /* Incident-wave view of a table source (layered.py:481-482): out[r] = quintic(T, t + i0[r], ...), r < n. */
 int lsf_add_table_view(lsf_engine* eng, int phase, int n, float* dev_out, const float* dev_T, int m, int64_t cmax,
@@ -149,11 +147,14 @@ int lsf_add_time_monitor(lsf_engine* eng, int64_t n_pts, const int64_t* dev_e_id
 /* Energy proxy slots (engine.py:315 "energy"), float[2] (or a field-precision pair). */
 int lsf_set_energy(lsf_engine* eng, float* dev_energy);
 
-/* Launch tuning: rows per lane (1, 2 or 4), waves per block (1..16), z planes per block (>= 1). */
+/* Launch tuning: rows per lane (1 or 2), waves per block (4 or 8), z planes per block (>= 1). */
 int lsf_set_tiling(lsf_engine* eng, int rows_per_lane, int waves, int z_chunk);
 
 /* Run n_steps steps (Engine.step, engine.py:665-678) on `stream` (a hipStream_t; NULL = default). */
 int lsf_step(lsf_engine* eng, int n_steps, void* stream);
+/* As lsf_step, bracketing each H and E kernel with HIP events on `stream`; adds the summed kernel times (ms)
+ * to ms[0] (H kernels) and ms[1] (E kernels). Synchronises the stream at the end. For measurement. */
+int lsf_step_timed(lsf_engine* eng, int n_steps, void* stream, double* ms);
 /* One DFT/time-monitor sample (Engine._sample, engine.py:568-592). */
 int lsf_sample(lsf_engine* eng, void* stream);
 /* n_chunks x (dft_stride steps + one sample) (Engine.chunk, engine.py:680-683). */

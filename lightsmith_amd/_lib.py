@@ -33,10 +33,10 @@ _SIGNATURES = {
     "lsf_set_material_dense": [c_vp, c_vp],
     "lsf_set_material_indexed": [c_vp, c_vp, c_vp],
     "lsf_set_pml": [c_vp, c_int, c_int, c_vp, c_int, c_vp, c_vp, c_vp, c_int],
-    "lsf_set_ade": [c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_int],
-    "lsf_add_gather": [c_vp, c_int, c_int, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
-    "lsf_add_modal": [c_vp, c_int, c_int, c_i64, c_int, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp],
-    "lsf_add_table": [c_vp, c_int, c_int, c_i64, c_vp, c_vp, c_vp, c_int, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "lsf_set_ade": [c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_int],
+    "lsf_add_gather": [c_vp, c_int, c_int, c_i64, c_int, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "lsf_add_modal": [c_vp, c_int, c_int, c_i64, c_int, c_vp, c_vp, c_vp, c_i64, c_vp],
+    "lsf_add_table": [c_vp, c_int, c_int, c_i64, c_vp, c_vp, c_vp, c_int, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp],
     "lsf_add_table_view": [c_vp, c_int, c_int, c_vp, c_vp, c_int, c_i64, c_vp, c_vp, c_vp],
     "lsf_add_line1d": [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_i64],
     "lsf_add_subengine": [c_vp, c_vp],
@@ -46,6 +46,7 @@ _SIGNATURES = {
     "lsf_set_energy": [c_vp, c_vp],
     "lsf_set_tiling": [c_vp, c_int, c_int, c_int],
     "lsf_step": [c_vp, c_int, c_vp],
+    "lsf_step_timed": [c_vp, c_int, c_vp, ctypes.POINTER(c_dbl)],
     "lsf_sample": [c_vp, c_vp],
     "lsf_chunk": [c_vp, c_int, c_int, c_vp],
     "lsf_graph_capture": [c_vp, c_int, c_vp],

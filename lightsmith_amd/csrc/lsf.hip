@@ -39,8 +39,7 @@ struct Op {
   const void* src = nullptr;
   const long long* sidx = nullptr;
   const void* w = nullptr;
-  const int* ade_idx = nullptr;
-  const float* ade_scale = nullptr;
+  const float* scale = nullptr;
   const float* table = nullptr;
   long long table_len = 0;
   const int* row_ptr = nullptr;
@@ -98,7 +97,6 @@ struct lsf_engine {
   const float* mtab = nullptr;
   lsf::PmlAx pml[2][3]{};  // [field E=0, H=1][axis]
   const int* chunk_base = nullptr;
-  void* D = nullptr;
   void* P = nullptr;
   long long n_ade = 0;
   int max_poles = 0;
@@ -132,13 +130,13 @@ StepArgs make_args(const lsf_engine* e, int field) {
   a.mtab = e->mtab;
   for (int ax = 0; ax < 3; ++ax) a.pml[ax] = e->pml[field][ax];
   a.chunk_base = e->chunk_base;
-  a.D = e->D;
   a.P = e->P;
-  a.n_ade = e->n_ade;
+  a.n_ade = (int)e->n_ade;
   a.nchunk = nchunk_of(e);
   a.max_poles = e->max_poles;
   a.media_stride = 2 + 3 * e->max_poles;
   a.media = e->media;
+  a.n_media_floats = e->n_ade > 0 ? e->n_media * (2 + 3 * e->max_poles) : 0;
   a.counter = e->counter;
   a.tick = e->tick;
   const int rows = e->ry * e->waves;
@@ -151,50 +149,65 @@ StepArgs make_args(const lsf_engine* e, int field) {
   return a;
 }
 
-template <class T, int RY, int W>
+template <class T, int RY, int W, bool PER>
 void launch_h_t(const StepArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((lsf::k_update_h<T, RY, W>), dim3(a.nbp), dim3(64 * W), 0, s, a);
+  hipLaunchKernelGGL((lsf::k_update_h<T, RY, W, PER>), dim3(a.nbp), dim3(64 * W), 0, s, a);
 }
 
-template <class T, int MAT, int RY, int W>
-void launch_e_t(const StepArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((lsf::k_update_e<T, MAT, RY, W>), dim3(a.nbp), dim3(64 * W), 0, s, a);
-}
+bool any_periodic(const lsf_engine* e) { return e->g.per[0] || e->g.per[1] || e->g.per[2]; }
 
-template <class T, int RY>
-void launch_h_w(const StepArgs& a, int w, hipStream_t s) {
-  if (w == 2) launch_h_t<T, RY, 2>(a, s);
-  else if (w == 8) launch_h_t<T, RY, 8>(a, s);
-  else launch_h_t<T, RY, 4>(a, s);
-}
-
-template <class T, int MAT, int RY>
-void launch_e_w(const StepArgs& a, int w, hipStream_t s) {
-  if (w == 2) launch_e_t<T, MAT, RY, 2>(a, s);
-  else if (w == 8) launch_e_t<T, MAT, RY, 8>(a, s);
-  else launch_e_t<T, MAT, RY, 4>(a, s);
+template <class T, bool PER>
+void launch_h_p(const lsf_engine* e, hipStream_t s) {
+  const StepArgs a = make_args(e, 1);
+  const bool w8 = e->waves == 8;
+  if (e->ry == 2) {
+    if (w8) launch_h_t<T, 2, 8, PER>(a, s);
+    else launch_h_t<T, 2, 4, PER>(a, s);
+  } else {
+    if (w8) launch_h_t<T, 1, 8, PER>(a, s);
+    else launch_h_t<T, 1, 4, PER>(a, s);
+  }
 }
 
 template <class T>
 void launch_h(const lsf_engine* e, hipStream_t s) {
-  const StepArgs a = make_args(e, 1);
-  if (e->ry == 1) launch_h_w<T, 1>(a, e->waves, s);
-  else if (e->ry == 4) launch_h_w<T, 4>(a, e->waves, s);
-  else launch_h_w<T, 2>(a, e->waves, s);
+  if (any_periodic(e)) launch_h_p<T, true>(e, s);
+  else launch_h_p<T, false>(e, s);
 }
 
-template <class T, int MAT>
-void launch_e_m(const lsf_engine* e, hipStream_t s) {
+template <class T, int MAT, bool PER, int NP>
+void launch_e_np(const lsf_engine* e, hipStream_t s) {
   const StepArgs a = make_args(e, 0);
-  if (e->ry == 1) launch_e_w<T, MAT, 1>(a, e->waves, s);
-  else if (e->ry == 4) launch_e_w<T, MAT, 4>(a, e->waves, s);
-  else launch_e_w<T, MAT, 2>(a, e->waves, s);
+  const dim3 grid(a.nbp);
+  const bool w8 = e->waves == 8;
+  if (e->ry == 2) {
+    if (w8) hipLaunchKernelGGL((lsf::k_update_e<T, MAT, 2, 8, PER, NP>), grid, dim3(512), 0, s, a);
+    else hipLaunchKernelGGL((lsf::k_update_e<T, MAT, 2, 4, PER, NP>), grid, dim3(256), 0, s, a);
+  } else {
+    if (w8) hipLaunchKernelGGL((lsf::k_update_e<T, MAT, 1, 8, PER, NP>), grid, dim3(512), 0, s, a);
+    else hipLaunchKernelGGL((lsf::k_update_e<T, MAT, 1, 4, PER, NP>), grid, dim3(256), 0, s, a);
+  }
+}
+
+template <class T, int MAT, bool PER>
+void launch_e_m(const lsf_engine* e, hipStream_t s) {
+  const int np = e->n_ade > 0 ? e->max_poles : 0;
+  if (np == 0) launch_e_np<T, MAT, PER, 0>(e, s);
+  else if (np <= 2) launch_e_np<T, MAT, PER, 2>(e, s);
+  else if (np == 3) launch_e_np<T, MAT, PER, 3>(e, s);
+  else launch_e_np<T, MAT, PER, 4>(e, s);
 }
 
 template <class T>
 void launch_e(const lsf_engine* e, hipStream_t s) {
-  if (e->mid) launch_e_m<T, 1>(e, s);
-  else launch_e_m<T, 0>(e, s);
+  const bool per = any_periodic(e);
+  if (e->mid) {
+    if (per) launch_e_m<T, 1, true>(e, s);
+    else launch_e_m<T, 1, false>(e, s);
+  } else {
+    if (per) launch_e_m<T, 0, true>(e, s);
+    else launch_e_m<T, 0, false>(e, s);
+  }
 }
 
 inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -205,24 +218,22 @@ template <class T>
 void run_op(lsf_engine* e, const Op& o, hipStream_t s) {
   const int slot = o.phase <= LSF_PH_ADVANCE_H ? 0 : 1;
   T* F = (T*)(o.field == 0 ? e->E : e->H);
-  T* D = (T*)e->D;
   switch (o.kind) {
     case OP_GATHER:
       if (o.n > 0)
         hipLaunchKernelGGL(lsf::k_gather<T>, dim3(nblk(o.n, 256)), dim3(256), 0, s, o.n, o.width, o.target, F,
-                           (const T*)o.src, o.sidx, (const T*)o.w, o.ade_idx, o.ade_scale, D);
+                           (const T*)o.src, o.sidx, (const T*)o.w, o.scale);
       break;
     case OP_MODAL:
       if (o.n > 0)
         hipLaunchKernelGGL(lsf::k_modal<T>, dim3(nblk(o.n, 256)), dim3(256), 0, s, o.n, o.width, o.target, F,
-                           (const T*)o.w, o.table, o.table_len, (const long long*)e->counter, slot, o.ade_idx,
-                           o.ade_scale, D);
+                           (const T*)o.w, o.table, o.table_len, (const long long*)e->counter, slot, o.scale);
       break;
     case OP_TABLE:
       if (o.n > 0)
         hipLaunchKernelGGL(lsf::k_table<T>, dim3(nblk(o.n, 256)), dim3(256), 0, s, o.n, o.target, o.row_ptr, F,
                            o.table, o.m, o.cmax, o.base, o.i0, o.u0, o.coef, (const long long*)e->counter, slot,
-                           o.ade_idx, o.ade_scale, D);
+                           o.scale);
       break;
     case OP_VIEW:
       hipLaunchKernelGGL(lsf::k_table_view, dim3(1), dim3(64), 0, s, (int)o.n, o.out, o.table, o.m, o.cmax, o.base,
@@ -338,6 +349,10 @@ int lsf_create(const lsf_grid* gd, lsf_engine** out) {
     if (!g.per[a]) g.ph[a] = lsf::mk(1.f, 0.f);
   }
   g.N = (long long)g.n[0] * g.n[1] * g.n[2];
+  if (3 * g.N >= (1ll << 31)) {
+    delete e;
+    return fail(LSF_EINVAL, "grid too large: 3 * cells must stay below 2^31 (32-bit offsets)");
+  }
   const double cdx = cubic ? gd->dt / gd->spacing[0] : gd->dt;
   g.cH = (float)cdx;
   g.cdx = (float)cdx;
@@ -406,13 +421,16 @@ int lsf_set_pml(lsf_engine* e, int field, int axis, const void* prof, int n_slab
   return LSF_OK;
 }
 
-int lsf_set_ade(lsf_engine* e, const int32_t* chunk_base, void* D, void* P, int64_t n, int max_poles,
-                const float* media, int n_media) {
+int lsf_set_ade(lsf_engine* e, const int32_t* chunk_base, void* P, int64_t n, int max_poles, const float* media,
+                int n_media) {
   if (valid(e)) return LSF_EINVAL;
-  if (n > 0 && (!chunk_base || !D || !P || !media)) return fail(LSF_EINVAL, "missing ADE arrays");
-  if (max_poles < 0 || max_poles > 16) return fail(LSF_EINVAL, "max_poles must be in [0, 16]");
+  if (n > 0 && (!chunk_base || !P || !media)) return fail(LSF_EINVAL, "missing ADE arrays");
+  if (max_poles < 0 || max_poles > lsf::MAX_POLES) return fail(LSF_EINVAL, "pure D-form media may have at most 4 poles");
+  if (n > 0 && max_poles < 1) return fail(LSF_EINVAL, "D-form nodes need at least one pole");
+  if (n >= (1ll << 30)) return fail(LSF_EINVAL, "too many D-form nodes for 32-bit indexing");
+  if (n > 0 && n_media * (2 + 3 * max_poles) > lsf::MAX_MEDIA_FLOATS)
+    return fail(LSF_EINVAL, "too many D-form media for the E kernel's table (at most 16 media of 4 poles)");
   e->chunk_base = (const int*)chunk_base;
-  e->D = D;
   e->P = P;
   e->n_ade = n;
   e->max_poles = max_poles;
@@ -422,7 +440,7 @@ int lsf_set_ade(lsf_engine* e, const int32_t* chunk_base, void* D, void* P, int6
 }
 
 int lsf_add_gather(lsf_engine* e, int phase, int field, int64_t n, int width, const int64_t* target, const void* src,
-                   const int64_t* sidx, const void* w, const int32_t* ade_idx, const float* ade_scale) {
+                   const int64_t* sidx, const void* w, const float* scale) {
   if (valid(e)) return LSF_EINVAL;
   if (n > 0 && (!target || !src || !sidx || !w || width < 1)) return fail(LSF_EINVAL, "missing gather arrays");
   Op o;
@@ -435,13 +453,12 @@ int lsf_add_gather(lsf_engine* e, int phase, int field, int64_t n, int width, co
   o.src = src;
   o.sidx = (const long long*)sidx;
   o.w = w;
-  o.ade_idx = (const int*)ade_idx;
-  o.ade_scale = ade_scale;
+  o.scale = scale;
   return add_op(e, o);
 }
 
 int lsf_add_modal(lsf_engine* e, int phase, int field, int64_t n, int nm, const int64_t* target, const void* m,
-                  const float* table, int64_t table_len, const int32_t* ade_idx, const float* ade_scale) {
+                  const float* table, int64_t table_len, const float* scale) {
   if (valid(e)) return LSF_EINVAL;
   if (n > 0 && (!target || !m || !table || nm < 1)) return fail(LSF_EINVAL, "missing modal arrays");
   Op o;
@@ -454,14 +471,13 @@ int lsf_add_modal(lsf_engine* e, int phase, int field, int64_t n, int nm, const 
   o.w = m;
   o.table = table;
   o.table_len = table_len;
-  o.ade_idx = (const int*)ade_idx;
-  o.ade_scale = ade_scale;
+  o.scale = scale;
   return add_op(e, o);
 }
 
 int lsf_add_table(lsf_engine* e, int phase, int field, int64_t n, const int64_t* target, const int32_t* row_ptr,
                   const float* T, int m, int64_t cmax, const int64_t* base, const int64_t* i0, const float* u0,
-                  const float* coef, const int32_t* ade_idx, const float* ade_scale) {
+                  const float* coef, const float* scale) {
   if (valid(e)) return LSF_EINVAL;
   if (m < 1) return fail(LSF_EINVAL, "table stride must be >= 1");
   if (n > 0 && (!target || !row_ptr || !T || !base || !i0 || !u0 || !coef))
@@ -480,8 +496,7 @@ int lsf_add_table(lsf_engine* e, int phase, int field, int64_t n, const int64_t*
   o.i0 = (const long long*)i0;
   o.u0 = u0;
   o.coef = coef;
-  o.ade_idx = (const int*)ade_idx;
-  o.ade_scale = ade_scale;
+  o.scale = scale;
   return add_op(e, o);
 }
 
@@ -634,8 +649,8 @@ int lsf_add_time_monitor(lsf_engine* e, int64_t np, const int64_t* e_idx, void* 
 
 int lsf_set_tiling(lsf_engine* e, int ry, int waves, int zc) {
   if (valid(e)) return LSF_EINVAL;
-  if (!(ry == 1 || ry == 2 || ry == 4)) return fail(LSF_EINVAL, "rows per lane must be 1, 2 or 4");
-  if (!(waves == 2 || waves == 4 || waves == 8)) return fail(LSF_EINVAL, "waves per block must be 2, 4 or 8");
+  if (!(ry == 1 || ry == 2)) return fail(LSF_EINVAL, "rows per lane must be 1 or 2");
+  if (!(waves == 4 || waves == 8)) return fail(LSF_EINVAL, "waves per block must be 4 or 8");
   if (zc < 1) return fail(LSF_EINVAL, "z chunk must be >= 1");
   if (e->exec) return fail(LSF_ESTATE, "the engine has a captured graph; reset it before retiling");
   e->ry = ry;
@@ -650,6 +665,49 @@ int lsf_step(lsf_engine* e, int n, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   for (int q = 0; q < n; ++q) one_step(e, s);
   return hip_check(hipGetLastError(), "step launch");
+}
+
+int lsf_step_timed(lsf_engine* e, int n, void* stream, double* ms) {
+  if (valid(e)) return LSF_EINVAL;
+  if (!e->cbd && !e->mid) return fail(LSF_ESTATE, "no material set");
+  if (!ms || n < 1) return fail(LSF_EINVAL, "bad timed step arguments");
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<hipEvent_t> ev(4 * (size_t)n, nullptr);
+  int rc = LSF_OK;
+  for (auto& x : ev)
+    if ((rc = hip_check(hipEventCreate(&x), "event create"))) break;
+  if (!rc) {
+    for (int q = 0; q < n; ++q) {
+      (void)hipEventRecord(ev[4 * q + 0], s);
+      if (e->cplx) launch_h<c32>(e, s);
+      else launch_h<float>(e, s);
+      (void)hipEventRecord(ev[4 * q + 1], s);
+      run_phase(e, LSF_PH_CORRECT_H, s);
+      run_phase(e, LSF_PH_ADVANCE_H, s);
+      (void)hipEventRecord(ev[4 * q + 2], s);
+      if (e->cplx) launch_e<c32>(e, s);
+      else launch_e<float>(e, s);
+      (void)hipEventRecord(ev[4 * q + 3], s);
+      for (int ph = LSF_PH_OFFDIAG; ph <= LSF_PH_ADVANCE_E; ++ph) run_phase(e, ph, s);
+    }
+    rc = hip_check(hipGetLastError(), "timed step launch");
+  }
+  if (!rc) rc = hip_check(hipEventSynchronize(ev.back()), "event synchronize");
+  if (!rc) {
+    double th = 0.0, te = 0.0;
+    for (int q = 0; q < n; ++q) {
+      float a = 0.f, b = 0.f;
+      (void)hipEventElapsedTime(&a, ev[4 * q + 0], ev[4 * q + 1]);
+      (void)hipEventElapsedTime(&b, ev[4 * q + 2], ev[4 * q + 3]);
+      th += a;
+      te += b;
+    }
+    ms[0] += th;
+    ms[1] += te;
+  }
+  for (auto& x : ev)
+    if (x) (void)hipEventDestroy(x);
+  return rc;
 }
 
 int lsf_sample(lsf_engine* e, void* stream) {

@@ -7,26 +7,18 @@
 
 namespace lsf {
 
-// A correction v for E (or H) node t. E nodes integrated in D-form take it through D (the reference adds
-// it to E before the group update: D += E_now - E_prev picks it up, and the new E is (D - sum P) / eps_inf).
+// A correction v for E (or H) node t. The reference adds it to E before the D-form group update, where
+// D += E_now - E_prev picks it up and the new E is (D - sum P) / eps_inf: for a pure D-form node the
+// correction therefore moves E by v / eps_inf. scale[r] holds that factor (1 for other nodes).
 template <class T>
-__device__ __forceinline__ void apply_corr(T* F, long long t, T v, const int* ade_idx, const float* ade_scale,
-                                           T* D, long long r) {
-  if (ade_idx) {
-    const int q = ade_idx[r];
-    if (q >= 0) {
-      D[q] = add(D[q], v);
-      F[t] = add(F[t], scl(ade_scale[r], v));
-      return;
-    }
-  }
-  F[t] = add(F[t], v);
+__device__ __forceinline__ void apply_corr(T* F, long long t, T v, const float* scale, long long r) {
+  F[t] = add(F[t], scale ? scl(scale[r], v) : v);
 }
 
 template <class T>
 __global__ void k_gather(long long n_rows, int width, const long long* __restrict__ target, T* F,
                          const T* __restrict__ src, const long long* __restrict__ sidx, const T* __restrict__ w,
-                         const int* ade_idx, const float* ade_scale, T* D) {
+                         const float* scale) {
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_rows) return;
   T acc = zero<T>();
@@ -34,20 +26,20 @@ __global__ void k_gather(long long n_rows, int width, const long long* __restric
     const long long e = r * width + q;
     acc = add(acc, mul(w[e], src[sidx[e]]));
   }
-  apply_corr(F, target[r], acc, ade_idx, ade_scale, D, r);
+  apply_corr(F, target[r], acc, scale, r);
 }
 
 template <class T>
 __global__ void k_modal(long long n_rows, int nm, const long long* __restrict__ target, T* F,
                         const T* __restrict__ m, const float* __restrict__ table, long long table_len,
-                        const long long* counter, int slot, const int* ade_idx, const float* ade_scale, T* D) {
+                        const long long* counter, int slot, const float* scale) {
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_rows) return;
   const long long t = counter[slot];
   if (t < 0 || t >= table_len) return;
   T acc = zero<T>();
   for (int q = 0; q < nm; ++q) acc = add(acc, scl(table[t * nm + q], m[r * nm + q]));
-  apply_corr(F, target[r], acc, ade_idx, ade_scale, D, r);
+  apply_corr(F, target[r], acc, scale, r);
 }
 
 // Quintic interpolation of tables sampled every m steps (layered.py:164-187), float32 as on the GPU
@@ -76,7 +68,7 @@ template <class T>
 __global__ void k_table(long long n_rows, const long long* __restrict__ target, const int* __restrict__ row_ptr, T* F,
                         const float* __restrict__ tab, int m, long long cmax, const long long* __restrict__ base,
                         const long long* __restrict__ i0, const float* __restrict__ u0, const float* __restrict__ coef,
-                        const long long* counter, int slot, const int* ade_idx, const float* ade_scale, T* D) {
+                        const long long* counter, int slot, const float* scale) {
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_rows) return;
   const long long t = counter[slot];
@@ -85,7 +77,7 @@ __global__ void k_table(long long n_rows, const long long* __restrict__ target, 
   T v = zero<T>();
   if constexpr (sizeof(T) == sizeof(float)) v = acc;
   else v = mk(acc, 0.f);
-  apply_corr(F, target[r], v, ade_idx, ade_scale, D, r);
+  apply_corr(F, target[r], v, scale, r);
 }
 
 __global__ void k_table_view(int n, float* out, const float* __restrict__ tab, int m, long long cmax,

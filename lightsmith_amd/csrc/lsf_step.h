@@ -2,10 +2,11 @@
 // psi recursions and (E kernel) the pure-ADE D-form update for its cells.
 //
 // Mapping: a wavefront's 64 lanes take 64 consecutive x cells; each lane holds RY consecutive y rows in
-// registers and marches up a chunk of z planes, carrying the previous/next plane in registers. x
+// registers and marches up a chunk of z planes, carrying the neighbouring plane in registers. x
 // neighbours come from the adjacent lane (cross-lane shuffle; the edge lane loads its halo cell), y
 // neighbours from the lane's own rows (the edge row loads one halo row), z neighbours from the marching
-// registers. Each field value is read from HBM about once per kernel.
+// registers. Every z iteration issues all of its loads before any store, so one memory round trip
+// covers the iteration (stores to H or E would otherwise order each following load behind them).
 #pragma once
 
 #include "lsf_common.h"
@@ -28,65 +29,102 @@ struct StepArgs {
   const float* mtab;     // indexed material table [256]
   PmlAx pml[3];          // this kernel's field
   const int* chunk_base;
-  void* D;
   void* P;
-  long long n_ade;
+  int n_ade;
   int nchunk;
   int max_poles;
   int media_stride;
   const float* media;
+  int n_media_floats;
   long long* counter;
   int tick;
   int tiles_x, tiles_y, zc, nb, nbp;
 };
+
+// D-form media table held in LDS by the E kernel: n_media * (2 + 3 * max_poles) floats
+constexpr int MAX_MEDIA_FLOATS = 16 * (2 + 3 * 4);
 
 template <class T>
 struct Pair {
   T p, q;  // P^n, P^{n-1}
 };
 
+// pure-ADE media: at most this many poles each (the E kernel keeps every pole pair of a row in registers)
+constexpr int MAX_POLES = 4;
+
+__device__ __forceinline__ float4 no_pml() { return make_float4(1.f, 0.f, 1.f, -1.f); }
+
+// A load whose address is the same for every lane, through the constant address space: it becomes a scalar
+// load into SGPRs (the data is never written by the kernels that read it this way).
+typedef const __attribute__((address_space(4))) float* cfloat_ptr;
+typedef const __attribute__((address_space(4))) int* cint_ptr;
+__device__ __forceinline__ float4 ld_uniform(const float4* p, unsigned idx) {
+  cfloat_ptr q = (cfloat_ptr)(p + idx);
+  return make_float4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ int ld_uniform(const int* p, unsigned idx) { return ((cint_ptr)p)[idx]; }
+
+__device__ __forceinline__ unsigned cell32(unsigned nx, unsigned ny, int i, int j, int k) {
+  return ((unsigned)k * ny + (unsigned)j) * nx + (unsigned)i;
+}
+
+// psi index of a cell in the slab array of `axis` (slab extent ns along the axis, full extent elsewhere)
+__device__ __forceinline__ unsigned psi_index(int axis, int p, int ns, unsigned nx, unsigned ny, int i, int j, int k) {
+  if (axis == 0) return ((unsigned)k * ny + (unsigned)j) * (unsigned)ns + (unsigned)p;
+  if (axis == 1) return ((unsigned)k * (unsigned)ns + (unsigned)p) * nx + (unsigned)i;
+  return ((unsigned)p * ny + (unsigned)j) * nx + (unsigned)i;
+}
+
+// F(i, j, k) where an index may be a periodic axis' high ghost (n - 1): its image at 1 times exp(+i k L)
+// (engine.py:595-611, low=False). Without periodic axes (PER false) a plain load.
+template <bool PER, class T>
+__device__ __forceinline__ T load_hi(const T* __restrict__ F, const Grid& g, int i, int j, int k) {
+  if (!PER) return F[cell32(g.n[0], g.n[1], i, j, k)];
+  return ld_hi(F, g, i, j, k);
+}
+
+// F(i, j, k) where an index may be a periodic axis' low ghost (0): its image at n - 2 times exp(-i k L).
+template <bool PER, class T>
+__device__ __forceinline__ T load_lo(const T* __restrict__ F, const Grid& g, int i, int j, int k) {
+  if (!PER) return F[cell32(g.n[0], g.n[1], i, j, k)];
+  return ld_lo(F, g, i, j, k);
+}
+
 template <int MAT>
-__device__ __forceinline__ float load_cb(const StepArgs& a, const float* tab, int c, long long cell, uint32_t ids) {
-  if (MAT == 0) return a.cbd[c * a.g.N + cell];
+__device__ __forceinline__ float load_cb(const StepArgs& a, const float* tab, int c, unsigned cell, uint32_t ids) {
+  if (MAT == 0) return a.cbd[(unsigned)c * (unsigned)a.g.N + cell];
   return tab[(ids >> (8 * c)) & 255u];
 }
 
-// One CPML term of a component (engine.py:636-643 for H, 656-663 for E): psi = b psi + cp der,
-// field += kp der (when kappa > 1) + psi, with cp = sgn coef s c and kp = sgn coef s (1/kappa - 1).
+// psi = b psi + cp der; returns the field increment kp der (when kappa > 1) + psi_new
+// (engine.py:636-643, 656-663) with cp = coef_s c and kp = coef_s (1/kappa - 1).
 template <class T>
-__device__ __forceinline__ T pml_term(const PmlAx& ax, int c, int axis, int pos, int i, int j, int k, const Grid& g,
-                                      T der, float coef_s, T acc) {
-  const float4 pr = ax.prof[pos];
-  const int p = (int)pr.w;
-  if (p < 0) return acc;
-  long long q;
-  if (axis == 0) q = ((long long)k * g.n[1] + j) * ax.ns + p;
-  else if (axis == 1) q = ((long long)k * ax.ns + p) * g.n[0] + i;
-  else q = ((long long)p * g.n[1] + j) * g.n[0] + i;
-  T* psi = (T*)ax.psi[c];
-  T v = psi[q];
-  v = add(scl(pr.x, v), scl(coef_s * pr.y, der));
-  psi[q] = v;
-  if (ax.has_k) acc = add(acc, scl(coef_s * (pr.z - 1.f), der));
-  return add(acc, v);
+__device__ __forceinline__ T pml_apply(const float4 pr, int has_k, float coef_s, T der, T& psi) {
+  psi = add(scl(pr.x, psi), scl(coef_s * pr.y, der));
+  return has_k ? add(scl(coef_s * (pr.z - 1.f), der), psi) : psi;
+}
+
+__device__ __forceinline__ bool in_box(const int (&r)[3][2], int i, int j, int k) {
+  return i >= r[0][0] && i < r[0][1] && j >= r[1][0] && j < r[1][1] && k >= r[2][0] && k < r[2][1];
 }
 
 // ---------------------------------------------------------------------------------------------
 // H update (engine.py:630-643): H_c -= cH [s_d1 (E_d2(+e_d1) - E_d2) - s_d2 (E_d1(+e_d2) - E_d1)] + CPML.
 // ---------------------------------------------------------------------------------------------
-template <class T, int RY, int WAVES>
+template <class T, int RY, int WAVES, bool PER>
 __global__ void __launch_bounds__(64 * WAVES) k_update_h(StepArgs a) {
   const Grid& g = a.g;
   const int t = xcd_tile(blockIdx.x, a.nbp);
   if (t >= a.nb) return;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tx = t % a.tiles_x, rest = t / a.tiles_x;
   const int ty = rest % a.tiles_y, tz = rest / a.tiles_y;
   const int i = tx * 64 + lane;
   const int j0 = (ty * WAVES + wv) * RY;
   if (j0 >= g.n[1]) return;
   const int z0 = tz * a.zc, z1 = min(z0 + a.zc, g.n[2]);
-  const long long N = g.N;
+  const unsigned N = (unsigned)g.N, nx = g.n[0], ny = g.n[1];
   const T* __restrict__ Ex = (const T*)a.E;
   const T* __restrict__ Ey = Ex + N;
   const T* __restrict__ Ez = Ex + 2 * N;
@@ -95,28 +133,60 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_h(StepArgs a) {
   const int ih = tx * 64 + 64;
   const bool hx_ok = (lane == 63) && ih < g.n[0];
 
+  // CPML profiles: x per lane and y per row are constant over the march
+  const float4 px = (a.pml[0].prof && inx) ? a.pml[0].prof[i] : no_pml();
+  float4 py[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) py[r] = (a.pml[1].prof && j0 + r < g.n[1]) ? ld_uniform(a.pml[1].prof, j0 + r) : no_pml();
+
   T cEx[RY + 1], cEy[RY], cEz[RY + 1], hEy[RY], hEz[RY];
   T nEx[RY + 1], nEy[RY], nEz[RY + 1], nhEy[RY], nhEz[RY];
 
-  auto load = [&](int k, T* xE, T* yE, T* zE, T* hy, T* hz) {
+  auto load_plane = [&](int k, T* xE, T* yE, T* zE, T* hy, T* hz) {
 #pragma unroll
     for (int r = 0; r <= RY; ++r) {
       const int j = j0 + r;
       const bool ok = inx && j < g.n[1] && k < g.n[2];
-      xE[r] = ok ? ld_hi(Ex, g, i, j, k) : zero<T>();
-      zE[r] = ok ? ld_hi(Ez, g, i, j, k) : zero<T>();
+      xE[r] = ok ? load_hi<PER>(Ex, g, i, j, k) : zero<T>();
+      zE[r] = ok ? load_hi<PER>(Ez, g, i, j, k) : zero<T>();
       if (r < RY) {
-        yE[r] = ok ? ld_hi(Ey, g, i, j, k) : zero<T>();
+        yE[r] = ok ? load_hi<PER>(Ey, g, i, j, k) : zero<T>();
         const bool hok = hx_ok && j < g.n[1] && k < g.n[2];
-        hy[r] = hok ? ld_hi(Ey, g, ih, j, k) : zero<T>();
-        hz[r] = hok ? ld_hi(Ez, g, ih, j, k) : zero<T>();
+        hy[r] = hok ? load_hi<PER>(Ey, g, ih, j, k) : zero<T>();
+        hz[r] = hok ? load_hi<PER>(Ez, g, ih, j, k) : zero<T>();
       }
     }
   };
 
-  load(z0, cEx, cEy, cEz, hEy, hEz);
+  load_plane(z0, cEx, cEy, cEz, hEy, hEz);
   for (int k = z0; k < z1; ++k) {
-    load(k + 1, nEx, nEy, nEz, nhEy, nhEz);
+    const float4 pz = a.pml[2].prof ? ld_uniform(a.pml[2].prof, k) : no_pml();
+    // ---- loads: the next E plane, old H, psi
+    load_plane(k + 1, nEx, nEy, nEz, nhEy, nhEz);
+    bool st[RY][3];
+    T hv[RY][3];
+    T ps[RY][3][2];  // psi of component c along axes d1 = c+1, d2 = c+2 (mod 3)
+    float4 pr[RY][3][2];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int j = j0 + r;
+      const unsigned cell = cell32(nx, ny, i, j, k);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        st[r][c] = inx && j < g.n[1] && in_box(g.rH[c], i, j, k);
+        hv[r][c] = st[r][c] ? H[(unsigned)c * N + cell] : zero<T>();
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const int ax = (c + 1 + s) % 3;
+          pr[r][c][s] = ax == 0 ? px : (ax == 1 ? py[r] : pz);
+          const int p = (int)pr[r][c][s].w;
+          ps[r][c][s] = zero<T>();
+          if (st[r][c] && p >= 0)
+            ps[r][c][s] = ((const T*)a.pml[ax].psi[c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)];
+        }
+      }
+    }
+    // ---- compute
     T ezp[RY], eyp[RY];
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
@@ -129,38 +199,50 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_h(StepArgs a) {
     }
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        // da: difference along d1 of E_d2; db: difference along d2 of E_d1
+        T da, db;
+        if (c == 0) {
+          da = sub(cEz[r + 1], cEz[r]);
+          db = sub(nEy[r], cEy[r]);
+        } else if (c == 1) {
+          da = sub(nEx[r], cEx[r]);
+          db = sub(ezp[r], cEz[r]);
+        } else {
+          da = sub(eyp[r], cEy[r]);
+          db = sub(cEx[r + 1], cEx[r]);
+        }
+        const int d1 = (c + 1) % 3, d2 = (c + 2) % 3;
+        T acc = scl(-g.cH, sub(scl(g.s[d1], da), scl(g.s[d2], db)));
+        // CPML terms in ascending axis order (the reference's term order)
+        const int first = d1 < d2 ? 0 : 1;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int s = q == 0 ? first : 1 - first;
+          const int ax = (c + 1 + s) % 3;
+          if (a.pml[ax].prof && (int)pr[r][c][s].w >= 0)
+            acc = add(acc, pml_apply(pr[r][c][s], a.pml[ax].has_k, s == 0 ? -g.cH * g.s[d1] : g.cH * g.s[d2],
+                                     s == 0 ? da : db, ps[r][c][s]));
+        }
+        hv[r][c] = add(hv[r][c], acc);
+      }
+    }
+    // ---- stores
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
       const int j = j0 + r;
-      if (!inx || j >= g.n[1]) continue;
-      const long long cell = cidx(g, i, j, k);
-      // H_x: d1 = y, d2 = z
-      if (i >= g.rH[0][0][0] && i < g.rH[0][0][1] && j >= g.rH[0][1][0] && j < g.rH[0][1][1] &&
-          k >= g.rH[0][2][0] && k < g.rH[0][2][1]) {
-        const T da = sub(cEz[r + 1], cEz[r]);
-        const T db = sub(nEy[r], cEy[r]);
-        T acc = scl(-g.cH, sub(scl(g.s[1], da), scl(g.s[2], db)));
-        if (a.pml[1].prof) acc = pml_term(a.pml[1], 0, 1, j, i, j, k, g, da, -g.cH * g.s[1], acc);
-        if (a.pml[2].prof) acc = pml_term(a.pml[2], 0, 2, k, i, j, k, g, db, g.cH * g.s[2], acc);
-        H[cell] = add(H[cell], acc);
-      }
-      // H_y: d1 = z, d2 = x
-      if (i >= g.rH[1][0][0] && i < g.rH[1][0][1] && j >= g.rH[1][1][0] && j < g.rH[1][1][1] &&
-          k >= g.rH[1][2][0] && k < g.rH[1][2][1]) {
-        const T da = sub(nEx[r], cEx[r]);
-        const T db = sub(ezp[r], cEz[r]);
-        T acc = scl(-g.cH, sub(scl(g.s[2], da), scl(g.s[0], db)));
-        if (a.pml[0].prof) acc = pml_term(a.pml[0], 1, 0, i, i, j, k, g, db, g.cH * g.s[0], acc);
-        if (a.pml[2].prof) acc = pml_term(a.pml[2], 1, 2, k, i, j, k, g, da, -g.cH * g.s[2], acc);
-        H[N + cell] = add(H[N + cell], acc);
-      }
-      // H_z: d1 = x, d2 = y
-      if (i >= g.rH[2][0][0] && i < g.rH[2][0][1] && j >= g.rH[2][1][0] && j < g.rH[2][1][1] &&
-          k >= g.rH[2][2][0] && k < g.rH[2][2][1]) {
-        const T da = sub(eyp[r], cEy[r]);
-        const T db = sub(cEx[r + 1], cEx[r]);
-        T acc = scl(-g.cH, sub(scl(g.s[0], da), scl(g.s[1], db)));
-        if (a.pml[0].prof) acc = pml_term(a.pml[0], 2, 0, i, i, j, k, g, da, -g.cH * g.s[0], acc);
-        if (a.pml[1].prof) acc = pml_term(a.pml[1], 2, 1, j, i, j, k, g, db, g.cH * g.s[1], acc);
-        H[2 * N + cell] = add(H[2 * N + cell], acc);
+      const unsigned cell = cell32(nx, ny, i, j, k);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (!st[r][c]) continue;
+        H[(unsigned)c * N + cell] = hv[r][c];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const int ax = (c + 1 + s) % 3;
+          const int p = (int)pr[r][c][s].w;
+          if (p >= 0) ((T*)a.pml[ax].psi[c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)] = ps[r][c][s];
+        }
       }
     }
 #pragma unroll
@@ -178,32 +260,38 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_h(StepArgs a) {
 
 // ---------------------------------------------------------------------------------------------
 // E update (engine.py:645-663) + pure-ADE groups (engine.py:769-793):
-// E_c += cb [s_d1 (H_d2 - H_d2(-e_d1)) - s_d2 (H_d1 - H_d1(-e_d2))] + CPML; D-form nodes then take
-// D += dE, P^{n+1} = c1 P^n + c2 P^{n-1} + c3 E^n per pole, E = (D - sum P) / eps_inf.
+// E_c += cb [s_d1 (H_d2 - H_d2(-e_d1)) - s_d2 (H_d1 - H_d1(-e_d2))] + CPML. A D-form node (NP poles at most)
+// advances its poles from E^n, P^{n+1} = c1 P^n + c2 P^{n-1} + c3 E^n, and takes
+// E^{n+1} = E^n + (dE - sum_p (P_p^{n+1} - P_p^n)) / eps_inf, which is (D^{n+1} - sum P^{n+1}) / eps_inf with
+// D^{n+1} = D^n + dE and D^n = eps_inf E^n + sum P^n (the reference's D-form, without storing D).
+// The H plane and the material of plane k+1 are loaded during iteration k.
 // ---------------------------------------------------------------------------------------------
-template <class T, int MAT, int RY, int WAVES>
+template <class T, int MAT, int RY, int WAVES, bool PER, int NP>
 __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
   __shared__ float tab[256];
+  __shared__ float med_tab[MAX_MEDIA_FLOATS];
   if (a.tick && blockIdx.x == 0 && threadIdx.x == 0) {
     const long long n = a.counter[0];
     a.counter[1] = n;
     a.counter[0] = n + 1;
   }
-  if (MAT == 1) {
+  if (MAT == 1)
     for (int q = threadIdx.x; q < 256; q += 64 * WAVES) tab[q] = a.mtab[q];
-    __syncthreads();
-  }
+  if (NP > 0)
+    for (int q = threadIdx.x; q < a.n_media_floats; q += 64 * WAVES) med_tab[q] = a.media[q];
+  __syncthreads();
   const Grid& g = a.g;
   const int t = xcd_tile(blockIdx.x, a.nbp);
   if (t >= a.nb) return;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tx = t % a.tiles_x, rest = t / a.tiles_x;
   const int ty = rest % a.tiles_y, tz = rest / a.tiles_y;
   const int i = tx * 64 + lane;
   const int j0 = (ty * WAVES + wv) * RY;
   if (j0 >= g.n[1]) return;
   const int z0 = tz * a.zc, z1 = min(z0 + a.zc, g.n[2]);
-  const long long N = g.N;
+  const unsigned N = (unsigned)g.N, nx = g.n[0], ny = g.n[1];
   const T* __restrict__ Hx = (const T*)a.H;
   const T* __restrict__ Hy = Hx + N;
   const T* __restrict__ Hz = Hx + 2 * N;
@@ -212,37 +300,100 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
   const int il = tx * 64 - 1;
   const bool xl_ok = (lane == 0) && il >= 0;
   const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  const float4 px = (a.pml[0].prof && inx) ? a.pml[0].prof[i] : no_pml();
+  float4 py[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) py[r] = (a.pml[1].prof && j0 + r < g.n[1]) ? ld_uniform(a.pml[1].prof, j0 + r) : no_pml();
 
-  // rows r = -1 .. RY-1 of H_x and H_z are stored at r + 1
-  T cHx[RY + 1], cHy[RY], cHz[RY + 1], pHx[RY], pHy[RY], xHy[RY], xHz[RY];
+  // H rows r = -1 .. RY-1 of H_x and H_z are stored at r + 1
+  T cHx[RY + 1], cHy[RY], cHz[RY + 1], xHy[RY], xHz[RY];
+  T nHx[RY + 1], nHy[RY], nHz[RY + 1], nxHy[RY], nxHz[RY];
+  T pHx[RY], pHy[RY];
+  // material of the current / next plane: packed 8-bit ids (MAT 1) or the three coefficients (MAT 0)
+  uint32_t id_c[RY], id_n[RY];
+  float cbd_c[RY][MAT == 0 ? 3 : 1], cbd_n[RY][MAT == 0 ? 3 : 1];
 
-  auto load = [&](int k) {
+  auto load_plane = [&](int k, T* xH, T* yH, T* zH, T* hy, T* hz, uint32_t* ids, float (*cbd)[MAT == 0 ? 3 : 1]) {
 #pragma unroll
     for (int r = -1; r < RY; ++r) {
       const int j = j0 + r;
-      const bool ok = inx && j >= 0 && j < g.n[1] && k >= 0 && k < g.n[2];
-      cHx[r + 1] = ok ? ld_lo(Hx, g, i, j, k) : zero<T>();
-      cHz[r + 1] = ok ? ld_lo(Hz, g, i, j, k) : zero<T>();
+      const bool ok = inx && j >= 0 && j < g.n[1] && k < g.n[2];
+      xH[r + 1] = ok ? load_lo<PER>(Hx, g, i, j, k) : zero<T>();
+      zH[r + 1] = ok ? load_lo<PER>(Hz, g, i, j, k) : zero<T>();
       if (r >= 0) {
-        cHy[r] = ok ? ld_lo(Hy, g, i, j, k) : zero<T>();
-        const bool lok = xl_ok && j < g.n[1] && k >= 0 && k < g.n[2];
-        xHy[r] = lok ? ld_lo(Hy, g, il, j, k) : zero<T>();
-        xHz[r] = lok ? ld_lo(Hz, g, il, j, k) : zero<T>();
+        yH[r] = ok ? load_lo<PER>(Hy, g, i, j, k) : zero<T>();
+        const bool lok = xl_ok && j < g.n[1] && k < g.n[2];
+        hy[r] = lok ? load_lo<PER>(Hy, g, il, j, k) : zero<T>();
+        hz[r] = lok ? load_lo<PER>(Hz, g, il, j, k) : zero<T>();
+        const unsigned cell = cell32(nx, ny, i, ok ? j : 0, ok ? k : 0);
+        if (MAT == 1) {
+          ids[r] = ok ? a.mid[cell] : 0u;
+        } else {
+#pragma unroll
+          for (int c = 0; c < (MAT == 0 ? 3 : 1); ++c) cbd[r][c] = ok ? a.cbd[(unsigned)c * N + cell] : 0.f;
+        }
       }
     }
   };
+  auto coef = [&](int r, int c) -> float {
+    if (MAT == 1) return tab[(id_c[r] >> (8 * c)) & 255u];
+    return cbd_c[r][MAT == 0 ? c : 0];
+  };
 
-  // previous plane (H_x, H_y at k - 1)
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
     const int j = j0 + r, k = z0 - 1;
     const bool ok = inx && j < g.n[1] && k >= 0;
-    pHx[r] = ok ? ld_lo(Hx, g, i, j, k) : zero<T>();
-    pHy[r] = ok ? ld_lo(Hy, g, i, j, k) : zero<T>();
+    pHx[r] = ok ? load_lo<PER>(Hx, g, i, j, k) : zero<T>();
+    pHy[r] = ok ? load_lo<PER>(Hy, g, i, j, k) : zero<T>();
   }
+  load_plane(z0, cHx, cHy, cHz, xHy, xHz, id_c, cbd_c);
 
+  Pair<T>* P = (Pair<T>*)a.P;
   for (int k = z0; k < z1; ++k) {
-    load(k);
+    const float4 pz = a.pml[2].prof ? ld_uniform(a.pml[2].prof, k) : no_pml();
+    // ---- loads: next H plane and material (prefetch), old E, psi, pole state
+    if (k + 1 < z1) load_plane(k + 1, nHx, nHy, nHz, nxHy, nxHz, id_n, cbd_n);
+    bool st[RY][3];
+    T ev[RY][3];
+    T ps[RY][3][2];
+    int aq[RY][3];
+    Pair<T> pp[RY][3][NP > 0 ? NP : 1];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int j = j0 + r;
+      const unsigned cell = cell32(nx, ny, i, j, k);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        st[r][c] = inx && j < g.n[1] && in_box(g.rE[c], i, j, k);
+        ev[r][c] = st[r][c] ? E[(unsigned)c * N + cell] : zero<T>();
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const int ax = (c + 1 + s) % 3;
+          const float4 pr = ax == 0 ? px : (ax == 1 ? py[r] : pz);
+          const int p = (int)pr.w;
+          ps[r][c][s] = zero<T>();
+          if (st[r][c] && p >= 0)
+            ps[r][c][s] = ((const T*)a.pml[ax].psi[c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)];
+        }
+        aq[r][c] = -1;
+        if (NP > 0) {
+          // D-form node: its slot in the compact pole arrays (chunk base + ballot rank in the wavefront)
+          const bool dsp = st[r][c] && coef(r, c) < 0.f;
+          const unsigned long long m = __ballot(dsp);
+          if (m) {
+            const int base = ld_uniform(a.chunk_base, (((unsigned)c * g.n[2] + (unsigned)k) * ny + (unsigned)j) * a.nchunk + tx);
+            if (dsp) aq[r][c] = base + __popcll(m & below);
+          }
+#pragma unroll
+          for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
+            pp[r][c][p] = Pair<T>{zero<T>(), zero<T>()};
+            if (aq[r][c] >= 0 && p < a.max_poles) pp[r][c][p] = P[(unsigned)p * (unsigned)a.n_ade + (unsigned)aq[r][c]];
+          }
+        }
+      }
+    }
+    // ---- compute
     T hzm[RY], hym[RY];
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
@@ -255,86 +406,94 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
     }
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
-      const int j = j0 + r;
-      const bool row = inx && j < g.n[1];
-      const long long cell = cidx(g, i, row ? j : 0, k);
-      uint32_t ids = 0;
-      if (MAT == 1 && row) ids = a.mid[cell];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const bool st = row && i >= g.rE[c][0][0] && i < g.rE[c][0][1] && j >= g.rE[c][1][0] &&
-                        j < g.rE[c][1][1] && k >= g.rE[c][2][0] && k < g.rE[c][2][1];
-        float cbv = 0.f;
-        T dE = zero<T>(), eold = zero<T>();
-        if (st) {
-          cbv = load_cb<MAT>(a, tab, c, cell, ids);
-          const float cb = cbv < 0.f ? g.cdx : cbv;
-          T da, db;
-          int d1, d2;
-          if (c == 0) {
-            da = sub(cHz[r + 1], cHz[r]);
-            db = sub(cHy[r], pHy[r]);
-            d1 = 1;
-            d2 = 2;
-          } else if (c == 1) {
-            da = sub(cHx[r + 1], pHx[r]);
-            db = sub(cHz[r + 1], hzm[r]);
-            d1 = 2;
-            d2 = 0;
-          } else {
-            da = sub(cHy[r], hym[r]);
-            db = sub(cHx[r + 1], cHx[r]);
-            d1 = 0;
-            d2 = 1;
-          }
-          dE = scl(cb, sub(scl(g.s[d1], da), scl(g.s[d2], db)));
-          // CPML terms in ascending axis order (the reference's term list order)
+        T da, db;
+        if (c == 0) {
+          da = sub(cHz[r + 1], cHz[r]);
+          db = sub(cHy[r], pHy[r]);
+        } else if (c == 1) {
+          da = sub(cHx[r + 1], pHx[r]);
+          db = sub(cHz[r + 1], hzm[r]);
+        } else {
+          da = sub(cHy[r], hym[r]);
+          db = sub(cHx[r + 1], cHx[r]);
+        }
+        const int d1 = (c + 1) % 3, d2 = (c + 2) % 3;
+        const float cbv = coef(r, c);
+        const float cb = cbv < 0.f ? g.cdx : cbv;
+        T dE = scl(cb, sub(scl(g.s[d1], da), scl(g.s[d2], db)));
+        const int first = d1 < d2 ? 0 : 1;
 #pragma unroll
-          for (int ax = 0; ax < 3; ++ax) {
-            if (ax == c || !a.pml[ax].prof) continue;
-            const int pos = ax == 0 ? i : (ax == 1 ? j : k);
-            if (ax == d1) dE = pml_term(a.pml[ax], c, ax, pos, i, j, k, g, da, cb * g.s[d1], dE);
-            else dE = pml_term(a.pml[ax], c, ax, pos, i, j, k, g, db, -cb * g.s[d2], dE);
-          }
-          eold = E[c * N + cell];
+        for (int q = 0; q < 2; ++q) {
+          const int s = q == 0 ? first : 1 - first;
+          const int ax = (c + 1 + s) % 3;
+          const float4 pr = ax == 0 ? px : (ax == 1 ? py[r] : pz);
+          if (a.pml[ax].prof && (int)pr.w >= 0)
+            dE = add(dE, pml_apply(pr, a.pml[ax].has_k, s == 0 ? cb * g.s[d1] : -cb * g.s[d2], s == 0 ? da : db,
+                                   ps[r][c][s]));
         }
-        const bool dsp = st && cbv < 0.f;
-        const unsigned long long m = __ballot(dsp);
+        const T eold = ev[r][c];
         T enew = add(eold, dE);
-        if (m) {
-          if (dsp) {
-            const long long base = a.chunk_base[(((long long)c * g.n[2] + k) * g.n[1] + j) * a.nchunk + tx];
-            const long long q = base + __popcll(m & below);
-            const int med = (int)(-cbv) - 1;
-            const float* mp = a.media + med * a.media_stride;
-            const float ie = mp[0];
-            const int np = (int)mp[1];
-            T* D = (T*)a.D;
-            Pair<T>* P = (Pair<T>*)a.P;
-            const T Dn = add(D[q], dE);
-            T tot = zero<T>();
-            for (int p = 0; p < a.max_poles; ++p) {
-              if (p < np) {
-                const float c1 = mp[2 + 3 * p], c2 = mp[3 + 3 * p], c3 = mp[4 + 3 * p];
-                Pair<T> pp = P[(long long)p * a.n_ade + q];
-                const T pn = add(add(scl(c1, pp.p), scl(c2, pp.q)), scl(c3, eold));
-                pp.q = pp.p;
-                pp.p = pn;
-                P[(long long)p * a.n_ade + q] = pp;
-                tot = (p == 0) ? pn : add(tot, pn);
-              }
+        if (NP > 0 && aq[r][c] >= 0) {
+          const float* mp = med_tab + ((int)(-cbv) - 1) * a.media_stride;
+          const int np = (int)mp[1];
+          T dp = zero<T>();  // sum of the poles' increments
+#pragma unroll
+          for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
+            if (p < np) {
+              const T pn = add(add(scl(mp[2 + 3 * p], pp[r][c][p].p), scl(mp[3 + 3 * p], pp[r][c][p].q)),
+                               scl(mp[4 + 3 * p], eold));
+              dp = add(dp, sub(pn, pp[r][c][p].p));
+              pp[r][c][p].q = pp[r][c][p].p;
+              pp[r][c][p].p = pn;
             }
-            D[q] = Dn;
-            enew = scl(ie, sub(Dn, tot));
           }
+          enew = add(eold, scl(mp[0], sub(dE, dp)));
         }
-        if (st) E[c * N + cell] = enew;
+        ev[r][c] = enew;
+      }
+    }
+    // ---- stores
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int j = j0 + r;
+      const unsigned cell = cell32(nx, ny, i, j, k);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (!st[r][c]) continue;
+        E[(unsigned)c * N + cell] = ev[r][c];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const int ax = (c + 1 + s) % 3;
+          const float4 pr = ax == 0 ? px : (ax == 1 ? py[r] : pz);
+          const int p = (int)pr.w;
+          if (p >= 0) ((T*)a.pml[ax].psi[c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)] = ps[r][c][s];
+        }
+        if (NP > 0 && aq[r][c] >= 0) {
+#pragma unroll
+          for (int p = 0; p < (NP > 0 ? NP : 1); ++p)
+            if (p < a.max_poles) P[(unsigned)p * (unsigned)a.n_ade + (unsigned)aq[r][c]] = pp[r][c][p];
+        }
       }
     }
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       pHx[r] = cHx[r + 1];
       pHy[r] = cHy[r];
+      id_c[r] = id_n[r];
+#pragma unroll
+      for (int c = 0; c < (MAT == 0 ? 3 : 1); ++c) cbd_c[r][c] = cbd_n[r][c];
+    }
+#pragma unroll
+    for (int r = 0; r <= RY; ++r) {
+      cHx[r] = nHx[r];
+      cHz[r] = nHz[r];
+      if (r < RY) {
+        cHy[r] = nHy[r];
+        xHy[r] = nxHy[r];
+        xHz[r] = nxHz[r];
+      }
     }
   }
 }

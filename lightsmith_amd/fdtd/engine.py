@@ -130,6 +130,89 @@ def cpml_profile(spec: PMLSpec, n_lo: int, n_hi: int, N: int, h: float, dt: floa
     return b, c, kap
 
 
+def to_native(flat, shape) -> np.ndarray:
+    """Native flat index (c*N + (k*N1 + j)*N0 + i) of reference flat indices ((c*N0 + i)*N1 + j)*N2 + k."""
+    n0, n1, n2 = shape
+    flat = np.asarray(flat, dtype=np.int64)
+    k = flat % n2
+    rest = flat // n2
+    j = rest % n1
+    rest //= n1
+    i = rest % n0
+    c = rest // n0
+    return c * (n0 * n1 * n2) + (k * n1 + j) * n0 + i
+
+
+def to_reference(native, shape) -> np.ndarray:
+    """Inverse of to_native."""
+    n0, n1, n2 = shape
+    N = n0 * n1 * n2
+    native = np.asarray(native, dtype=np.int64)
+    c, cell = np.divmod(native, N)
+    i = cell % n0
+    rest = cell // n0
+    j = rest % n1
+    k = rest // n1
+    return ((c * n0 + i) * n1 + j) * n2 + k
+
+
+def ade_chunk_base(nodes, shape) -> np.ndarray:
+    """First compact slot of each 64-cell chunk (c, k, j, x // 64) for D-form nodes sorted by native index: the
+    E kernel finds a node's slot as chunk_base + (D-form lanes below it in its wavefront)."""
+    n0, n1, n2 = shape
+    N = n0 * n1 * n2
+    nchunk = (n0 + 63) // 64
+    c, cell = np.divmod(np.asarray(nodes, dtype=np.int64), N)
+    keys = (c * (n1 * n2) + cell // n0) * nchunk + (cell % n0) // 64
+    return np.searchsorted(keys, np.arange(3 * n1 * n2 * nchunk), side="left").astype(np.int32)
+
+
+def encode_material(cb: np.ndarray):
+    """(ids uint32 [N], table float32 [256]) with one 8-bit table index per component in bytes 0..2, or None
+    when the (3, N) coefficients hold more than 256 distinct values."""
+    flat = cb.reshape(-1)
+    try:
+        import pandas as pd  # hash-based unique: O(N)
+        uniq = pd.unique(flat)
+    except ImportError:  # pragma: no cover
+        uniq = np.unique(flat)
+    if len(uniq) > 256:
+        return None, len(uniq)
+    table = np.sort(np.asarray(uniq, dtype=np.float32))
+    ids = np.searchsorted(table, cb).astype(np.uint32)
+    packed = ids[0] | (ids[1] << 8) | (ids[2] << 16)
+    tab = np.zeros(256, np.float32)
+    tab[:len(table)] = table
+    return (packed.astype(np.uint32), tab), len(uniq)
+
+
+def pml_axis_profile(spec, shape, dt: float, spacing, d: int, field_: int):
+    """(profile float64 [N_d, 4] = {b, c, 1/kappa, slab index or -1}, slab extent, has_kappa) of the CPML on
+    axis d for E (field_ 0: stepped nodes 1..N-2 along d) or H (1: half-integer positions 0..N-2), the slab
+    ranges of engine.py:388-395."""
+    n_lo, n_hi = (int(v) for v in spec.cells[d])
+    N, h = shape[d], spacing[d]
+    start, stop, half = (1, N - 1, 0.0) if field_ == 0 else (0, N - 1, 0.5)
+    ranges = []
+    if n_lo:
+        ranges.append((start, min(stop, n_lo + 1)))
+    if n_hi:
+        ranges.append((max(start, N - 2 - n_hi), stop))
+    prof = np.zeros((N, 4))
+    prof[:, 0], prof[:, 2], prof[:, 3] = 1.0, 1.0, -1.0
+    ns, has_k = 0, False
+    for a, b in ranges:
+        if b <= a:
+            continue
+        u = np.arange(a, b) + half
+        bb, cc, kk = cpml_profile(spec, n_lo, n_hi, N, h, dt, u)
+        prof[a:b, 0], prof[a:b, 1], prof[a:b, 2] = bb, cc, 1.0 / kk
+        prof[a:b, 3] = ns + np.arange(b - a)
+        has_k |= bool(np.any(kk > 1))
+        ns += b - a
+    return prof, ns, has_k
+
+
 def _device_of(device) -> torch.device:
     if hasattr(device, "native_device"):
         device = device.native_device()
@@ -145,7 +228,7 @@ class Engine:
     """FDTD engine whose stepping runs in the HIP library (engine.py:230-315 constructor contract)."""
 
     # launch tiling (rows per lane, waves per block, z planes per block); see lsf_set_tiling
-    tiling = (2, 4, 16)
+    tiling = (1, 8, 16)
 
     def __init__(self, device=None, *, shape, dx, dt: float, single: bool = True, inv_eps: np.ndarray,
                  groups=(), pml: PMLSpec | None = None, sources=(), monitors=(), dft_stride: int = 1,
@@ -241,26 +324,11 @@ class Engine:
     # --- helpers ---------------------------------------------------------------------------
     def native_index(self, flat) -> np.ndarray:
         """Native flat index of reference flat indices into the stacked (3, N0, N1, N2) arrays."""
-        n0, n1, n2 = self.shape
-        flat = np.asarray(flat, dtype=np.int64)
-        k = flat % n2
-        rest = flat // n2
-        j = rest % n1
-        rest //= n1
-        i = rest % n0
-        c = rest // n0
-        return c * self.N + (k * n1 + j) * n0 + i
+        return to_native(flat, self.shape)
 
     def reference_index(self, native) -> np.ndarray:
         """Inverse of native_index."""
-        n0, n1, n2 = self.shape
-        native = np.asarray(native, dtype=np.int64)
-        c, cell = np.divmod(native, self.N)
-        i = cell % n0
-        rest = cell // n0
-        j = rest % n1
-        k = rest // n1
-        return ((c * n0 + i) * n1 + j) * n2 + k
+        return to_reference(native, self.shape)
 
     def to_device(self, a, dtype=None):
         """Upload a NumPy array (kept alive for the engine's lifetime)."""
@@ -339,42 +407,35 @@ class Engine:
         if not n:
             return
         max_p = max(len(m.poles) for m in self.ade_media)
+        if max_p > 4:
+            raise ValueError("pure D-form media may have at most 4 poles on the HIP engine")
         self.ade_max_poles = max_p
         tab = np.zeros((len(self.ade_media), 2 + 3 * max_p))
         for q, m in enumerate(self.ade_media):
             tab[q, 0], tab[q, 1] = 1.0 / m.eps_inf, len(m.poles)
             for p, pole in enumerate(m.poles):
                 tab[q, 2 + 3 * p:5 + 3 * p] = pole_coefficients(pole, self.dt)
-        n0, n1, n2 = self.shape
-        nchunk = (n0 + 63) // 64
-        c, cell = np.divmod(self.ade_nodes, self.N)
-        i = cell % n0
-        row = cell // n0  # k * n1 + j
-        keys = (c * (n1 * n2) + row) * nchunk + i // 64
-        chunk_base = np.searchsorted(keys, np.arange(3 * n1 * n2 * nchunk), side="left").astype(np.int32)
-        self._ade_D = self.zeros(n)
+        chunk_base = ade_chunk_base(self.ade_nodes, self.shape)
         self._ade_P = torch.zeros(max_p * n * 2, dtype=self.fdt, device=self._dev)
         self._keep.append(self._ade_P)
         self._ade_tab = self.to_device(tab.astype(np.float32))
         self._ade_chunk = self.to_device(chunk_base)
         self._ade_inv_eps_inf = np.array([1.0 / m.eps_inf for m in self.ade_media])
-        check(self.lib.lsf_set_ade(self._h, ptr(self._ade_chunk), ptr(self._ade_D), ptr(self._ade_P), n, max_p,
-                                   ptr(self._ade_tab), len(self.ade_media)), "lsf_set_ade")
+        check(self.lib.lsf_set_ade(self._h, ptr(self._ade_chunk), ptr(self._ade_P), n, max_p, ptr(self._ade_tab),
+                                   len(self.ade_media)), "lsf_set_ade")
 
-    def ade_redirect(self, native_targets):
-        """(ade_idx, ade_scale) device arrays for E corrections at native_targets (None if no target is a
-        pure D-form node)."""
+    def correction_scale(self, native_targets):
+        """Per-target factor of E corrections (device array, or None when every factor is 1): 1/eps_inf at pure
+        D-form nodes, where the reference's correction enters through D before the group update."""
         if not self.n_ade:
-            return None, None
+            return None
         t = np.asarray(native_targets, dtype=np.int64)
         pos = np.searchsorted(self.ade_nodes, t)
         pos_c = np.minimum(pos, self.n_ade - 1)
         hit = (pos < self.n_ade) & (self.ade_nodes[pos_c] == t)
         if not hit.any():
-            return None, None
-        idx = np.where(hit, pos_c, -1).astype(np.int32)
-        scale = np.where(hit, self._ade_inv_eps_inf[self.ade_medium[pos_c]], 0.0).astype(np.float32)
-        return self.to_device(idx), self.to_device(scale)
+            return None
+        return self.to_device(np.where(hit, self._ade_inv_eps_inf[self.ade_medium[pos_c]], 1.0).astype(np.float32))
 
     def _set_material(self, inv_eps, mode: str):
         """E coefficients dt/dx * inv_eps per node (engine.py:299), D-form nodes encoded -(medium + 1).
@@ -388,24 +449,16 @@ class Engine:
             flat[self.ade_nodes] = -(self.ade_medium.astype(np.float32) + 1.0)
         self.material_mode = "dense"
         if mode in ("auto", "indexed"):
-            try:
-                import pandas as pd  # hash-based unique: O(N)
-                uniq = pd.unique(flat)
-            except ImportError:  # pragma: no cover
-                uniq = np.unique(flat)
-            if len(uniq) <= 256:
-                table = np.sort(np.asarray(uniq, dtype=np.float32))
-                ids = np.searchsorted(table, cb).astype(np.uint32)  # (3, N)
-                packed = ids[0] | (ids[1] << 8) | (ids[2] << 16)
-                tab = np.zeros(256, np.float32)
-                tab[:len(table)] = table
-                self._mat_ids = self.to_device(packed.astype(np.uint32))
+            enc, n_uniq = encode_material(cb)
+            if enc is not None:
+                packed, tab = enc
+                self._mat_ids = self.to_device(packed)
                 self._mat_tab = self.to_device(tab)
                 check(self.lib.lsf_set_material_indexed(self._h, ptr(self._mat_ids), ptr(self._mat_tab)))
                 self.material_mode = "indexed"
                 return
             if mode == "indexed":
-                raise ValueError(f"{len(uniq)} distinct material coefficients: more than an 8-bit table holds")
+                raise ValueError(f"{n_uniq} distinct material coefficients: more than an 8-bit table holds")
         self._mat = self.to_device(flat.astype(np.float32))
         check(self.lib.lsf_set_material_dense(self._h, ptr(self._mat)))
 
@@ -419,24 +472,8 @@ class Engine:
                 continue
             if self.periodic[d]:
                 raise ValueError("a periodic axis cannot have a PML")
-            N, h = self.shape[d], self.spacing[d]
-            for fld, (start, stop), half in ((0, (1, N - 1), 0.0), (1, (0, N - 1), 0.5)):
-                ranges = []
-                if n_lo:
-                    ranges.append((start, min(stop, n_lo + 1)))
-                if n_hi:
-                    ranges.append((max(start, N - 2 - n_hi), stop))
-                ranges = [(a, b) for a, b in ranges if b > a]
-                prof = np.zeros((N, 4), np.float64)
-                prof[:, 0], prof[:, 2], prof[:, 3] = 1.0, 1.0, -1.0
-                ns, has_k = 0, False
-                for a, b in ranges:
-                    u = np.arange(a, b) + half
-                    bb, cc, kk = cpml_profile(spec, n_lo, n_hi, N, h, self.dt, u)
-                    prof[a:b, 0], prof[a:b, 1], prof[a:b, 2] = bb, cc, 1.0 / kk
-                    prof[a:b, 3] = ns + np.arange(b - a)
-                    has_k |= bool(np.any(kk > 1))
-                    ns += b - a
+            for fld in (0, 1):
+                prof, ns, has_k = pml_axis_profile(spec, self.shape, self.dt, self.spacing, d, fld)
                 if not ns:
                     continue
                 others = [self.shape[a] for a in range(3) if a != d]
@@ -548,7 +585,7 @@ class Engine:
         src = self.native_index(hh.reshape(-1))
         self._od = [self.to_device(tgt), self.to_device(src), self.field_values(ww.reshape(-1))]
         check(self.lib.lsf_add_gather(self._h, PH_OFFDIAG, 0, len(tgt), 4 * width, ptr(self._od[0]),
-                                      ptr(self.H_flat), ptr(self._od[1]), ptr(self._od[2]), None, None),
+                                      ptr(self.H_flat), ptr(self._od[1]), ptr(self._od[2]), None),
               "lsf_add_gather (off-diagonal)")
 
     # --- sparse work used by sources ---------------------------------------------------------------
@@ -571,10 +608,10 @@ class Engine:
         fld = 0 if field_ == "E" else 1
         self.check_index(uniq, 3 * self.N, "gather target")
         self.check_index(S, src_tensor.numel(), "gather source")
-        ai, asc = self.ade_redirect(uniq) if fld == 0 else (None, None)
+        sc = self.correction_scale(uniq) if fld == 0 else None
         arrs = [self.to_device(uniq), self.to_device(S.reshape(-1)), self.field_values(W.reshape(-1))]
         check(self.lib.lsf_add_gather(self._h, phase, fld, len(uniq), width, ptr(arrs[0]), ptr(src_tensor),
-                                      ptr(arrs[1]), ptr(arrs[2]), ptr(ai), ptr(asc)), "lsf_add_gather")
+                                      ptr(arrs[1]), ptr(arrs[2]), ptr(sc)), "lsf_add_gather")
 
     def add_modal(self, phase: int, field_: str, targets, m, table):
         """F[target[r]] += sum_q m[r, q] table[n, q] (targets must be distinct)."""
@@ -585,10 +622,10 @@ class Engine:
         table = np.asarray(table, dtype=np.float64).reshape(-1, m.shape[1])
         self.check_index(targets, 3 * self.N, "modal target")
         fld = 0 if field_ == "E" else 1
-        ai, asc = self.ade_redirect(targets) if fld == 0 else (None, None)
+        sc = self.correction_scale(targets) if fld == 0 else None
         arrs = [self.to_device(targets), self.field_values(m.reshape(-1)), self.to_device(table.astype(np.float32))]
         check(self.lib.lsf_add_modal(self._h, phase, fld, len(targets), m.shape[1], ptr(arrs[0]), ptr(arrs[1]),
-                                     ptr(arrs[2]), table.shape[0], ptr(ai), ptr(asc)), "lsf_add_modal")
+                                     ptr(arrs[2]), table.shape[0], ptr(sc)), "lsf_add_modal")
 
     def add_modal_to(self, phase: int, field_: str, targets, m, table):
         self.add_modal(phase, field_, targets, m, table)
@@ -604,13 +641,13 @@ class Engine:
         if base.size and (base.min() < 0 or i0.min() < 0 or base.max() + int(cmax) + 6 > T.numel()):
             raise ValueError("table source: reads outside the tables")
         fld = 0 if field_ == "E" else 1
-        ai, asc = self.ade_redirect(uniq) if fld == 0 else (None, None)
+        sc = self.correction_scale(uniq) if fld == 0 else None
         arrs = [self.to_device(uniq), self.to_device(row_ptr), T,
                 self.to_device(np.asarray(base, np.int64)[order]), self.to_device(np.asarray(i0, np.int64)[order]),
                 self.to_device(np.asarray(u0, np.float32)[order]), self.to_device(np.asarray(coef, np.float32)[order])]
         check(self.lib.lsf_add_table(self._h, phase, fld, len(uniq), ptr(arrs[0]), ptr(arrs[1]), ptr(arrs[2]), int(m),
-                                     int(cmax), ptr(arrs[3]), ptr(arrs[4]), ptr(arrs[5]), ptr(arrs[6]), ptr(ai),
-                                     ptr(asc)), "lsf_add_table")
+                                     int(cmax), ptr(arrs[3]), ptr(arrs[4]), ptr(arrs[5]), ptr(arrs[6]), ptr(sc)),
+              "lsf_add_table")
 
     # --- monitors (engine.py:556-592) ----------------------------------------------------------
     def _bind_monitor(self, m: DFTMonitor):
@@ -718,7 +755,8 @@ class Engine:
     # --- diagnostics -------------------------------------------------------------------------------
     def bytes_per_step(self) -> dict:
         """Algorithmic HBM bytes of one step: fields read + written, 3 float32 coefficients per cell,
-        psi read + written, D-form state (D, E_prev, P^n, P^{n-1} per pole) read + written."""
+        psi read + written, D-form state (D, E_prev, P^n, P^{n-1} per pole) read + written, as counted for
+        the reference's update; the kernels keep neither D nor E_prev, so they move less than this."""
         s = 8 if self.complex else 4
         n_ade = self.n_ade
         p = getattr(self, "ade_max_poles", 0)

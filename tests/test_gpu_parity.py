@@ -37,7 +37,7 @@ def test_pec_box_random_fields(gpu):
     _compare(o, g, "PEC box 60 steps")
 
 
-@pytest.mark.parametrize("tiling", [(1, 2, 5), (2, 4, 16), (4, 8, 7)])
+@pytest.mark.parametrize("tiling", [(1, 4, 5), (2, 4, 16), (2, 8, 7), (1, 8, 3)])
 def test_tilings_agree(gpu, tiling):
     shape = (67, 45, 23)
     eps = np.random.default_rng(3).uniform(1, 4, size=(3,) + shape)

@@ -1,0 +1,290 @@
+"""What pins the FP64 oracle (CPU): closed forms, invariants and brute force, each chosen so that a
+plausible mistake (dropped term, wrong sign or index, transposed operand) fails it. The reference
+ships no golden vectors for this path (its tests compare against Mie/TMM/closed forms), so these
+checks stand in for them.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import fdtd, media, sources
+
+DT = 0.95 / np.sqrt(3)
+
+
+def _energy(eng, eps, h_old):
+    return eps * np.sum(eng.E.real ** 2) + np.sum(h_old.real * eng.H.real)
+
+
+def test_pec_box_conserves_discrete_energy():
+    """Lossless PEC box: W = eps E^n.E^n + H^{n-1/2}.H^{n+1/2} is conserved to rounding (the restatement of
+    test_fdtd_graded.py:200-237 on a uniform grid): the two curls are adjoint, so a wrong index or sign
+    in either curl breaks it."""
+    shape, eps = (11, 13, 12), 2.25
+    eng = fdtd.Engine(shape=shape, dx=1.0, dt=DT, inv_eps=np.full((3,) + shape, 1 / eps))
+    rng = np.random.default_rng(1)
+    for c in range(3):
+        eng.E[c][eng.RE[c]] = rng.standard_normal(eng.E[c][eng.RE[c]].shape)
+        eng.H[c][eng.RH[c]] = rng.standard_normal(eng.H[c][eng.RH[c]].shape)
+    W = []
+    for _ in range(1500):
+        h_old = eng.H.copy()
+        eng.update_H()
+        W.append(_energy(eng, eps, h_old))
+        eng.update_E()
+    W = np.asarray(W)
+    assert np.abs(W / W[0] - 1).max() < 1e-10
+
+
+def _recurrence_residual(series, cos_wdt):
+    s = np.asarray(series)
+    r = s[2:] - 2 * cos_wdt * s[1:-1] + s[:-2]
+    return np.abs(r).max() / np.abs(s).max()
+
+
+def test_pec_cavity_mode_has_the_discrete_eigenfrequency():
+    """TM110 mode E_z = sin(pi i / (n0-1)) sin(pi j / (n1-1)) of a PEC box is an exact eigenmode of the Yee
+    update: E^{n+1} - 2 cos(w dt) E^n + E^{n-1} = 0 with sin^2(w dt/2) = dt^2/(4 eps) (Kx^2 + Ky^2),
+    K = 2 sin(k d / 2) / d."""
+    shape, eps, dx, dt = (17, 13, 6), 2.0, 1.5, 0.9 * 1.5 / np.sqrt(3)
+    eng = fdtd.Engine(shape=shape, dx=dx, dt=dt, inv_eps=np.full((3,) + shape, 1 / eps))
+    i, j = np.arange(shape[0]), np.arange(shape[1])
+    mode = np.outer(np.sin(np.pi * i / (shape[0] - 1)), np.sin(np.pi * j / (shape[1] - 1)))
+    eng.E[2][:, :, :-1] = mode[:, :, None]
+    K2 = sum((2 * np.sin(np.pi / (2 * (n - 1))) / dx) ** 2 for n in shape[:2])
+    s2 = dt ** 2 / (4 * eps) * K2
+    cos_wdt = 1 - 2 * s2
+    series = []
+    for _ in range(300):
+        series.append(eng.E[2][5, 4, 2])
+        eng.step()
+    assert _recurrence_residual(series, cos_wdt) < 1e-12
+    assert abs(_recurrence_residual(series, cos_wdt * (1 + 1e-4))) > 1e-6  # sensitive to the frequency
+
+
+def test_bloch_mode_has_the_discrete_eigenfrequency():
+    """A Bloch plane wave E_y = exp(i k x) in a cell periodic along x (Bloch k), y and z: its ghost planes
+    carry exp(+-i k L) (engine.py:595-611), so it is an exact eigenmode with
+    sin^2(w dt/2) = dt^2/eps (sin(k dx/2)/dx)^2. A wrong ghost phase sign makes it no eigenmode."""
+    shape, eps, dx = (12, 5, 5), 1.7, 1.0
+    k = 0.37
+    dt = 0.9 / np.sqrt(3)
+    eng = fdtd.Engine(shape=shape, dx=dx, dt=dt, inv_eps=np.full((3,) + shape, 1 / eps),
+                      periodic=(True, True, True), bloch=(k, 0.0, 0.0))
+    x = np.arange(shape[0]) * dx
+    eng.E[1][:] = np.exp(1j * k * x)[:, None, None]
+    s2 = dt ** 2 / eps * (np.sin(k * dx / 2) / dx) ** 2
+    series = []
+    for _ in range(200):
+        series.append(eng.E[1][4, 2, 2])
+        eng.step()
+    assert _recurrence_residual(series, 1 - 2 * s2) < 1e-12
+    bad = fdtd.Engine(shape=shape, dx=dx, dt=dt, inv_eps=np.full((3,) + shape, 1 / eps),
+                      periodic=(True, True, True), bloch=(-k, 0.0, 0.0))
+    bad.E[1][:] = np.exp(1j * k * x)[:, None, None]
+    series = []
+    for _ in range(200):
+        series.append(bad.E[1][4, 2, 2])
+        bad.step()
+    assert _recurrence_residual(series, 1 - 2 * s2) > 1e-3
+
+
+class _Point:
+    """A one-node stand-in engine for driving an ADE group directly."""
+
+    def __init__(self, dt, n=1):
+        self.dt = dt
+        self.fdt = np.float64
+        self.E_flat = np.zeros(n)
+
+
+@pytest.mark.parametrize("poles", [
+    [(0.8, 0.0, 0.05)],                               # Drude
+    [(0.5, 0.6, 0.07), (0.3, 1.1, 0.2)],              # two Lorentz poles
+    [(0.9, 0.0, 0.04), (0.4, 0.9, 0.1), (0.2, 1.6, 0.3)],
+])
+def test_ade_realises_the_discrete_permittivity(poles):
+    """Driving a pure D-form group with D^n = Re(D0 Z^n), Z = exp(-i w dt), its E settles to Re(D0 Z^n / eps_d)
+    with eps_d = eps_inf + sum c3 / (Z - c1 - c2 / Z) (layered.py:24-31): pins c1, c2, c3 and the order
+    of the D, P and E updates (engine.py:108-111, 769-793)."""
+    m = media.Medium(1.8, tuple(media.Pole(*p) for p in poles))
+    dt, w = 0.5, 0.45
+    g = fdtd.PureGroup(_Point(dt), fdtd.DispersiveGroup(idx=np.array([0]), media=[m], frac=np.ones((1, 1)), w=None))
+    Z = np.exp(-1j * w * dt)
+    D0 = 1.0 + 0.3j
+    eng = g and _Point(dt)
+    d_prev = 0.0
+    out = []
+    n_steps = 6000
+    for n in range(1, n_steps + 1):
+        d_now = (D0 * Z ** n).real
+        eng.E_flat[0] = g.E_prev[0] + (d_now - d_prev)
+        g.step(eng)
+        d_prev = d_now
+        out.append(eng.E_flat[0])
+    eps_d = media.discrete_eps(m, w, dt)
+    n = np.arange(1, n_steps + 1)
+    expect = (D0 * Z ** n / eps_d).real
+    tail = slice(n_steps - 400, n_steps)
+    err = np.abs(np.asarray(out)[tail] - expect[tail]).max() / np.abs(expect[tail]).max()
+    assert err < 1e-8
+    # the continuous eps at the same frequency differs: the check resolves the discretisation
+    assert abs(m.eps(w) - eps_d) > 1e-4
+
+
+def test_amplification_is_bounded_at_the_chosen_dt():
+    """time_step lowers dt until one step of the Yee + ADE update is non-amplifying (setup.py:26-91;
+    test_fdtd.py:102-120)."""
+    m = media.drude_lorentz(1.0, 9.0, 0.07, [(2.0, 3.0, 0.5)])
+    for dx in (40.0, 5.0):
+        dt = media.time_step(dx, 0.95, [m])
+        assert media.amplification(m, dt, np.linspace(0, 12 / dx ** 2, 401)) <= 1 + 1e-6
+
+
+def _dipole_run(shape, pml, pos, steps, probe):
+    inv = np.ones((3,) + shape)
+    src = sources.PointDipole(shape=shape, origin=np.zeros(3), dx=1.0, dt=DT, pos=pos, pdir=(0, 0, 1.0),
+                              inv_eps=inv, omega_c=0.5, omega_half=0.25, max_steps=steps + 5)
+    eng = fdtd.Engine(shape=shape, dx=1.0, dt=DT, inv_eps=inv,
+                      pml=fdtd.PMLSpec(cells=((pml, pml),) * 3) if pml else None, sources=[src])
+    out = []
+    for _ in range(steps):
+        eng.step()
+        out.append(eng.E[2][probe])
+    return np.asarray(out)
+
+
+def test_cpml_reflection_is_small_against_a_large_domain():
+    """Brute force: a pulse from a dipole seen by a probe next to a 6-cell CPML, against the same probe in a
+    domain so large that its walls' echo has not arrived. The difference is the CPML's reflection."""
+    small, big = (30, 30, 30), (74, 74, 74)
+    steps = 150
+    a = _dipole_run(small, 6, (15.0, 15.0, 15.2), steps, (8, 15, 15))
+    b = _dipole_run(big, 0, (37.0, 37.0, 37.2), steps, (30, 37, 37))
+    rel = np.abs(a - b).max() / np.abs(b).max()
+    assert rel < 5e-3
+    # without the CPML the PEC wall reflects strongly in the same window
+    c = _dipole_run(small, 0, (15.0, 15.0, 15.2), steps, (8, 15, 15))
+    assert np.abs(c - b).max() / np.abs(b).max() > 0.1
+
+
+def test_dft_sample_phase_convention():
+    """acc_e += exp(+i w t) E(t) at t = n dt; acc_h uses t - dt/2 (engine.py:564-592). A sine over whole periods
+    sampled this way gives +i N/2 exactly when the sign and the H half step are right."""
+    N, stride = 64, 1
+    dt = 0.25
+    w0 = 2 * np.pi * 4 / (N * dt)
+    shape = (3, 3, 3)
+    eng = fdtd.Engine(shape=shape, dx=1.0, dt=dt, inv_eps=np.ones((3,) + shape))
+    mon = fdtd.DFTMonitor(np.array([w0, 2 * w0]), np.array([13]), np.array([40]), np.array([41]))
+    eng.monitors = [mon]
+    mon.acc_e = np.zeros((2, 1), complex)
+    mon.acc_h = np.zeros((2, 1), complex)
+    for n in range(1, N + 1):
+        eng.n = n
+        eng.E_flat[13] = np.sin(w0 * n * dt)
+        eng.H_flat[40] = eng.H_flat[41] = np.sin(w0 * (n - 0.5) * dt)
+        eng.sample()
+    e, h = mon.results()
+    assert abs(e[0, 0] - 0.5j * N) < 1e-9 and abs(h[0, 0] - 0.5j * N) < 1e-9
+    assert abs(e[1, 0]) < 1e-9
+
+
+def test_dipole_current_has_no_net_charge_and_unit_weights():
+    """j is the exact difference of g over omega_c dt, so sum j dt = (g(end) - g(0)) / omega_c ~ 0
+    (emission.py:120-127); the trilinear weights of each component sum to its direction cosine."""
+    shape = (20, 20, 20)
+    src = sources.PointDipole(shape=shape, origin=np.zeros(3), dx=2.0, dt=DT * 2, pos=(20.3, 19.1, 21.7),
+                              pdir=(0.6, 0.0, 0.8), inv_eps=np.ones((3,) + shape), omega_c=0.3,
+                              omega_half=0.05, max_steps=3000)
+    assert abs(np.sum(src.wave)) < 1e-9 * np.abs(src.wave).max() * len(src.wave)
+    comp = src.idx // int(np.prod(shape))
+    assert abs(src.w[comp == 0].sum() - 0.6) < 1e-12 and abs(src.w[comp == 2].sum() - 0.8) < 1e-12
+    assert not np.any(comp == 1)
+
+
+def test_tfsf_box_does_not_leak_axis_aligned():
+    """Axis-aligned incidence: the 1D line's nodes coincide with the 3D Yee positions, so the scattered-field
+    region of an empty box stays at rounding level (test_fdtd.py:179-200 is this test at solve level)."""
+    shape = (26, 26, 30)
+    inv = np.full((3,) + shape, 1 / 1.77)
+    src = sources.TFSFPlaneWave(shape=shape, origin=np.zeros(3), dx=1.0, dt=DT, r0=np.array([13.0, 13, 15]),
+                                lo=np.array([8, 8, 8]), hi=np.array([18, 18, 22]), khat=(0, 0, 1.0),
+                                ehat=(1.0, 0, 0), eps_b=1.77, omega_c=0.6, omega_half=0.2, max_steps=400)
+    eng = fdtd.Engine(shape=shape, dx=1.0, dt=DT, inv_eps=inv, pml=fdtd.PMLSpec(cells=((5, 5),) * 3, eps_ref=1.77),
+                      sources=[src])
+    peak_tot, peak_sca = 0.0, 0.0
+    inside = (slice(None), slice(10, 17), slice(10, 17), slice(10, 21))
+    out = np.ones(shape, bool)
+    out[7:20, 7:20, 7:24] = False
+    for _ in range(250):
+        eng.step()
+        peak_tot = max(peak_tot, np.abs(eng.E[inside]).max())
+        peak_sca = max(peak_sca, np.abs(eng.E[:, out]).max())
+    assert peak_tot > 0.1
+    assert peak_sca < 1e-12 * peak_tot
+
+
+def test_tfsf_oblique_leak_is_small():
+    """Oblique incidence through the interpolated 1D line leaks ~1e-3 of the incident field (tfsf.py:8-10)."""
+    shape = (26, 26, 26)
+    inv = np.ones((3,) + shape)
+    k = np.array([1.0, 2.0, 3.0]) / np.sqrt(14)
+    e = np.cross(k, [0, 1.0, 0])
+    e /= np.linalg.norm(e)
+    src = sources.TFSFPlaneWave(shape=shape, origin=np.zeros(3), dx=1.0, dt=DT, r0=np.array([13.0, 13, 13]),
+                                lo=np.array([8, 8, 8]), hi=np.array([18, 18, 18]), khat=k, ehat=e, eps_b=1.0,
+                                omega_c=0.5, omega_half=0.2, max_steps=400)
+    eng = fdtd.Engine(shape=shape, dx=1.0, dt=DT, inv_eps=inv, pml=fdtd.PMLSpec(cells=((5, 5),) * 3), sources=[src])
+    out = np.ones(shape, bool)
+    out[6:21, 6:21, 6:21] = False
+    peak_tot = peak_sca = 0.0
+    for _ in range(250):
+        eng.step()
+        peak_tot = max(peak_tot, np.abs(eng.E[:, 10:17, 10:17, 10:17]).max())
+        peak_sca = max(peak_sca, np.abs(eng.E[:, out]).max())
+    assert peak_tot > 0.1
+    assert 1e-8 < peak_sca / peak_tot < 2e-2
+
+
+def test_bloch_plane_source_does_not_leak():
+    """The auxiliary Bloch column satisfies the main grid's discrete equations exactly (tfsf.py:214-222), so
+    the scattered-field side of the plane stays at rounding level at any k_par and polarisation."""
+    nz = 60
+    shape = (8, 7, nz)
+    sp = (20.0, 24.0, 15.0)
+    kpar = (0.004, 0.002)
+    dt = 0.9 / np.sqrt(sum(1 / d ** 2 for d in sp))
+    pml = fdtd.PMLSpec(cells=((0, 0), (0, 0), (8, 8)), eps_ref=1.0)
+    for pol in ("s", "p"):
+        src = sources.BlochPlaneSource(shape=shape, spacing=sp, dt=dt, k_src=25, k_launch=24, kpar=kpar, pol=pol,
+                                       eps_b=1.0, pml=pml, omega_c=2 * np.pi / 600, omega_half=0.3 * 2 * np.pi / 600,
+                                       max_steps=900)
+        eng = fdtd.Engine(shape=shape, dx=sp, dt=dt, inv_eps=np.ones((3,) + shape), pml=pml, sources=[src],
+                          periodic=(True, True, False), bloch=(*kpar, 0.0))
+        peak_tot = peak_sca = 0.0
+        for _ in range(500):
+            eng.step()
+            peak_tot = max(peak_tot, np.abs(eng.E[:, 1:-1, 1:-1, 27:50]).max())
+            peak_sca = max(peak_sca, np.abs(eng.E[:, 1:-1, 1:-1, 9:23]).max())
+        assert peak_tot > 1e-3
+        assert peak_sca < 1e-11 * peak_tot, pol
+
+
+def test_mixed_group_reduces_to_pure():
+    """One dispersive medium at fraction 1: both paths of the mixed update are the pure update, whatever w
+    (engine.py:722-728)."""
+    m = media.Medium(2.0, (media.Pole(0.5, 0.4, 0.05), media.Pole(0.2, 1.3, 0.1)))
+    n = 5
+    w = np.linspace(0, 1, n)
+    rng = np.random.default_rng(0)
+    drive = rng.standard_normal((200, n))
+    pure = fdtd.PureGroup(_Point(0.5, n), fdtd.DispersiveGroup(idx=np.arange(n), media=[m], frac=np.ones((1, n)), w=None))
+    mixed = fdtd.MixedGroup(_Point(0.5, n), fdtd.DispersiveGroup(idx=np.arange(n), media=[m], frac=np.ones((1, n)), w=w))
+    ea, eb = _Point(0.5, n), _Point(0.5, n)
+    for row in drive:
+        ea.E_flat[:] = pure.E_prev + row
+        eb.E_flat[:] = mixed.E_prev + row
+        pure.step(ea)
+        mixed.step(eb)
+    assert np.allclose(ea.E_flat, eb.E_flat, rtol=1e-12, atol=1e-14)
