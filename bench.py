@@ -98,6 +98,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=4, help="timed steps of the CPU baseline (0: skip)")
     ap.add_argument("--tiling", default=None, help="rows_per_lane,waves,z_chunk")
     ap.add_argument("--no-roofline-pass", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="launch each chunk's kernels directly instead of a HIP graph")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB (for scaling rocprof FETCH/WRITE counters)")
     args = ap.parse_args()
@@ -135,7 +136,7 @@ def main() -> int:
     log(f"set-up {time.perf_counter() - t_setup:.1f} s: {scene['shape']} cells, dt {scene['dt']:.4f} nm/c, "
         f"stride {stride}, {eng.n_ade} D-form nodes, {eng.n_psi} psi values, material {eng.material_mode}")
 
-    run, used = eng.make_runner(True)
+    run, used = eng.make_runner(not args.eager)
     for _ in range(W // stride):
         run()
     torch.cuda.synchronize()
@@ -153,6 +154,7 @@ def main() -> int:
     ms_ev = start.elapsed_time(stop)
     eng.release()
     elapsed = max(wall, ms_ev / 1e3)
+    graphs = used
     value = scene["n_cells"] * K / elapsed / 1e9
     log(f"timed {K} steps: {elapsed * 1e3:.1f} ms ({value:.2f} Gcell-updates/s)")
     energy = float(eng.energy[0])
@@ -198,7 +200,7 @@ def main() -> int:
                       "cells": scene["n_cells"], "dx_nm": scene["dx"], "dt_nm_per_c": scene["dt"],
                       "dft_stride": stride, "dispersive_nodes": int(eng.n_ade), "psi_values": int(eng.n_psi),
                       "material": eng.material_mode, "tiling": list(eng.tiling if tiling is None else tiling),
-                      "parallelism": "single GPU"},
+                      "hip_graphs": graphs, "parallelism": "single GPU"},
            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
     eng.close()

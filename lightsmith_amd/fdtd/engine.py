@@ -465,7 +465,8 @@ class Engine:
     # --- CPML (engine.py:344-418) --------------------------------------------------------------
     def _build_pml(self, spec: PMLSpec, inv_eps):
         self.pml_cells = tuple(tuple(int(v) for v in c) for c in spec.cells)
-        self.n_psi = 0
+        self.n_psi = 0  # psi values allocated (slab extent x full extent of the other two axes)
+        self.n_psi_stepped = 0  # psi values the update touches (slab extent x the component's stepped box)
         for d in range(3):
             n_lo, n_hi = self.pml_cells[d]
             if not (n_lo or n_hi):
@@ -479,10 +480,14 @@ class Engine:
                 others = [self.shape[a] for a in range(3) if a != d]
                 size = ns * others[0] * others[1]
                 psis = [None, None, None]
+                regions = self.RE if fld == 0 else self.RH
                 for c in range(3):
                     if c != d:
                         psis[c] = self.zeros(size)
                         self.n_psi += size
+                        o1, o2 = (a for a in range(3) if a != d)
+                        R = regions[c]
+                        self.n_psi_stepped += ns * (R[o1].stop - R[o1].start) * (R[o2].stop - R[o2].start)
                 tprof = self.to_device(prof.astype(np.float32))
                 check(self.lib.lsf_set_pml(self._h, fld, d, ptr(tprof), ns, ptr(psis[0]), ptr(psis[1]),
                                            ptr(psis[2]), int(has_k)), "lsf_set_pml")
@@ -638,7 +643,8 @@ class Engine:
         row_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
         self.check_index(uniq, 3 * self.N, "table target")
         base, i0 = np.asarray(base, np.int64), np.asarray(i0, np.int64)
-        if base.size and (base.min() < 0 or i0.min() < 0 or base.max() + int(cmax) + 6 > T.numel()):
+        # reads T[base + c + j], j = 0..5, with floor(i0 / m) <= c <= cmax at every step t >= 0
+        if base.size and (i0.min() < 0 or (base + i0 // int(m)).min() < 0 or base.max() + int(cmax) + 6 > T.numel()):
             raise ValueError("table source: reads outside the tables")
         fld = 0 if field_ == "E" else 1
         sc = self.correction_scale(uniq) if fld == 0 else None
@@ -761,6 +767,6 @@ class Engine:
         n_ade = self.n_ade
         p = getattr(self, "ade_max_poles", 0)
         fields = self.N * (12 * s + 12)
-        psi = self.n_psi * 2 * s
+        psi = self.n_psi_stepped * 2 * s
         ade = n_ade * (2 + 2 * p) * 2 * s
         return {"fields": fields, "psi": psi, "ade": ade, "total": fields + psi + ade}

@@ -146,3 +146,50 @@ def dispersive_box(ns, device=None, shape=(40, 36, 32), dx=2.0, seed=0, pml_cell
     if not ns.oracle:
         kw["material"] = material
     return make(ns, device, **kw)
+
+
+def tfsf_box(ns, device=None, shape=(30, 28, 34), khat=(0, 0, 1.0), ehat=(1.0, 0, 0), steps=300, particle=True):
+    """A 1D-fed TFSF box around a small dielectric block, CPML outside."""
+    dx = 1.0
+    dt = 0.95 / np.sqrt(3)
+    inv = np.full((3,) + shape, 1 / 1.77)
+    if particle:
+        inv[:, 13:18, 12:17, 15:20] = 1 / 6.0
+    k = np.asarray(khat, float)
+    k /= np.linalg.norm(k)
+    src = ns.TFSFPlaneWave(shape=shape, origin=np.zeros(3), dx=dx, dt=dt, r0=np.array([15.0, 14, 17]),
+                           lo=np.array([9, 8, 10]), hi=np.array([21, 20, 24]), khat=k, ehat=ehat, eps_b=1.77,
+                           omega_c=0.6, omega_half=0.2, max_steps=steps + 5)
+    mon = ns.DFTMonitor(np.array([0.5, 0.6]), *src.probe(), source=src)
+    return make(ns, device, shape=shape, dx=dx, dt=dt, inv_eps=inv,
+                pml=ns.PMLSpec(cells=((5, 5),) * 3, eps_ref=1.77), sources=[src], monitors=[mon], dft_stride=10)
+
+
+def synthetic_table_source(ns, shape, seed=0, m=3, n_rows=40, n_pts=None):
+    """A table-fed source over the TFSF faces of a box with random smooth tables (the per-step part only):
+    every face point of each kind reads its own row at its own delay."""
+    from oracle.sources import tfsf_slabs
+    rng = np.random.default_rng(seed)
+    L = 200
+    n_c = L - 8
+    t = np.arange(n_c)
+    rows = np.stack([np.sin(0.07 * t * (1 + 0.1 * q)) * np.exp(-((t - 90) / 60.0) ** 2) for q in range(n_rows)])
+    T = np.concatenate([rows, np.zeros((n_rows, 8))], axis=1)
+    cmax = n_c + 2
+    points = {}
+    for kind in ("E", "H"):
+        nodes, base, i0, u0, coef = [], [], [], [], []
+        for k_, c, region, g, cf, q in tfsf_slabs(shape, (8, 8, 9), (20, 19, 23), 0.5, 1.0):
+            if k_ != kind:
+                continue
+            ii, jj, kk = np.meshgrid(*[np.arange(r.start, r.stop) for r in region], indexing="ij")
+            nn = ns.flat_index(c, ii, jj, kk, shape).reshape(-1)
+            nodes.append(nn)
+            r_ = rng.integers(0, n_rows, len(nn))
+            base.append(r_ * L - 2)
+            i0.append(rng.integers(6 * m, 12 * m, len(nn)))
+            u0.append(rng.uniform(0, 1, len(nn)))
+            coef.append(np.full(len(nn), cf) * rng.uniform(0.5, 1.5, len(nn)))
+        points[kind] = tuple(np.concatenate(v) for v in (nodes, base, i0, u0, coef))
+    inc = (np.arange(3) * L - 2, np.full(3, 6 * m + 1), np.full(3, 0.25))
+    return ns.TableSource(T.reshape(-1), m, cmax, points, inc=inc, T_inc=T[:3].reshape(-1))

