@@ -73,7 +73,8 @@ int lsf_set_pml(lsf_engine* eng, int field, int axis, const void* dev_profile, i
 
 /* Pure ADE groups in D-form (engine.py:722-742, 759-793). n_nodes D-form nodes in native flat order;
  * dev_chunk_base[(c*N2 + k)*N1 + j][x/64] = index of the chunk's first D-form node.
- * 1 <= max_poles <= 4; dev_P[max_poles][n_nodes] pairs {P^n, P^{n-1}} (field type x2), zeroed by the caller.
+ * 1 <= max_poles <= 4; dev_P[n_nodes][max_poles] pairs {P^n, P^{n-1}} (field type x2, a node's poles adjacent),
+ * zeroed by the caller.
  * D is not stored: D^n = eps_inf E^n + sum P^n holds after every step, so the update advances E by
  * (dE - sum (P^{n+1} - P^n)) / eps_inf. dev_media[n_media][2 + 3*max_poles] float =
  * {1/eps_inf, n_poles, (c1, c2, c3) per pole} (pole_coefficients, engine.py:108-111); n_media <= 16. */

@@ -312,6 +312,16 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
   // material of the current / next plane: packed 8-bit ids (MAT 1) or the three coefficients (MAT 0)
   uint32_t id_c[RY], id_n[RY];
   float cbd_c[RY][MAT == 0 ? 3 : 1], cbd_n[RY][MAT == 0 ? 3 : 1];
+  int ab_c[RY][3], ab_n[RY][3];  // first D-form slot of each row's 64-cell chunk per component (wave-uniform)
+  auto load_bases = [&](int k, int (*ab)[3]) {
+#pragma unroll
+    for (int r = 0; r < RY; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        ab[r][c] = (NP > 0 && j0 + r < g.n[1] && k < g.n[2])
+                       ? ld_uniform(a.chunk_base, (((unsigned)c * g.n[2] + (unsigned)k) * ny + (unsigned)(j0 + r)) * a.nchunk + tx)
+                       : 0;
+  };
 
   auto load_plane = [&](int k, T* xH, T* yH, T* zH, T* hy, T* hz, uint32_t* ids, float (*cbd)[MAT == 0 ? 3 : 1]) {
 #pragma unroll
@@ -348,12 +358,16 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
     pHy[r] = ok ? load_lo<PER>(Hy, g, i, j, k) : zero<T>();
   }
   load_plane(z0, cHx, cHy, cHz, xHy, xHz, id_c, cbd_c);
+  load_bases(z0, ab_c);
 
   Pair<T>* P = (Pair<T>*)a.P;
   for (int k = z0; k < z1; ++k) {
     const float4 pz = a.pml[2].prof ? ld_uniform(a.pml[2].prof, k) : no_pml();
     // ---- loads: next H plane and material (prefetch), old E, psi, pole state
-    if (k + 1 < z1) load_plane(k + 1, nHx, nHy, nHz, nxHy, nxHz, id_n, cbd_n);
+    if (k + 1 < z1) {
+      load_plane(k + 1, nHx, nHy, nHz, nxHy, nxHz, id_n, cbd_n);
+      load_bases(k + 1, ab_n);
+    }
     bool st[RY][3];
     T ev[RY][3];
     T ps[RY][3][2];
@@ -381,14 +395,11 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
           // D-form node: its slot in the compact pole arrays (chunk base + ballot rank in the wavefront)
           const bool dsp = st[r][c] && coef(r, c) < 0.f;
           const unsigned long long m = __ballot(dsp);
-          if (m) {
-            const int base = ld_uniform(a.chunk_base, (((unsigned)c * g.n[2] + (unsigned)k) * ny + (unsigned)j) * a.nchunk + tx);
-            if (dsp) aq[r][c] = base + __popcll(m & below);
-          }
+          if (m && dsp) aq[r][c] = ab_c[r][c] + __popcll(m & below);
 #pragma unroll
           for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
             pp[r][c][p] = Pair<T>{zero<T>(), zero<T>()};
-            if (aq[r][c] >= 0 && p < a.max_poles) pp[r][c][p] = P[(unsigned)p * (unsigned)a.n_ade + (unsigned)aq[r][c]];
+            if (aq[r][c] >= 0 && p < a.max_poles) pp[r][c][p] = P[(unsigned)aq[r][c] * (unsigned)a.max_poles + (unsigned)p];
           }
         }
       }
@@ -473,7 +484,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
         if (NP > 0 && aq[r][c] >= 0) {
 #pragma unroll
           for (int p = 0; p < (NP > 0 ? NP : 1); ++p)
-            if (p < a.max_poles) P[(unsigned)p * (unsigned)a.n_ade + (unsigned)aq[r][c]] = pp[r][c][p];
+            if (p < a.max_poles) P[(unsigned)aq[r][c] * (unsigned)a.max_poles + (unsigned)p] = pp[r][c][p];
         }
       }
     }
@@ -484,6 +495,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
       id_c[r] = id_n[r];
 #pragma unroll
       for (int c = 0; c < (MAT == 0 ? 3 : 1); ++c) cbd_c[r][c] = cbd_n[r][c];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ab_c[r][c] = ab_n[r][c];
     }
 #pragma unroll
     for (int r = 0; r <= RY; ++r) {
