@@ -96,7 +96,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=320, help="grid size per axis (the workload is 320)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="timed steps of the CPU baseline (0: skip)")
-    ap.add_argument("--tiling", default=None, help="rows_per_lane,waves,z_chunk")
+    ap.add_argument("--tiling", default=None, help="rows_per_lane,waves,x_cells,z_chunk")
     ap.add_argument("--no-roofline-pass", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch each chunk's kernels directly instead of a HIP graph")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -72,7 +72,8 @@ int lsf_set_pml(lsf_engine* eng, int field, int axis, const void* dev_profile, i
                 void* dev_psi1, void* dev_psi2, int has_kappa);
 
 /* Pure ADE groups in D-form (engine.py:722-742, 759-793). n_nodes D-form nodes in native flat order;
- * dev_chunk_base[(c*N2 + k)*N1 + j][x/64] = index of the chunk's first D-form node.
+ * dev_chunk_base[(c*N2 + k)*N1 + j][x/32] = index of the 32-cell chunk's first D-form node
+ * (ceil(N0/32) chunks per row).
  * 1 <= max_poles <= 4; dev_P[n_nodes][max_poles] pairs {P^n, P^{n-1}} (field type x2, a node's poles adjacent),
  * zeroed by the caller.
  * D is not stored: D^n = eps_inf E^n + sum P^n holds after every step, so the update advances E by
@@ -148,8 +149,9 @@ int lsf_add_time_monitor(lsf_engine* eng, int64_t n_pts, const int64_t* dev_e_id
 /* Energy proxy slots (engine.py:315 "energy"), float[2] (or a field-precision pair). */
 int lsf_set_energy(lsf_engine* eng, float* dev_energy);
 
-/* Launch tuning: rows per lane (1 or 2), waves per block (4 or 8), z planes per block (>= 1). */
-int lsf_set_tiling(lsf_engine* eng, int rows_per_lane, int waves, int z_chunk);
+/* Launch tuning: rows per lane (1 or 2), wavefronts per block (4 or 8), x cells per wavefront row (64: one
+ * row per wavefront; 32: two rows), z planes per block (>= 1). No reference counterpart (a launch choice). */
+int lsf_set_tiling(lsf_engine* eng, int rows_per_lane, int waves, int x_cells, int z_chunk);
 
 /* Run n_steps steps (Engine.step, engine.py:665-678) on `stream` (a hipStream_t; NULL = default). */
 int lsf_step(lsf_engine* eng, int n_steps, void* stream);
@@ -166,6 +168,33 @@ int lsf_chunk(lsf_engine* eng, int n_chunks, int stride, void* stream);
 int lsf_graph_capture(lsf_engine* eng, int stride, void* stream);
 int lsf_graph_replay(lsf_engine* eng, int n, void* stream);
 int lsf_graph_reset(lsf_engine* eng);
+
+/* ---- Batches: many independent engines stepped together -----------------------------------------
+ * No reference counterpart: the reference runs the angles/wavelengths/polarisations of an oblique sweep
+ * one Engine after another (periodic.py:503-519). A batch steps its members with shared launches: per
+ * half-step one bulk-kernel launch per kernel configuration over all members' tiles (one block per tile,
+ * or a persistent grid that strides over them), each phase's per-step operations fused by position over
+ * the members (gathers and modal sources one launch each, sub-engines as a nested batch), and one DFT
+ * launch per source type for all monitors. Members are fully configured top-level engines with their own
+ * step counters; while the batch exists they refuse configuration calls. Destroy the batch before its
+ * members. Each member's state evolves exactly as under lsf_step / lsf_chunk of that member alone. */
+typedef struct lsf_batch lsf_batch;
+int lsf_batch_create(lsf_engine* const* members, int n_members, lsf_batch** out);
+void lsf_batch_destroy(lsf_batch* batch);
+/* Tile shape as lsf_set_tiling, for all members' bulk kernels; grid_blocks = 0 launches one block per tile,
+ * grid_blocks > 0 a persistent grid of that many blocks (rounded up to a multiple of 8). */
+int lsf_batch_set_schedule(lsf_batch* batch, int rows_per_lane, int waves, int x_cells, int z_chunk,
+                           int grid_blocks);
+/* Tiles of one E half-step over all members under the current schedule. */
+int lsf_batch_tiles(lsf_batch* batch, int64_t* n_tiles);
+int lsf_batch_step(lsf_batch* batch, int n_steps, void* stream);
+/* As lsf_batch_step, adding the summed bulk-kernel times (ms) to ms[0] (H) and ms[1] (E). Synchronises. */
+int lsf_batch_step_timed(lsf_batch* batch, int n_steps, void* stream, double* ms);
+int lsf_batch_sample(lsf_batch* batch, void* stream);
+int lsf_batch_chunk(lsf_batch* batch, int n_chunks, int stride, void* stream);
+int lsf_batch_graph_capture(lsf_batch* batch, int stride, void* stream);
+int lsf_batch_graph_replay(lsf_batch* batch, int n, void* stream);
+int lsf_batch_graph_reset(lsf_batch* batch);
 
 /* Message of the last failing call on this thread. */
 const char* lsf_last_error(void);

@@ -44,7 +44,7 @@ _SIGNATURES = {
     "lsf_add_dft_monitor": [c_vp, c_int, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_int],
     "lsf_add_time_monitor": [c_vp, c_i64, c_vp, c_vp, c_i64, c_vp],
     "lsf_set_energy": [c_vp, c_vp],
-    "lsf_set_tiling": [c_vp, c_int, c_int, c_int],
+    "lsf_set_tiling": [c_vp, c_int, c_int, c_int, c_int],
     "lsf_step": [c_vp, c_int, c_vp],
     "lsf_step_timed": [c_vp, c_int, c_vp, ctypes.POINTER(c_dbl)],
     "lsf_sample": [c_vp, c_vp],
@@ -52,10 +52,21 @@ _SIGNATURES = {
     "lsf_graph_capture": [c_vp, c_int, c_vp],
     "lsf_graph_replay": [c_vp, c_int, c_vp],
     "lsf_graph_reset": [c_vp],
+    "lsf_batch_create": [ctypes.POINTER(c_vp), c_int, ctypes.POINTER(c_vp)],
+    "lsf_batch_destroy": [c_vp],
+    "lsf_batch_set_schedule": [c_vp, c_int, c_int, c_int, c_int, c_int],
+    "lsf_batch_tiles": [c_vp, ctypes.POINTER(c_i64)],
+    "lsf_batch_step": [c_vp, c_int, c_vp],
+    "lsf_batch_step_timed": [c_vp, c_int, c_vp, ctypes.POINTER(c_dbl)],
+    "lsf_batch_sample": [c_vp, c_vp],
+    "lsf_batch_chunk": [c_vp, c_int, c_int, c_vp],
+    "lsf_batch_graph_capture": [c_vp, c_int, c_vp],
+    "lsf_batch_graph_replay": [c_vp, c_int, c_vp],
+    "lsf_batch_graph_reset": [c_vp],
     "lsf_last_error": [],
     "lsf_version": [],
 }
-_RESTYPES = {"lsf_destroy": None, "lsf_last_error": c_char_p, "lsf_version": c_char_p}
+_RESTYPES = {"lsf_destroy": None, "lsf_batch_destroy": None, "lsf_last_error": c_char_p, "lsf_version": c_char_p}
 
 _lib = None
 

@@ -1,8 +1,12 @@
 """Build the HIP library in-tree: ``python -m lightsmith_amd.build``.
 
-One translation unit (csrc/lsf.hip) compiled by hipcc for gfx950 into lightsmith_amd/_lsf.so.
-Packed-FP32 instructions are switched off (-packed-fp32-ops): the kernels are bandwidth-bound and
-the scalar forms are the ones this project trusts.
+Translation units (compiled by hipcc for gfx950 in parallel, then linked into lightsmith_amd/_lsf.so):
+  csrc/lsf.hip     host API, batches, sparse per-step kernels
+  csrc/lsf_k_h.hip the bulk H kernel instantiations, once per field type (float, complex64)
+  csrc/lsf_k_e.hip the bulk E kernel instantiations, once per field type, material mode (dense, indexed)
+                   and periodicity
+Packed-FP32 instructions are switched off (-packed-fp32-ops): the kernels are bandwidth-bound and the
+scalar forms are the ones this project trusts.
 """
 
 from __future__ import annotations
@@ -11,12 +15,24 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "lsf.hip")
+CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_lsf.so")
 ARCH = "gfx950"
+
+# (object name, source, extra defines)
+UNITS = [("lsf", "lsf.hip", [])] + [
+    (f"k_h_{t}", "lsf_k_h.hip", [f"-DLSF_H_CPLX={1 if t == 'c' else 0}", f"-DLSF_H_ENTRY=launch_update_h_{t}"])
+    for t in ("f", "c")
+] + [
+    (f"k_e_{t}{m}{p}", "lsf_k_e.hip", [f"-DLSF_E_CPLX={1 if t == 'c' else 0}", f"-DLSF_E_MAT={m}", f"-DLSF_E_PER={p}",
+                                       f"-DLSF_E_ENTRY=launch_update_e_{t}{m}{p}"])
+    for t in ("f", "c") for m in (0, 1) for p in (0, 1)
+]
 
 
 def hipcc() -> str:
@@ -26,24 +42,45 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm is required to build lightsmith_amd)")
 
 
-def command(out: str = OUT) -> list[str]:
-    return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+def flags() -> list[str]:
+    return [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-command-line-argument",
-            "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"), SRC, "-o", out]
+            "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
 
-def build(verbose: bool = False) -> str:
+def compile_cmd(src: str, obj: str, defines: list[str]) -> list[str]:
+    return [hipcc(), *flags(), *defines, "-c", os.path.join(CSRC, src), "-o", obj]
+
+
+def link_cmd(objs: list[str], out: str) -> list[str]:
+    return [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out]
+
+
+def _run(cmd: list[str]) -> subprocess.CompletedProcess:
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
     """Compile the library (always; the build is the check that the sources compile)."""
-    tmp = OUT + ".tmp"
-    cmd = command(tmp)
-    proc = subprocess.run(cmd, capture_output=True, text=True)
-    if proc.returncode != 0:
-        sys.stderr.write(proc.stdout + proc.stderr)
-        raise RuntimeError(f"hipcc failed ({proc.returncode}): {' '.join(cmd)}")
-    if verbose:
-        sys.stderr.write(proc.stderr)
-    os.replace(tmp, OUT)
+    jobs = jobs or min(len(UNITS), max(1, min(16, os.cpu_count() or 1)))
+    with tempfile.TemporaryDirectory(prefix="lsf_build_") as tmp:
+        cmds = [compile_cmd(src, os.path.join(tmp, name + ".o"), defs) for name, src, defs in UNITS]
+        with ThreadPoolExecutor(max_workers=jobs) as pool:
+            procs = list(pool.map(_run, cmds))
+        for cmd, proc in zip(cmds, procs):
+            if proc.returncode != 0:
+                sys.stderr.write(proc.stdout + proc.stderr)
+                raise RuntimeError(f"hipcc failed ({proc.returncode}): {' '.join(cmd)}")
+            if verbose:
+                sys.stderr.write(proc.stderr)
+        out_tmp = OUT + ".tmp"
+        cmd = link_cmd([c[-1] for c in cmds], out_tmp)
+        proc = _run(cmd)
+        if proc.returncode != 0:
+            sys.stderr.write(proc.stdout + proc.stderr)
+            raise RuntimeError(f"link failed ({proc.returncode}): {' '.join(cmd)}")
+    os.replace(out_tmp, OUT)
     return OUT
 
 

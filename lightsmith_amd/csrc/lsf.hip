@@ -3,15 +3,17 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "lsf.h"
+#include "lsf_launch.h"
 #include "lsf_sparse.h"
-#include "lsf_step.h"
 
 using lsf::c32;
 
@@ -109,18 +111,28 @@ struct lsf_engine {
   float* partial = nullptr;  // library scratch: DFT energy partials
   int* mon_tab = nullptr;    // library scratch: [n_mon + 1] offsets, [n_mon] groups
   int n_partial = 0;
-  int ry = 2, waves = 4, zc = 16;
+  int ry = 2, waves = 4, xw = 64, zc = 16;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  lsf::StepArgs* dev_args = nullptr;  // library scratch: [0] E kernel, [1] H kernel arguments
+  int nb = 0, nbp = 0;                 // tiles of this engine's own launches (and padded to a multiple of 8)
+  bool dirty = true;                   // dev_args out of date
+  bool launched = false;               // kernels may be reading dev_args
+  int held = 0;                        // batches holding this engine (its configuration is then frozen)
 };
 
 namespace {
 
 using lsf::StepArgs;
 
-int nchunk_of(const lsf_engine* e) { return (e->g.n[0] + 63) / 64; }
+struct Tiling {
+  int ry, waves, xw, zc;
+};
 
-StepArgs make_args(const lsf_engine* e, int field) {
+Tiling own_tiling(const lsf_engine* e) { return Tiling{e->ry, e->waves, e->xw, e->zc}; }
+
+// Kernel arguments of engine e for field 0 (E kernel) or 1 (H kernel) under tiling t.
+StepArgs make_args(const lsf_engine* e, int field, const Tiling& t) {
   StepArgs a{};
   a.g = e->g;
   a.E = e->E;
@@ -132,87 +144,96 @@ StepArgs make_args(const lsf_engine* e, int field) {
   a.chunk_base = e->chunk_base;
   a.P = e->P;
   a.n_ade = (int)e->n_ade;
-  a.nchunk = nchunk_of(e);
+  a.nchunk = (e->g.n[0] + 31) / 32;
   a.max_poles = e->max_poles;
   a.media_stride = 2 + 3 * e->max_poles;
   a.media = e->media;
   a.n_media_floats = e->n_ade > 0 ? e->n_media * (2 + 3 * e->max_poles) : 0;
   a.counter = e->counter;
   a.tick = e->tick;
-  const int rows = e->ry * e->waves;
-  a.tiles_x = (e->g.n[0] + 63) / 64;
+  const int rows = t.ry * t.waves * (64 / t.xw);
+  a.tiles_x = (e->g.n[0] + t.xw - 1) / t.xw;
   a.tiles_y = (e->g.n[1] + rows - 1) / rows;
-  a.zc = e->zc;
-  const int tiles_z = (e->g.n[2] + e->zc - 1) / e->zc;
+  a.zc = t.zc;
+  const int tiles_z = (e->g.n[2] + t.zc - 1) / t.zc;
   a.nb = a.tiles_x * a.tiles_y * tiles_z;
   a.nbp = (a.nb + 7) / 8 * 8;
   return a;
 }
 
-template <class T, int RY, int W, bool PER>
-void launch_h_t(const StepArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((lsf::k_update_h<T, RY, W, PER>), dim3(a.nbp), dim3(64 * W), 0, s, a);
-}
-
 bool any_periodic(const lsf_engine* e) { return e->g.per[0] || e->g.per[1] || e->g.per[2]; }
 
-template <class T, bool PER>
-void launch_h_p(const lsf_engine* e, hipStream_t s) {
-  const StepArgs a = make_args(e, 1);
-  const bool w8 = e->waves == 8;
-  if (e->ry == 2) {
-    if (w8) launch_h_t<T, 2, 8, PER>(a, s);
-    else launch_h_t<T, 2, 4, PER>(a, s);
-  } else {
-    if (w8) launch_h_t<T, 1, 8, PER>(a, s);
-    else launch_h_t<T, 1, 4, PER>(a, s);
-  }
+// pole registers of the E kernel instantiation for this engine
+int pole_regs(const lsf_engine* e) {
+  if (e->n_ade <= 0) return 0;
+  return e->max_poles <= 2 ? 2 : (e->max_poles == 3 ? 3 : 4);
 }
 
-template <class T>
-void launch_h(const lsf_engine* e, hipStream_t s) {
-  if (any_periodic(e)) launch_h_p<T, true>(e, s);
-  else launch_h_p<T, false>(e, s);
+// Upload the engine's kernel arguments (and its sub-engines') when its configuration changed. Not called
+// during graph capture.
+int sync_args(lsf_engine* e) {
+  for (const Op& o : e->ops)
+    if (o.kind == OP_SUB_H) {
+      const int rc = sync_args(o.sub);
+      if (rc) return rc;
+    }
+  if (!e->dirty) return LSF_OK;
+  int rc = LSF_OK;
+  if (!e->dev_args) rc = hip_check(hipMalloc(&e->dev_args, 2 * sizeof(StepArgs)), "hipMalloc kernel arguments");
+  else if (e->launched) rc = hip_check(hipDeviceSynchronize(), "synchronize before an argument update");
+  if (rc) return rc;
+  const Tiling t = own_tiling(e);
+  const StepArgs h[2] = {make_args(e, 0, t), make_args(e, 1, t)};
+  rc = hip_check(hipMemcpy(e->dev_args, h, sizeof h, hipMemcpyHostToDevice), "kernel argument upload");
+  if (rc) return rc;
+  e->nb = h[0].nb;
+  e->nbp = h[0].nbp;
+  e->dirty = false;
+  return LSF_OK;
 }
 
-template <class T, int MAT, bool PER, int NP>
-void launch_e_np(const lsf_engine* e, hipStream_t s) {
-  const StepArgs a = make_args(e, 0);
-  const dim3 grid(a.nbp);
-  const bool w8 = e->waves == 8;
-  if (e->ry == 2) {
-    if (w8) hipLaunchKernelGGL((lsf::k_update_e<T, MAT, 2, 8, PER, NP>), grid, dim3(512), 0, s, a);
-    else hipLaunchKernelGGL((lsf::k_update_e<T, MAT, 2, 4, PER, NP>), grid, dim3(256), 0, s, a);
-  } else {
-    if (w8) hipLaunchKernelGGL((lsf::k_update_e<T, MAT, 1, 8, PER, NP>), grid, dim3(512), 0, s, a);
-    else hipLaunchKernelGGL((lsf::k_update_e<T, MAT, 1, 4, PER, NP>), grid, dim3(256), 0, s, a);
-  }
+lsf::KLaunch own_launch(lsf_engine* e, int field) {
+  lsf::KLaunch L{};
+  L.args = e->dev_args + field;
+  L.sc = lsf::Sched{nullptr, nullptr, e->nb, e->nbp};
+  L.grid = e->nbp;
+  L.ry = e->ry;
+  L.waves = e->waves;
+  L.xw = e->xw;
+  L.cplx = e->cplx ? 1 : 0;
+  L.per = any_periodic(e) ? 1 : 0;
+  L.mat = e->mid ? 1 : 0;
+  L.np = pole_regs(e);
+  return L;
 }
 
-template <class T, int MAT, bool PER>
-void launch_e_m(const lsf_engine* e, hipStream_t s) {
-  const int np = e->n_ade > 0 ? e->max_poles : 0;
-  if (np == 0) launch_e_np<T, MAT, PER, 0>(e, s);
-  else if (np <= 2) launch_e_np<T, MAT, PER, 2>(e, s);
-  else if (np == 3) launch_e_np<T, MAT, PER, 3>(e, s);
-  else launch_e_np<T, MAT, PER, 4>(e, s);
+void launch_h(lsf_engine* e, hipStream_t s) {
+  lsf::launch_update_h(own_launch(e, 1), s);
+  e->launched = true;
 }
 
-template <class T>
-void launch_e(const lsf_engine* e, hipStream_t s) {
-  const bool per = any_periodic(e);
-  if (e->mid) {
-    if (per) launch_e_m<T, 1, true>(e, s);
-    else launch_e_m<T, 1, false>(e, s);
-  } else {
-    if (per) launch_e_m<T, 0, true>(e, s);
-    else launch_e_m<T, 0, false>(e, s);
-  }
+void launch_e(lsf_engine* e, hipStream_t s) {
+  lsf::launch_update_e(own_launch(e, 0), s);
+  e->launched = true;
 }
 
 inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 void run_phase(lsf_engine* e, int phase, hipStream_t s);
+
+template <class T>
+lsf::GatherJob<T> gather_job(lsf_engine* e, const Op& o) {
+  T* F = (T*)(o.field == 0 ? e->E : e->H);
+  return lsf::GatherJob<T>{o.n, o.width, o.target, F, (const T*)o.src, o.sidx, (const T*)o.w, o.scale};
+}
+
+template <class T>
+lsf::ModalJob<T> modal_job(lsf_engine* e, const Op& o) {
+  T* F = (T*)(o.field == 0 ? e->E : e->H);
+  const int slot = o.phase <= LSF_PH_ADVANCE_H ? 0 : 1;
+  return lsf::ModalJob<T>{o.n, o.width, o.target, F, (const T*)o.w, o.table, o.table_len,
+                          (const long long*)e->counter, slot, o.scale};
+}
 
 template <class T>
 void run_op(lsf_engine* e, const Op& o, hipStream_t s) {
@@ -221,13 +242,11 @@ void run_op(lsf_engine* e, const Op& o, hipStream_t s) {
   switch (o.kind) {
     case OP_GATHER:
       if (o.n > 0)
-        hipLaunchKernelGGL(lsf::k_gather<T>, dim3(nblk(o.n, 256)), dim3(256), 0, s, o.n, o.width, o.target, F,
-                           (const T*)o.src, o.sidx, (const T*)o.w, o.scale);
+        hipLaunchKernelGGL(lsf::k_gather<T>, dim3(nblk(o.n, 256)), dim3(256), 0, s, gather_job<T>(e, o));
       break;
     case OP_MODAL:
       if (o.n > 0)
-        hipLaunchKernelGGL(lsf::k_modal<T>, dim3(nblk(o.n, 256)), dim3(256), 0, s, o.n, o.width, o.target, F,
-                           (const T*)o.w, o.table, o.table_len, (const long long*)e->counter, slot, o.scale);
+        hipLaunchKernelGGL(lsf::k_modal<T>, dim3(nblk(o.n, 256)), dim3(256), 0, s, modal_job<T>(e, o));
       break;
     case OP_TABLE:
       if (o.n > 0)
@@ -247,14 +266,12 @@ void run_op(lsf_engine* e, const Op& o, hipStream_t s) {
                          o.m_src, o.table, o.table_len, (const long long*)e->counter, slot);
       break;
     case OP_SUB_H:
-      if (o.sub->cplx) launch_h<c32>(o.sub, s);
-      else launch_h<float>(o.sub, s);
+      launch_h(o.sub, s);
       run_phase(o.sub, LSF_PH_CORRECT_H, s);
       run_phase(o.sub, LSF_PH_ADVANCE_H, s);
       break;
     case OP_SUB_E:
-      if (o.sub->cplx) launch_e<c32>(o.sub, s);
-      else launch_e<float>(o.sub, s);
+      launch_e(o.sub, s);
       for (int ph = LSF_PH_OFFDIAG; ph <= LSF_PH_ADVANCE_E; ++ph) run_phase(o.sub, ph, s);
       break;
     case OP_MIXED:
@@ -274,36 +291,47 @@ void run_phase(lsf_engine* e, int phase, hipStream_t s) {
 }
 
 void one_step(lsf_engine* e, hipStream_t s) {
-  if (e->cplx) launch_h<c32>(e, s);
-  else launch_h<float>(e, s);
+  launch_h(e, s);
   run_phase(e, LSF_PH_CORRECT_H, s);
   run_phase(e, LSF_PH_ADVANCE_H, s);
-  if (e->cplx) launch_e<c32>(e, s);
-  else launch_e<float>(e, s);
+  launch_e(e, s);
   for (int ph = LSF_PH_OFFDIAG; ph <= LSF_PH_ADVANCE_E; ++ph) run_phase(e, ph, s);
 }
 
-void sample(lsf_engine* e, hipStream_t s) {
-  for (const Monitor& m : e->mons) {
-    const size_t shm = sizeof(c32) * 2 * m.nf;
-    float* part = m.group >= 0 ? e->partial + m.poff : nullptr;
-    const void* es = m.esrc ? m.esrc : e->E;
-    const void* hs = m.hsrc ? m.hsrc : e->H;
-    const bool cx = (m.esrc ? m.src_complex : e->cplx);
-    if (cx)
-      hipLaunchKernelGGL(lsf::k_dft<c32>, dim3(m.nblocks), dim3(256), shm, s, m.nf, m.np, m.omega, e->dt,
-                         (const long long*)e->counter, (const c32*)es, (const c32*)hs, m.e, m.h1, m.h2,
-                         (c32*)m.acc_e, (c32*)m.acc_h, part);
-    else
-      hipLaunchKernelGGL(lsf::k_dft<float>, dim3(m.nblocks), dim3(256), shm, s, m.nf, m.np, m.omega,
-                         e->dt, (const long long*)e->counter, (const float*)es, (const float*)hs, m.e,
-                         m.h1, m.h2, (c32*)m.acc_e, (c32*)m.acc_h, part);
-  }
-  bool any_energy = false;
-  for (const Monitor& m : e->mons) any_energy |= m.group >= 0;
-  if (any_energy && e->energy)
-    hipLaunchKernelGGL(lsf::k_energy, dim3(1), dim3(64), 0, s, (int)e->mons.size(), e->mon_tab,
-                       e->mon_tab + e->mons.size() + 1, e->partial, e->energy);
+template <class S>
+lsf::DftJob<S> dft_job(const lsf_engine* e, const Monitor& m) {
+  lsf::DftJob<S> J{};
+  J.nf = m.nf;
+  J.np = m.np;
+  J.omega = m.omega;
+  J.dt = e->dt;
+  J.counter = e->counter;
+  J.esrc = (const S*)(m.esrc ? m.esrc : e->E);
+  J.hsrc = (const S*)(m.hsrc ? m.hsrc : e->H);
+  J.eidx = m.e;
+  J.h1 = m.h1;
+  J.h2 = m.h2;
+  J.acc_e = (c32*)m.acc_e;
+  J.acc_h = (c32*)m.acc_h;
+  J.partial = m.group >= 0 ? e->partial + m.poff : nullptr;
+  return J;
+}
+
+bool mon_complex(const lsf_engine* e, const Monitor& m) { return m.esrc ? m.src_complex != 0 : e->cplx; }
+
+bool has_energy(const lsf_engine* e) {
+  if (!e->energy) return false;
+  for (const Monitor& m : e->mons)
+    if (m.group >= 0) return true;
+  return false;
+}
+
+lsf::EnergyJob energy_job(const lsf_engine* e) {
+  const int nm = (int)e->mons.size();
+  return lsf::EnergyJob{nm, e->mon_tab, e->mon_tab + nm + 1, e->partial, e->energy};
+}
+
+void sample_time(lsf_engine* e, hipStream_t s) {
   for (const TimeMon& t : e->tmons) {
     if (e->cplx)
       hipLaunchKernelGGL(lsf::k_timemon<c32>, dim3(nblk(t.np, 256)), dim3(256), 0, s, t.np, t.e, (const c32*)e->E,
@@ -315,9 +343,30 @@ void sample(lsf_engine* e, hipStream_t s) {
   }
 }
 
+void sample(lsf_engine* e, hipStream_t s) {
+  for (const Monitor& m : e->mons) {
+    const size_t shm = sizeof(c32) * 2 * m.nf;
+    if (mon_complex(e, m))
+      hipLaunchKernelGGL(lsf::k_dft<c32>, dim3(m.nblocks), dim3(256), shm, s, dft_job<c32>(e, m));
+    else
+      hipLaunchKernelGGL(lsf::k_dft<float>, dim3(m.nblocks), dim3(256), shm, s, dft_job<float>(e, m));
+  }
+  if (has_energy(e)) hipLaunchKernelGGL(lsf::k_energy, dim3(1), dim3(64), 0, s, energy_job(e));
+  sample_time(e, s);
+}
+
 int valid(lsf_engine* e) { return e ? LSF_OK : fail(LSF_EINVAL, "null engine handle"); }
 
+// Configuration calls: refused while a batch holds the engine; they invalidate the uploaded arguments.
+int configurable(lsf_engine* e) {
+  if (!e) return fail(LSF_EINVAL, "null engine handle");
+  if (e->held) return fail(LSF_ESTATE, "the engine is held by a batch; destroy the batch before changing it");
+  e->dirty = true;
+  return LSF_OK;
+}
+
 int add_op(lsf_engine* e, const Op& o) {
+  if (int rc0 = configurable(e)) return rc0;
   if (o.phase < LSF_PH_CORRECT_H || o.phase > LSF_PH_ADVANCE_E) return fail(LSF_EINVAL, "bad phase");
   if (o.field != 0 && o.field != 1) return fail(LSF_EINVAL, "field must be 0 (E) or 1 (H)");
   if (e->exec) return fail(LSF_ESTATE, "the engine has a captured graph; reset it before adding work");
@@ -383,11 +432,12 @@ void lsf_destroy(lsf_engine* e) {
   if (e->graph) (void)hipGraphDestroy(e->graph);
   if (e->partial) (void)hipFree(e->partial);
   if (e->mon_tab) (void)hipFree(e->mon_tab);
+  if (e->dev_args) (void)hipFree(e->dev_args);
   delete e;
 }
 
 int lsf_set_material_dense(lsf_engine* e, const float* cb) {
-  if (valid(e)) return LSF_EINVAL;
+  if (int rc0 = configurable(e)) return rc0;
   if (!cb) return fail(LSF_EINVAL, "null material");
   e->cbd = cb;
   e->mid = nullptr;
@@ -396,7 +446,7 @@ int lsf_set_material_dense(lsf_engine* e, const float* cb) {
 }
 
 int lsf_set_material_indexed(lsf_engine* e, const uint32_t* ids, const float* table) {
-  if (valid(e)) return LSF_EINVAL;
+  if (int rc0 = configurable(e)) return rc0;
   if (!ids || !table) return fail(LSF_EINVAL, "null material");
   e->cbd = nullptr;
   e->mid = ids;
@@ -406,7 +456,7 @@ int lsf_set_material_indexed(lsf_engine* e, const uint32_t* ids, const float* ta
 
 int lsf_set_pml(lsf_engine* e, int field, int axis, const void* prof, int n_slab, void* p0, void* p1, void* p2,
                 int has_k) {
-  if (valid(e)) return LSF_EINVAL;
+  if (int rc0 = configurable(e)) return rc0;
   if (field < 0 || field > 1 || axis < 0 || axis > 2) return fail(LSF_EINVAL, "bad field or axis");
   if (e->g.per[axis]) return fail(LSF_EINVAL, "a periodic axis cannot have a PML");
   lsf::PmlAx& ax = e->pml[field][axis];
@@ -423,7 +473,7 @@ int lsf_set_pml(lsf_engine* e, int field, int axis, const void* prof, int n_slab
 
 int lsf_set_ade(lsf_engine* e, const int32_t* chunk_base, void* P, int64_t n, int max_poles, const float* media,
                 int n_media) {
-  if (valid(e)) return LSF_EINVAL;
+  if (int rc0 = configurable(e)) return rc0;
   if (n > 0 && (!chunk_base || !P || !media)) return fail(LSF_EINVAL, "missing ADE arrays");
   if (max_poles < 0 || max_poles > lsf::MAX_POLES) return fail(LSF_EINVAL, "pure D-form media may have at most 4 poles");
   if (n > 0 && max_poles < 1) return fail(LSF_EINVAL, "D-form nodes need at least one pole");
@@ -583,7 +633,7 @@ int lsf_add_mixed_group(lsf_engine* e, int64_t n, int S, int max_poles, const in
 }
 
 int lsf_set_energy(lsf_engine* e, float* energy) {
-  if (valid(e)) return LSF_EINVAL;
+  if (int rc0 = configurable(e)) return rc0;
   e->energy = energy;
   return LSF_OK;
 }
@@ -591,7 +641,7 @@ int lsf_set_energy(lsf_engine* e, float* energy) {
 int lsf_add_dft_monitor(lsf_engine* e, int nf, int64_t np, const double* omega, const int64_t* e_idx,
                         const int64_t* h1, const int64_t* h2, void* acc_e, void* acc_h, const void* esrc,
                         const void* hsrc, int src_complex, int energy_group) {
-  if (valid(e)) return LSF_EINVAL;
+  if (int rc0 = configurable(e)) return rc0;
   if (nf < 1 || nf > 2048 || np < 1 || !omega || !e_idx || !acc_e) return fail(LSF_EINVAL, "bad DFT monitor");
   if (acc_h && (!h1 || !h2)) return fail(LSF_EINVAL, "H accumulators need h1 and h2");
   if (energy_group < 0 || energy_group > 2) return fail(LSF_EINVAL, "energy group must be 0, 1 or 2");
@@ -634,7 +684,7 @@ int lsf_add_dft_monitor(lsf_engine* e, int nf, int64_t np, const double* omega, 
 }
 
 int lsf_add_time_monitor(lsf_engine* e, int64_t np, const int64_t* e_idx, void* buf, int64_t n_max, int64_t* pos) {
-  if (valid(e)) return LSF_EINVAL;
+  if (int rc0 = configurable(e)) return rc0;
   if (np < 1 || !e_idx || !buf || !pos || n_max < 1) return fail(LSF_EINVAL, "bad time monitor");
   if (e->exec) return fail(LSF_ESTATE, "the engine has a captured graph; reset it before adding monitors");
   TimeMon t;
@@ -647,14 +697,15 @@ int lsf_add_time_monitor(lsf_engine* e, int64_t np, const int64_t* e_idx, void* 
   return LSF_OK;
 }
 
-int lsf_set_tiling(lsf_engine* e, int ry, int waves, int zc) {
-  if (valid(e)) return LSF_EINVAL;
-  if (!(ry == 1 || ry == 2)) return fail(LSF_EINVAL, "rows per lane must be 1 or 2");
-  if (!(waves == 4 || waves == 8)) return fail(LSF_EINVAL, "waves per block must be 4 or 8");
+int lsf_set_tiling(lsf_engine* e, int ry, int waves, int xw, int zc) {
+  if (int rc0 = configurable(e)) return rc0;
+  if (!lsf::tile_shape_ok(ry, waves, xw))
+    return fail(LSF_EINVAL, "unsupported tile shape: rows per lane 1 or 2, waves 4 or 8, x cells per row 64 or 32");
   if (zc < 1) return fail(LSF_EINVAL, "z chunk must be >= 1");
   if (e->exec) return fail(LSF_ESTATE, "the engine has a captured graph; reset it before retiling");
   e->ry = ry;
   e->waves = waves;
+  e->xw = xw;
   e->zc = zc;
   return LSF_OK;
 }
@@ -662,6 +713,7 @@ int lsf_set_tiling(lsf_engine* e, int ry, int waves, int zc) {
 int lsf_step(lsf_engine* e, int n, void* stream) {
   if (valid(e)) return LSF_EINVAL;
   if (!e->cbd && !e->mid) return fail(LSF_ESTATE, "no material set");
+  if (int rc1 = sync_args(e)) return rc1;
   hipStream_t s = (hipStream_t)stream;
   for (int q = 0; q < n; ++q) one_step(e, s);
   return hip_check(hipGetLastError(), "step launch");
@@ -671,6 +723,7 @@ int lsf_step_timed(lsf_engine* e, int n, void* stream, double* ms) {
   if (valid(e)) return LSF_EINVAL;
   if (!e->cbd && !e->mid) return fail(LSF_ESTATE, "no material set");
   if (!ms || n < 1) return fail(LSF_EINVAL, "bad timed step arguments");
+  if (int rc1 = sync_args(e)) return rc1;
   hipStream_t s = (hipStream_t)stream;
   std::vector<hipEvent_t> ev(4 * (size_t)n, nullptr);
   int rc = LSF_OK;
@@ -679,14 +732,12 @@ int lsf_step_timed(lsf_engine* e, int n, void* stream, double* ms) {
   if (!rc) {
     for (int q = 0; q < n; ++q) {
       (void)hipEventRecord(ev[4 * q + 0], s);
-      if (e->cplx) launch_h<c32>(e, s);
-      else launch_h<float>(e, s);
+      launch_h(e, s);
       (void)hipEventRecord(ev[4 * q + 1], s);
       run_phase(e, LSF_PH_CORRECT_H, s);
       run_phase(e, LSF_PH_ADVANCE_H, s);
       (void)hipEventRecord(ev[4 * q + 2], s);
-      if (e->cplx) launch_e<c32>(e, s);
-      else launch_e<float>(e, s);
+      launch_e(e, s);
       (void)hipEventRecord(ev[4 * q + 3], s);
       for (int ph = LSF_PH_OFFDIAG; ph <= LSF_PH_ADVANCE_E; ++ph) run_phase(e, ph, s);
     }
@@ -720,6 +771,7 @@ int lsf_chunk(lsf_engine* e, int n_chunks, int stride, void* stream) {
   if (valid(e)) return LSF_EINVAL;
   if (!e->cbd && !e->mid) return fail(LSF_ESTATE, "no material set");
   if (stride < 1) return fail(LSF_EINVAL, "stride must be >= 1");
+  if (int rc1 = sync_args(e)) return rc1;
   hipStream_t s = (hipStream_t)stream;
   for (int c = 0; c < n_chunks; ++c) {
     for (int q = 0; q < stride; ++q) one_step(e, s);
@@ -733,6 +785,7 @@ int lsf_graph_capture(lsf_engine* e, int stride, void* stream) {
   if (!stream) return fail(LSF_EINVAL, "graph capture needs a non-default stream");
   if (e->exec) return fail(LSF_ESTATE, "a graph is already captured");
   if (!e->cbd && !e->mid) return fail(LSF_ESTATE, "no material set");
+  if (int rc1 = sync_args(e)) return rc1;
   hipStream_t s = (hipStream_t)stream;
   int rc = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
   if (rc) return rc;
@@ -769,6 +822,506 @@ int lsf_graph_reset(lsf_engine* e) {
   if (e->graph) (void)hipGraphDestroy(e->graph);
   e->exec = nullptr;
   e->graph = nullptr;
+  return rc;
+}
+
+}  // extern "C"
+
+// =============================================================================================
+// Batches: many independent engines stepped by shared launches. Each half-step is one launch of the bulk
+// kernel per kernel configuration over every member's tiles; the members' per-step operations are fused
+// by position (the q-th operation of a phase over all members is one slot: gathers and modal sources of a
+// slot become one launch each, sub-engines of a slot become a batch level of their own), and the DFT
+// monitors of all members are sampled by one launch per source type.
+// =============================================================================================
+
+struct lsf_batch;
+
+namespace {
+
+struct KGroup {
+  int cplx = 0, per = 0, mat = 0, np = 0;
+  std::vector<lsf_engine*> mem;
+  lsf::KLaunch L{};
+};
+
+struct Fused {
+  int kind = 0;  // OP_GATHER or OP_MODAL
+  int cplx = 0;
+  void* jobs = nullptr;
+  lsf::Sched sc{};
+};
+
+struct Level;
+
+struct Slot {
+  std::vector<Fused> fused;
+  std::vector<std::pair<lsf_engine*, Op>> single;  // run one by one
+  std::unique_ptr<Level> sub;                     // the sub-engines of SUB_H / SUB_E operations
+};
+
+struct DftLaunch {
+  int cplx = 0;
+  void* jobs = nullptr;
+  lsf::Sched sc{};
+  size_t shm = 0;
+};
+
+struct Level {
+  std::vector<lsf_engine*> mem;
+  std::vector<KGroup> hg, eg;
+  std::vector<Slot> slots[LSF_PH_ADVANCE_E + 1];
+  std::vector<DftLaunch> dft;
+  lsf::EnergyJob* energy = nullptr;
+  int n_energy = 0;
+};
+
+}  // namespace
+
+struct lsf_batch {
+  std::vector<lsf_engine*> members;
+  Level root;
+  Tiling tiling{2, 4, 64, 16};
+  int grid_blocks = 0;  // 0: one block per tile; else a persistent grid of this many blocks
+  std::vector<void*> owned;         // device allocations of the plan
+  std::vector<void*> owned_groups;  // device allocations of the root kernel groups (rebuilt on retiling)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int dev_upload(const void* src, size_t bytes, void** dst, std::vector<void*>& owned) {
+  *dst = nullptr;
+  if (bytes == 0) return LSF_OK;
+  int rc = hip_check(hipMalloc(dst, bytes), "hipMalloc batch plan");
+  if (rc) return rc;
+  owned.push_back(*dst);
+  return hip_check(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice), "batch plan upload");
+}
+
+// A work list over members with the given block counts: block -> member, member -> first block.
+int make_sched(const std::vector<int>& counts, lsf::Sched* sc, std::vector<void*>& owned) {
+  std::vector<int> inst, first;
+  for (size_t m = 0; m < counts.size(); ++m) {
+    first.push_back((int)inst.size());
+    inst.insert(inst.end(), counts[m], (int)m);
+  }
+  sc->n_tiles = (int)inst.size();
+  sc->n_pad = (sc->n_tiles + 7) / 8 * 8;
+  void *di = nullptr, *df = nullptr;
+  int rc = dev_upload(inst.data(), sizeof(int) * inst.size(), &di, owned);
+  if (!rc) rc = dev_upload(first.data(), sizeof(int) * first.size(), &df, owned);
+  sc->inst_of = (const int*)di;
+  sc->first = (const int*)df;
+  return rc;
+}
+
+// Bulk-kernel groups of a level under tiling t: members sharing a kernel instantiation share a launch.
+int build_groups(Level& L, const Tiling& t, int grid_blocks, std::vector<void*>& owned) {
+  L.hg.clear();
+  L.eg.clear();
+  for (int field = 0; field < 2; ++field) {
+    std::vector<KGroup>& groups = field ? L.hg : L.eg;
+    for (lsf_engine* e : L.mem) {
+      KGroup key;
+      key.cplx = e->cplx ? 1 : 0;
+      key.per = any_periodic(e) ? 1 : 0;
+      key.mat = field ? 0 : (e->mid ? 1 : 0);
+      key.np = field ? 0 : pole_regs(e);
+      KGroup* g = nullptr;
+      for (KGroup& x : groups)
+        if (x.cplx == key.cplx && x.per == key.per && x.mat == key.mat && x.np == key.np) g = &x;
+      if (!g) {
+        groups.push_back(key);
+        g = &groups.back();
+      }
+      g->mem.push_back(e);
+    }
+    for (KGroup& g : groups) {
+      std::vector<StepArgs> args;
+      std::vector<int> counts;
+      for (lsf_engine* e : g.mem) {
+        args.push_back(make_args(e, field, t));
+        counts.push_back(args.back().nb);
+      }
+      void* da = nullptr;
+      int rc = dev_upload(args.data(), sizeof(StepArgs) * args.size(), &da, owned);
+      if (rc) return rc;
+      g.L.args = (const StepArgs*)da;
+      if ((rc = make_sched(counts, &g.L.sc, owned))) return rc;
+      g.L.grid = g.L.sc.n_pad;
+      if (grid_blocks > 0) g.L.grid = std::min(g.L.sc.n_pad, (grid_blocks + 7) / 8 * 8);
+      g.L.ry = t.ry;
+      g.L.waves = t.waves;
+      g.L.xw = t.xw;
+      g.L.cplx = g.cplx;
+      g.L.per = g.per;
+      g.L.mat = g.mat;
+      g.L.np = g.np;
+    }
+  }
+  return LSF_OK;
+}
+
+template <class T>
+int fuse_ops(const std::vector<std::pair<lsf_engine*, Op>>& jobs, int kind, Slot& sl, std::vector<void*>& owned) {
+  std::vector<int> counts;
+  std::vector<lsf::GatherJob<T>> gj;
+  std::vector<lsf::ModalJob<T>> mj;
+  for (const auto& j : jobs) {
+    counts.push_back((int)nblk(j.second.n, 256));
+    if (kind == OP_GATHER) gj.push_back(gather_job<T>(j.first, j.second));
+    else mj.push_back(modal_job<T>(j.first, j.second));
+  }
+  Fused f;
+  f.kind = kind;
+  f.cplx = sizeof(T) == sizeof(c32);
+  int rc = kind == OP_GATHER ? dev_upload(gj.data(), sizeof(gj[0]) * gj.size(), &f.jobs, owned)
+                             : dev_upload(mj.data(), sizeof(mj[0]) * mj.size(), &f.jobs, owned);
+  if (!rc) rc = make_sched(counts, &f.sc, owned);
+  if (!rc) sl.fused.push_back(f);
+  return rc;
+}
+
+int build_level(Level& L, const Tiling& t, int grid_blocks, bool sample, std::vector<void*>& owned,
+                std::vector<void*>& owned_groups);
+
+int build_slot(const std::vector<std::pair<lsf_engine*, Op>>& jobs, Slot& sl, std::vector<void*>& owned) {
+  // fusable kinds by field type; sub-engines as a level; the rest one by one
+  std::vector<std::pair<lsf_engine*, Op>> part[2][2];  // [gather, modal][float, c32]
+  std::vector<lsf_engine*> subs;
+  for (const auto& j : jobs) {
+    const Op& o = j.second;
+    if ((o.kind == OP_GATHER || o.kind == OP_MODAL) && o.n > 0)
+      part[o.kind == OP_MODAL][j.first->cplx ? 1 : 0].push_back(j);
+    else if (o.kind == OP_SUB_H || o.kind == OP_SUB_E)
+      subs.push_back(o.sub);
+    else if (!((o.kind == OP_GATHER || o.kind == OP_MODAL) && o.n <= 0))
+      sl.single.push_back(j);
+  }
+  int rc = LSF_OK;
+  for (int k = 0; k < 2 && !rc; ++k) {
+    const int kind = k ? OP_MODAL : OP_GATHER;
+    if (!part[k][0].empty()) rc = fuse_ops<float>(part[k][0], kind, sl, owned);
+    if (!rc && !part[k][1].empty()) rc = fuse_ops<c32>(part[k][1], kind, sl, owned);
+  }
+  if (!rc && !subs.empty()) {
+    sl.sub = std::make_unique<Level>();
+    sl.sub->mem = subs;
+    rc = build_level(*sl.sub, Tiling{1, 4, 64, 16}, 0, false, owned, owned);
+  }
+  return rc;
+}
+
+int build_sampling(Level& L, std::vector<void*>& owned) {
+  for (int cx = 0; cx < 2; ++cx) {
+    std::vector<lsf::DftJob<float>> jf;
+    std::vector<lsf::DftJob<c32>> jc;
+    std::vector<int> counts;
+    int max_nf = 0;
+    for (lsf_engine* e : L.mem)
+      for (const Monitor& m : e->mons) {
+        if ((int)mon_complex(e, m) != cx) continue;
+        if (cx) jc.push_back(dft_job<c32>(e, m));
+        else jf.push_back(dft_job<float>(e, m));
+        counts.push_back(m.nblocks);
+        max_nf = std::max(max_nf, m.nf);
+      }
+    if (counts.empty()) continue;
+    DftLaunch d;
+    d.cplx = cx;
+    d.shm = sizeof(c32) * 2 * max_nf;
+    int rc = cx ? dev_upload(jc.data(), sizeof(jc[0]) * jc.size(), &d.jobs, owned)
+                : dev_upload(jf.data(), sizeof(jf[0]) * jf.size(), &d.jobs, owned);
+    if (!rc) rc = make_sched(counts, &d.sc, owned);
+    if (rc) return rc;
+    L.dft.push_back(d);
+  }
+  std::vector<lsf::EnergyJob> ej;
+  for (lsf_engine* e : L.mem)
+    if (has_energy(e)) ej.push_back(energy_job(e));
+  L.n_energy = (int)ej.size();
+  void* d = nullptr;
+  const int rc = dev_upload(ej.data(), sizeof(lsf::EnergyJob) * ej.size(), &d, owned);
+  L.energy = (lsf::EnergyJob*)d;
+  return rc;
+}
+
+// Kernel groups of the level go to owned_groups (the root level's are rebuilt on retiling), the rest to owned.
+int build_level(Level& L, const Tiling& t, int grid_blocks, bool sample, std::vector<void*>& owned,
+                std::vector<void*>& owned_groups) {
+  int rc = build_groups(L, t, grid_blocks, owned_groups);
+  for (int ph = LSF_PH_CORRECT_H; ph <= LSF_PH_ADVANCE_E && !rc; ++ph) {
+    std::vector<std::vector<std::pair<lsf_engine*, Op>>> by_pos;
+    for (lsf_engine* e : L.mem) {
+      size_t q = 0;
+      for (const Op& o : e->ops) {
+        if (o.phase != ph) continue;
+        if (by_pos.size() <= q) by_pos.resize(q + 1);
+        by_pos[q++].push_back({e, o});
+      }
+    }
+    for (const auto& jobs : by_pos) {
+      L.slots[ph].emplace_back();
+      if ((rc = build_slot(jobs, L.slots[ph].back(), owned))) break;
+    }
+  }
+  if (!rc && sample) rc = build_sampling(L, owned);
+  return rc;
+}
+
+void level_h(Level& L, hipStream_t s);
+void level_e(Level& L, hipStream_t s);
+
+void run_slot(Slot& sl, bool h_half, hipStream_t s) {
+  for (const Fused& f : sl.fused) {
+    const dim3 grid(f.sc.n_tiles), block(256);
+    if (f.kind == OP_GATHER) {
+      if (f.cplx) hipLaunchKernelGGL(lsf::k_gather_multi<c32>, grid, block, 0, s, (const lsf::GatherJob<c32>*)f.jobs, f.sc);
+      else hipLaunchKernelGGL(lsf::k_gather_multi<float>, grid, block, 0, s, (const lsf::GatherJob<float>*)f.jobs, f.sc);
+    } else {
+      if (f.cplx) hipLaunchKernelGGL(lsf::k_modal_multi<c32>, grid, block, 0, s, (const lsf::ModalJob<c32>*)f.jobs, f.sc);
+      else hipLaunchKernelGGL(lsf::k_modal_multi<float>, grid, block, 0, s, (const lsf::ModalJob<float>*)f.jobs, f.sc);
+    }
+  }
+  for (auto& j : sl.single) {
+    if (j.first->cplx) run_op<c32>(j.first, j.second, s);
+    else run_op<float>(j.first, j.second, s);
+  }
+  if (sl.sub) {
+    if (h_half) level_h(*sl.sub, s);
+    else level_e(*sl.sub, s);
+  }
+}
+
+void level_h_kernels(Level& L, hipStream_t s) {
+  for (KGroup& g : L.hg) lsf::launch_update_h(g.L, s);
+}
+
+void level_e_kernels(Level& L, hipStream_t s) {
+  for (KGroup& g : L.eg) lsf::launch_update_e(g.L, s);
+}
+
+void level_h_ops(Level& L, hipStream_t s) {
+  for (int ph = LSF_PH_CORRECT_H; ph <= LSF_PH_ADVANCE_H; ++ph)
+    for (Slot& sl : L.slots[ph]) run_slot(sl, true, s);
+}
+
+void level_e_ops(Level& L, hipStream_t s) {
+  for (int ph = LSF_PH_OFFDIAG; ph <= LSF_PH_ADVANCE_E; ++ph)
+    for (Slot& sl : L.slots[ph]) run_slot(sl, false, s);
+}
+
+void level_h(Level& L, hipStream_t s) {
+  level_h_kernels(L, s);
+  level_h_ops(L, s);
+}
+
+void level_e(Level& L, hipStream_t s) {
+  level_e_kernels(L, s);
+  level_e_ops(L, s);
+}
+
+void batch_sample(lsf_batch* b, hipStream_t s) {
+  Level& L = b->root;
+  for (const DftLaunch& d : L.dft) {
+    const dim3 grid(d.sc.n_tiles), block(256);
+    if (d.cplx)
+      hipLaunchKernelGGL(lsf::k_dft_multi<c32>, grid, block, d.shm, s, (const lsf::DftJob<c32>*)d.jobs, d.sc);
+    else
+      hipLaunchKernelGGL(lsf::k_dft_multi<float>, grid, block, d.shm, s, (const lsf::DftJob<float>*)d.jobs, d.sc);
+  }
+  if (L.n_energy) hipLaunchKernelGGL(lsf::k_energy_multi, dim3(L.n_energy), dim3(64), 0, s, L.energy);
+  for (lsf_engine* e : L.mem) sample_time(e, s);
+}
+
+void batch_step(lsf_batch* b, hipStream_t s) {
+  level_h(b->root, s);
+  level_e(b->root, s);
+}
+
+void hold(lsf_engine* e, int d) {
+  e->held += d;
+  for (const Op& o : e->ops)
+    if (o.kind == OP_SUB_H) hold(o.sub, d);
+}
+
+void free_all(std::vector<void*>& v) {
+  for (void* p : v) (void)hipFree(p);
+  v.clear();
+}
+
+int valid_batch(lsf_batch* b) { return b ? LSF_OK : fail(LSF_EINVAL, "null batch handle"); }
+
+}  // namespace
+
+extern "C" {
+
+int lsf_batch_create(lsf_engine* const* members, int n, lsf_batch** out) {
+  if (!members || !out || n < 1) return fail(LSF_EINVAL, "a batch needs at least one engine");
+  for (int q = 0; q < n; ++q) {
+    lsf_engine* e = members[q];
+    if (!e) return fail(LSF_EINVAL, "null engine in batch");
+    if (!e->tick) return fail(LSF_EINVAL, "batch members are top-level engines (tick = 1)");
+    if (!e->cbd && !e->mid) return fail(LSF_ESTATE, "a batch member has no material");
+    for (int r = 0; r < q; ++r)
+      if (members[r] == e || members[r]->counter == e->counter)
+        return fail(LSF_EINVAL, "batch members must be distinct engines with their own step counters");
+  }
+  lsf_batch* b = new lsf_batch();
+  b->members.assign(members, members + n);
+  b->root.mem = b->members;
+  const int rc = build_level(b->root, b->tiling, b->grid_blocks, true, b->owned, b->owned_groups);
+  if (rc) {
+    free_all(b->owned);
+    free_all(b->owned_groups);
+    delete b;
+    return rc;
+  }
+  for (lsf_engine* e : b->members) hold(e, 1);
+  *out = b;
+  return LSF_OK;
+}
+
+void lsf_batch_destroy(lsf_batch* b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  (void)hipDeviceSynchronize();
+  free_all(b->owned);
+  free_all(b->owned_groups);
+  for (lsf_engine* e : b->members) hold(e, -1);
+  delete b;
+}
+
+int lsf_batch_set_schedule(lsf_batch* b, int ry, int waves, int xw, int zc, int grid_blocks) {
+  if (valid_batch(b)) return LSF_EINVAL;
+  if (!lsf::tile_shape_ok(ry, waves, xw))
+    return fail(LSF_EINVAL, "unsupported tile shape: rows per lane 1 or 2, waves 4 or 8, x cells per row 64 or 32");
+  if (zc < 1 || grid_blocks < 0) return fail(LSF_EINVAL, "z chunk must be >= 1 and grid blocks >= 0");
+  if (b->exec) return fail(LSF_ESTATE, "the batch has a captured graph; reset it before rescheduling");
+  int rc = hip_check(hipDeviceSynchronize(), "synchronize before rescheduling");
+  if (rc) return rc;
+  b->tiling = Tiling{ry, waves, xw, zc};
+  b->grid_blocks = grid_blocks;
+  free_all(b->owned_groups);
+  return build_groups(b->root, b->tiling, b->grid_blocks, b->owned_groups);
+}
+
+int lsf_batch_tiles(lsf_batch* b, int64_t* n_tiles) {
+  if (valid_batch(b)) return LSF_EINVAL;
+  if (!n_tiles) return fail(LSF_EINVAL, "null output");
+  int64_t t = 0;
+  for (const KGroup& g : b->root.eg) t += g.L.sc.n_tiles;
+  *n_tiles = t;
+  return LSF_OK;
+}
+
+int lsf_batch_step(lsf_batch* b, int n, void* stream) {
+  if (valid_batch(b)) return LSF_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  for (int q = 0; q < n; ++q) batch_step(b, s);
+  return hip_check(hipGetLastError(), "batch step launch");
+}
+
+int lsf_batch_step_timed(lsf_batch* b, int n, void* stream, double* ms) {
+  if (valid_batch(b)) return LSF_EINVAL;
+  if (!ms || n < 1) return fail(LSF_EINVAL, "bad timed step arguments");
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<hipEvent_t> ev(4 * (size_t)n, nullptr);
+  int rc = LSF_OK;
+  for (auto& x : ev)
+    if ((rc = hip_check(hipEventCreate(&x), "event create"))) break;
+  if (!rc) {
+    for (int q = 0; q < n; ++q) {
+      (void)hipEventRecord(ev[4 * q + 0], s);
+      level_h_kernels(b->root, s);
+      (void)hipEventRecord(ev[4 * q + 1], s);
+      level_h_ops(b->root, s);
+      (void)hipEventRecord(ev[4 * q + 2], s);
+      level_e_kernels(b->root, s);
+      (void)hipEventRecord(ev[4 * q + 3], s);
+      level_e_ops(b->root, s);
+    }
+    rc = hip_check(hipGetLastError(), "timed batch step launch");
+  }
+  if (!rc) rc = hip_check(hipEventSynchronize(ev.back()), "event synchronize");
+  if (!rc) {
+    double th = 0.0, te = 0.0;
+    for (int q = 0; q < n; ++q) {
+      float x = 0.f, y = 0.f;
+      (void)hipEventElapsedTime(&x, ev[4 * q + 0], ev[4 * q + 1]);
+      (void)hipEventElapsedTime(&y, ev[4 * q + 2], ev[4 * q + 3]);
+      th += x;
+      te += y;
+    }
+    ms[0] += th;
+    ms[1] += te;
+  }
+  for (auto& x : ev)
+    if (x) (void)hipEventDestroy(x);
+  return rc;
+}
+
+int lsf_batch_sample(lsf_batch* b, void* stream) {
+  if (valid_batch(b)) return LSF_EINVAL;
+  batch_sample(b, (hipStream_t)stream);
+  return hip_check(hipGetLastError(), "batch sample launch");
+}
+
+int lsf_batch_chunk(lsf_batch* b, int n_chunks, int stride, void* stream) {
+  if (valid_batch(b)) return LSF_EINVAL;
+  if (stride < 1) return fail(LSF_EINVAL, "stride must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  for (int c = 0; c < n_chunks; ++c) {
+    for (int q = 0; q < stride; ++q) batch_step(b, s);
+    batch_sample(b, s);
+  }
+  return hip_check(hipGetLastError(), "batch chunk launch");
+}
+
+int lsf_batch_graph_capture(lsf_batch* b, int stride, void* stream) {
+  if (valid_batch(b)) return LSF_EINVAL;
+  if (!stream) return fail(LSF_EINVAL, "graph capture needs a non-default stream");
+  if (b->exec) return fail(LSF_ESTATE, "a graph is already captured");
+  if (stride < 1) return fail(LSF_EINVAL, "stride must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+  if (rc) return rc;
+  for (int q = 0; q < stride; ++q) batch_step(b, s);
+  batch_sample(b, s);
+  hipGraph_t g = nullptr;
+  rc = hip_check(hipStreamEndCapture(s, &g), "end capture");
+  if (rc) return rc;
+  b->graph = g;
+  rc = hip_check(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "graph instantiate");
+  if (rc) {
+    (void)hipGraphDestroy(g);
+    b->graph = nullptr;
+    b->exec = nullptr;
+  }
+  return rc;
+}
+
+int lsf_batch_graph_replay(lsf_batch* b, int n, void* stream) {
+  if (valid_batch(b)) return LSF_EINVAL;
+  if (!b->exec) return fail(LSF_ESTATE, "no captured graph");
+  for (int q = 0; q < n; ++q) {
+    const int rc = hip_check(hipGraphLaunch(b->exec, (hipStream_t)stream), "graph launch");
+    if (rc) return rc;
+  }
+  return LSF_OK;
+}
+
+int lsf_batch_graph_reset(lsf_batch* b) {
+  if (valid_batch(b)) return LSF_EINVAL;
+  int rc = LSF_OK;
+  if (b->exec || b->graph) rc = hip_check(hipDeviceSynchronize(), "synchronize before graph reset");
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
   return rc;
 }
 

@@ -87,6 +87,17 @@ __device__ __forceinline__ T ld_lo(const T* __restrict__ F, const Grid& g, int i
   return v;
 }
 
+// Pointers into global memory. A pointer loaded from memory is generic (flat) to the compiler; held in a
+// gptr and converted back to the generic type where it is used, its loads and stores compile to global
+// instructions (a generic pointer captured by a lambda, or cast to global and straight back, loses that).
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(T* p) {
+  return (gptr<T>)p;
+}
+
 // Blocks are dealt round-robin to the 8 XCDs (each with its own L2). Give each XCD a contiguous range of
 // logical tiles so that tiles sharing halo planes share an L2. The grid is padded to a multiple of 8.
 __device__ __forceinline__ int xcd_tile(int b, int nblocks_padded) {

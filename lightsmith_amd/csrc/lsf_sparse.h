@@ -3,7 +3,7 @@
 // launch (the host merges them), so no atomics are needed and results are deterministic.
 #pragma once
 
-#include "lsf_common.h"
+#include "lsf_step.h"
 
 namespace lsf {
 
@@ -16,30 +16,75 @@ __device__ __forceinline__ void apply_corr(T* F, long long t, T v, const float* 
 }
 
 template <class T>
-__global__ void k_gather(long long n_rows, int width, const long long* __restrict__ target, T* F,
-                         const T* __restrict__ src, const long long* __restrict__ sidx, const T* __restrict__ w,
-                         const float* scale) {
-  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_rows) return;
+struct GatherJob {
+  long long n;
+  int width;
+  const long long* target;
+  T* F;
+  const T* src;
+  const long long* sidx;
+  const T* w;
+  const float* scale;
+};
+
+template <class T>
+__device__ __forceinline__ void gather_row(const GatherJob<T>& J, long long r) {
+  if (r >= J.n) return;
   T acc = zero<T>();
-  for (int q = 0; q < width; ++q) {
-    const long long e = r * width + q;
-    acc = add(acc, mul(w[e], src[sidx[e]]));
+  for (int q = 0; q < J.width; ++q) {
+    const long long e = r * J.width + q;
+    acc = add(acc, mul(J.w[e], J.src[J.sidx[e]]));
   }
-  apply_corr(F, target[r], acc, scale, r);
+  apply_corr(J.F, J.target[r], acc, J.scale, r);
 }
 
 template <class T>
-__global__ void k_modal(long long n_rows, int nm, const long long* __restrict__ target, T* F,
-                        const T* __restrict__ m, const float* __restrict__ table, long long table_len,
-                        const long long* counter, int slot, const float* scale) {
-  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_rows) return;
-  const long long t = counter[slot];
-  if (t < 0 || t >= table_len) return;
+__global__ void k_gather(GatherJob<T> J) {
+  gather_row(J, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <class T>
+struct ModalJob {
+  long long n;
+  int nm;
+  const long long* target;
+  T* F;
+  const T* m;
+  const float* table;
+  long long table_len;
+  const long long* counter;
+  int slot;
+  const float* scale;
+};
+
+template <class T>
+__device__ __forceinline__ void modal_row(const ModalJob<T>& J, long long r) {
+  if (r >= J.n) return;
+  const long long t = J.counter[J.slot];
+  if (t < 0 || t >= J.table_len) return;
   T acc = zero<T>();
-  for (int q = 0; q < nm; ++q) acc = add(acc, scl(table[t * nm + q], m[r * nm + q]));
-  apply_corr(F, target[r], acc, scale, r);
+  for (int q = 0; q < J.nm; ++q) acc = add(acc, scl(J.table[t * J.nm + q], J.m[r * J.nm + q]));
+  apply_corr(J.F, J.target[r], acc, J.scale, r);
+}
+
+template <class T>
+__global__ void k_modal(ModalJob<T> J) {
+  modal_row(J, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// One launch over the rows of many jobs (blocks of 256 rows; sc maps each block to its job).
+template <class T>
+__global__ void k_gather_multi(const GatherJob<T>* __restrict__ jobs, Sched sc) {
+  int m, lt;
+  member_of(sc, blockIdx.x, m, lt);
+  gather_row(jobs[m], (long long)lt * blockDim.x + threadIdx.x);
+}
+
+template <class T>
+__global__ void k_modal_multi(const ModalJob<T>* __restrict__ jobs, Sched sc) {
+  int m, lt;
+  member_of(sc, blockIdx.x, m, lt);
+  modal_row(jobs[m], (long long)lt * blockDim.x + threadIdx.x);
 }
 
 // Quintic interpolation of tables sampled every m steps (layered.py:164-187), float32 as on the GPU
@@ -179,69 +224,111 @@ __global__ void k_mixed(long long n, int S, int maxp, const long long* __restric
 // Running DFT monitor (engine.py:564-592). Block size 256; phases for all frequencies in LDS, computed in
 // double from the step counter and cast to complex64 as the reference does.
 template <class S>
-__global__ void k_dft(int nf, long long np, const double* __restrict__ omega, double dt,
-                      const long long* counter, const S* __restrict__ esrc, const S* __restrict__ hsrc,
-                      const long long* __restrict__ eidx, const long long* __restrict__ h1,
-                      const long long* __restrict__ h2, c32* acc_e, c32* acc_h, float* partial) {
-  extern __shared__ c32 ph[];  // [2 * nf]
-  __shared__ float red[256];
-  const double te = (double)counter[0] * dt;
-  const double th = te - 0.5 * dt;
+struct DftJob {
+  int nf;
+  long long np;
+  const double* omega;
+  double dt;
+  const long long* counter;
+  const S* esrc;
+  const S* hsrc;
+  const long long* eidx;
+  const long long* h1;
+  const long long* h2;
+  c32* acc_e;
+  c32* acc_h;
+  float* partial;  // per-block energy partials of this monitor, or null
+};
+
+template <class S>
+__device__ __forceinline__ void dft_block(const DftJob<S>& J, int blk, c32* ph, float* red) {
+  const double te = (double)J.counter[0] * J.dt;
+  const double th = te - 0.5 * J.dt;
+  const int nf = J.nf;
+  const long long np = J.np;
   for (int q = threadIdx.x; q < nf; q += blockDim.x) {
-    const double we = omega[q] * te, wh = omega[q] * th;
+    const double we = J.omega[q] * te, wh = J.omega[q] * th;
     ph[q] = mk((float)cos(we), (float)sin(we));
     ph[nf + q] = mk((float)cos(wh), (float)sin(wh));
   }
   __syncthreads();
-  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long p = (long long)blk * blockDim.x + threadIdx.x;
   float e2 = 0.f;
   if (p < np) {
-    const S se = esrc[eidx[p]];
+    const S se = J.esrc[J.eidx[p]];
     e2 += norm2(se);
     for (int q = 0; q < nf; ++q) {
       const c32 v = mul(ph[q], to_c(se));
-      c32 a = acc_e[q * np + p];
-      acc_e[q * np + p] = add(a, v);
+      c32 a = J.acc_e[q * np + p];
+      J.acc_e[q * np + p] = add(a, v);
     }
-    if (acc_h) {
-      const S sh = scl(0.5f, add(hsrc[h1[p]], hsrc[h2[p]]));
+    if (J.acc_h) {
+      const S sh = scl(0.5f, add(J.hsrc[J.h1[p]], J.hsrc[J.h2[p]]));
       e2 += norm2(sh);
       for (int q = 0; q < nf; ++q) {
         const c32 v = mul(ph[nf + q], to_c(sh));
-        c32 a = acc_h[q * np + p];
-        acc_h[q * np + p] = add(a, v);
+        c32 a = J.acc_h[q * np + p];
+        J.acc_h[q * np + p] = add(a, v);
       }
     }
   }
-  if (partial) {
+  if (J.partial) {
     red[threadIdx.x] = e2;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
       if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
       __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) J.partial[blk] = red[0];
   }
+}
+
+template <class S>
+__global__ void k_dft(DftJob<S> J) {
+  extern __shared__ c32 ph[];  // [2 * nf]
+  __shared__ float red[256];
+  dft_block(J, blockIdx.x, ph, red);
+}
+
+template <class S>
+__global__ void k_dft_multi(const DftJob<S>* __restrict__ jobs, Sched sc) {
+  extern __shared__ c32 ph[];  // [2 * max nf]
+  __shared__ float red[256];
+  int m, lt;
+  member_of(sc, blockIdx.x, m, lt);
+  dft_block(jobs[m], lt, ph, red);
 }
 
 // Sum each monitor's block partials in a fixed order and set the energy slots (engine.py:580-589).
 // mon_off[m] .. mon_off[m+1] are monitor m's partials, mon_group[m] its slot (0, 1), -1 for none.
-__global__ void k_energy(int n_mon, const int* __restrict__ mon_off, const int* __restrict__ mon_group,
-                         const float* __restrict__ partial, float* energy) {
+struct EnergyJob {
+  int n_mon;
+  const int* mon_off;
+  const int* mon_group;
+  const float* partial;
+  float* energy;
+};
+
+__device__ __forceinline__ void energy_sum(const EnergyJob& J) {
   if (threadIdx.x != 0) return;
   double tot[2] = {0.0, 0.0};
   int have[2] = {0, 0};
-  for (int m = 0; m < n_mon; ++m) {
-    const int g = mon_group[m];
+  for (int m = 0; m < J.n_mon; ++m) {
+    const int g = J.mon_group[m];
     if (g < 0) continue;
     double s = 0.0;
-    for (int q = mon_off[m]; q < mon_off[m + 1]; ++q) s += partial[q];
+    for (int q = J.mon_off[m]; q < J.mon_off[m + 1]; ++q) s += J.partial[q];
     tot[g] += s;
     have[g] = 1;
   }
   for (int g = 0; g < 2; ++g)
-    if (have[g]) energy[g] = (float)tot[g];
+    if (have[g]) J.energy[g] = (float)tot[g];
 }
+
+__global__ void k_energy(EnergyJob J) { energy_sum(J); }
+
+// one block per engine
+__global__ void k_energy_multi(const EnergyJob* __restrict__ jobs) { energy_sum(jobs[blockIdx.x]); }
 
 template <class T>
 __global__ void k_timemon(long long np, const long long* __restrict__ eidx, const T* __restrict__ E, T* buf,

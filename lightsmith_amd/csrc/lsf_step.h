@@ -1,12 +1,17 @@
 // The bulk Yee update: one H kernel and one E kernel per time step, each fusing the curl, the CPML
 // psi recursions and (E kernel) the pure-ADE D-form update for its cells.
 //
-// Mapping: a wavefront's 64 lanes take 64 consecutive x cells; each lane holds RY consecutive y rows in
-// registers and marches up a chunk of z planes, carrying the neighbouring plane in registers. x
-// neighbours come from the adjacent lane (cross-lane shuffle; the edge lane loads its halo cell), y
-// neighbours from the lane's own rows (the edge row loads one halo row), z neighbours from the marching
-// registers. Every z iteration issues all of its loads before any store, so one memory round trip
-// covers the iteration (stores to H or E would otherwise order each following load behind them).
+// Mapping: a wavefront's 64 lanes take XW consecutive x cells (XW = 64: one row; XW = 32: two rows) and
+// each lane holds RY consecutive y rows in registers and marches up a chunk of z planes, carrying the
+// neighbouring plane in registers. x neighbours come from the adjacent lane (cross-lane shuffle; the edge
+// lane loads its halo cell), y neighbours from the lane's own rows (the edge row loads one halo row), z
+// neighbours from the marching registers. Every z iteration issues all of its loads before any store, so
+// one memory round trip covers the iteration (stores to H or E would otherwise order each following load
+// behind them).
+//
+// A launch covers the tiles of one engine or of many (a batch): the engines' arguments sit in a device
+// array, a work list maps each tile to its engine, and each block walks the list with a grid stride (one
+// tile per block when the grid covers the list; a persistent loop when it is smaller).
 #pragma once
 
 #include "lsf_common.h"
@@ -28,10 +33,10 @@ struct StepArgs {
   const uint32_t* mid;   // indexed material [N], or null
   const float* mtab;     // indexed material table [256]
   PmlAx pml[3];          // this kernel's field
-  const int* chunk_base;
+  const int* chunk_base; // first D-form slot of each 32-cell chunk [3][N2][N1][nchunk]
   void* P;
   int n_ade;
-  int nchunk;
+  int nchunk;            // 32-cell chunks per x row
   int max_poles;
   int media_stride;
   const float* media;
@@ -39,6 +44,14 @@ struct StepArgs {
   long long* counter;
   int tick;
   int tiles_x, tiles_y, zc, nb, nbp;
+};
+
+// The tiles of one launch: every member's tiles laid end to end.
+struct Sched {
+  const int* inst_of;  // [n_tiles] member of each tile; null: one member (index 0)
+  const int* first;    // [members] first tile of each member
+  int n_tiles;
+  int n_pad;           // n_tiles rounded up to a multiple of 8 (the period of the XCD-aware order)
 };
 
 // D-form media table held in LDS by the E kernel: n_media * (2 + 3 * max_poles) floats
@@ -90,12 +103,6 @@ __device__ __forceinline__ T load_lo(const T* __restrict__ F, const Grid& g, int
   return ld_lo(F, g, i, j, k);
 }
 
-template <int MAT>
-__device__ __forceinline__ float load_cb(const StepArgs& a, const float* tab, int c, unsigned cell, uint32_t ids) {
-  if (MAT == 0) return a.cbd[(unsigned)c * (unsigned)a.g.N + cell];
-  return tab[(ids >> (8 * c)) & 255u];
-}
-
 // psi = b psi + cp der; returns the field increment kp der (when kappa > 1) + psi_new
 // (engine.py:636-643, 656-663) with cp = coef_s c and kp = coef_s (1/kappa - 1).
 template <class T>
@@ -108,41 +115,69 @@ __device__ __forceinline__ bool in_box(const int (&r)[3][2], int i, int j, int k
   return i >= r[0][0] && i < r[0][1] && j >= r[1][0] && j < r[1][1] && k >= r[2][0] && k < r[2][1];
 }
 
+// The y profile of a lane's row r: rows jw + r (lanes of the first x row) or jw + RY + r (the second x row
+// when XW = 32). Both are wavefront-uniform loads.
+template <int XW, int RY>
+__device__ __forceinline__ float4 row_prof(const float4* prof, int jw, int yl, int r, int ny) {
+  if (!prof) return no_pml();
+  const float4 pa = jw + r < ny ? ld_uniform(prof, jw + r) : no_pml();
+  if (XW == 64) return pa;
+  const float4 pb = jw + RY + r < ny ? ld_uniform(prof, jw + RY + r) : no_pml();
+  return yl ? pb : pa;
+}
+
+// Member and member-local tile of work-list entry t.
+__device__ __forceinline__ void member_of(const Sched& sc, int t, int& m, int& lt) {
+  m = 0;
+  lt = t;
+  if (sc.inst_of) {
+    m = ld_uniform(sc.inst_of, t);
+    lt = t - ld_uniform(sc.first, m);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // H update (engine.py:630-643): H_c -= cH [s_d1 (E_d2(+e_d1) - E_d2) - s_d2 (E_d1(+e_d2) - E_d1)] + CPML.
 // ---------------------------------------------------------------------------------------------
-template <class T, int RY, int WAVES, bool PER>
-__global__ void __launch_bounds__(64 * WAVES) k_update_h(StepArgs a) {
+template <class T, int RY, int WAVES, int XW, bool PER>
+__device__ __forceinline__ void h_tile(const StepArgs& a, int t) {
   const Grid& g = a.g;
-  const int t = xcd_tile(blockIdx.x, a.nbp);
-  if (t >= a.nb) return;
   const int lane = threadIdx.x & 63;
+  const int xl = lane & (XW - 1);
+  const int yl = XW == 64 ? 0 : (lane >> 5);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tx = t % a.tiles_x, rest = t / a.tiles_x;
   const int ty = rest % a.tiles_y, tz = rest / a.tiles_y;
-  const int i = tx * 64 + lane;
-  const int j0 = (ty * WAVES + wv) * RY;
-  if (j0 >= g.n[1]) return;
+  const int jw = (ty * WAVES + wv) * (64 / XW) * RY;  // the wavefront's first row
+  if (jw >= g.n[1]) return;
+  const int i = tx * XW + xl;
+  const int j0 = jw + yl * RY;
   const int z0 = tz * a.zc, z1 = min(z0 + a.zc, g.n[2]);
   const unsigned N = (unsigned)g.N, nx = g.n[0], ny = g.n[1];
-  const T* __restrict__ Ex = (const T*)a.E;
-  const T* __restrict__ Ey = Ex + N;
-  const T* __restrict__ Ez = Ex + 2 * N;
-  T* __restrict__ H = (T*)a.H;
+  const gptr<const T> Eg = as_global((const T*)a.E);
+  const gptr<T> Hg = as_global((T*)a.H);
   const bool inx = i < g.n[0];
-  const int ih = tx * 64 + 64;
-  const bool hx_ok = (lane == 63) && ih < g.n[0];
+  const int ih = tx * XW + XW;
+  const bool hx_ok = (xl == XW - 1) && ih < g.n[0];
+  gptr<T> psi[3][3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) psi[ax][c] = as_global((T*)a.pml[ax].psi[c]);
 
   // CPML profiles: x per lane and y per row are constant over the march
-  const float4 px = (a.pml[0].prof && inx) ? a.pml[0].prof[i] : no_pml();
+  const float4 px = (a.pml[0].prof && inx) ? ((const float4*)as_global(a.pml[0].prof))[i] : no_pml();
   float4 py[RY];
 #pragma unroll
-  for (int r = 0; r < RY; ++r) py[r] = (a.pml[1].prof && j0 + r < g.n[1]) ? ld_uniform(a.pml[1].prof, j0 + r) : no_pml();
+  for (int r = 0; r < RY; ++r) py[r] = row_prof<XW, RY>(a.pml[1].prof, jw, yl, r, g.n[1]);
 
   T cEx[RY + 1], cEy[RY], cEz[RY + 1], hEy[RY], hEz[RY];
   T nEx[RY + 1], nEy[RY], nEz[RY + 1], nhEy[RY], nhEz[RY];
 
   auto load_plane = [&](int k, T* xE, T* yE, T* zE, T* hy, T* hz) {
+    const T* __restrict__ Ex = (const T*)Eg;
+    const T* __restrict__ Ey = Ex + N;
+    const T* __restrict__ Ez = Ex + 2 * N;
 #pragma unroll
     for (int r = 0; r <= RY; ++r) {
       const int j = j0 + r;
@@ -163,6 +198,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_h(StepArgs a) {
     const float4 pz = a.pml[2].prof ? ld_uniform(a.pml[2].prof, k) : no_pml();
     // ---- loads: the next E plane, old H, psi
     load_plane(k + 1, nEx, nEy, nEz, nhEy, nhEz);
+    T* __restrict__ H = (T*)Hg;
     bool st[RY][3];
     T hv[RY][3];
     T ps[RY][3][2];  // psi of component c along axes d1 = c+1, d2 = c+2 (mod 3)
@@ -182,7 +218,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_h(StepArgs a) {
           const int p = (int)pr[r][c][s].w;
           ps[r][c][s] = zero<T>();
           if (st[r][c] && p >= 0)
-            ps[r][c][s] = ((const T*)a.pml[ax].psi[c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)];
+            ps[r][c][s] = ((const T*)psi[ax][c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)];
         }
       }
     }
@@ -192,7 +228,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_h(StepArgs a) {
     for (int r = 0; r < RY; ++r) {
       ezp[r] = shfl_dn(cEz[r]);
       eyp[r] = shfl_dn(cEy[r]);
-      if (lane == 63) {
+      if (xl == XW - 1) {
         ezp[r] = hEz[r];
         eyp[r] = hEy[r];
       }
@@ -241,7 +277,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_h(StepArgs a) {
         for (int s = 0; s < 2; ++s) {
           const int ax = (c + 1 + s) % 3;
           const int p = (int)pr[r][c][s].w;
-          if (p >= 0) ((T*)a.pml[ax].psi[c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)] = ps[r][c][s];
+          if (p >= 0) ((T*)psi[ax][c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)] = ps[r][c][s];
         }
       }
     }
@@ -258,6 +294,21 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_h(StepArgs a) {
   }
 }
 
+// LOOP false: one tile per block (the grid covers the work list). LOOP true: a persistent grid whose blocks
+// stride over the work list (a separate instantiation: the loop costs registers the one-tile form does not).
+template <class T, int RY, int WAVES, int XW, bool PER, bool LOOP>
+__global__ void __launch_bounds__(64 * WAVES) k_update_h(const StepArgs* __restrict__ args, Sched sc) {
+  for (int b = blockIdx.x; b < sc.n_pad; b += gridDim.x) {
+    const int t = xcd_tile(b, sc.n_pad);
+    if (t < sc.n_tiles) {
+      int m, lt;
+      member_of(sc, t, m, lt);
+      h_tile<T, RY, WAVES, XW, PER>(args[m], lt);
+    }
+    if (!LOOP) break;
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // E update (engine.py:645-663) + pure-ADE groups (engine.py:769-793):
 // E_c += cb [s_d1 (H_d2 - H_d2(-e_d1)) - s_d2 (H_d1 - H_d1(-e_d2))] + CPML. A D-form node (NP poles at most)
@@ -266,44 +317,39 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_h(StepArgs a) {
 // D^{n+1} = D^n + dE and D^n = eps_inf E^n + sum P^n (the reference's D-form, without storing D).
 // The H plane and the material of plane k+1 are loaded during iteration k.
 // ---------------------------------------------------------------------------------------------
-template <class T, int MAT, int RY, int WAVES, bool PER, int NP>
-__global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
-  __shared__ float tab[256];
-  __shared__ float med_tab[MAX_MEDIA_FLOATS];
-  if (a.tick && blockIdx.x == 0 && threadIdx.x == 0) {
-    const long long n = a.counter[0];
-    a.counter[1] = n;
-    a.counter[0] = n + 1;
-  }
-  if (MAT == 1)
-    for (int q = threadIdx.x; q < 256; q += 64 * WAVES) tab[q] = a.mtab[q];
-  if (NP > 0)
-    for (int q = threadIdx.x; q < a.n_media_floats; q += 64 * WAVES) med_tab[q] = a.media[q];
-  __syncthreads();
+template <class T, int MAT, int RY, int WAVES, int XW, bool PER, int NP>
+__device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* tab, const float* med_tab) {
   const Grid& g = a.g;
-  const int t = xcd_tile(blockIdx.x, a.nbp);
-  if (t >= a.nb) return;
   const int lane = threadIdx.x & 63;
+  const int xl = lane & (XW - 1);
+  const int yl = XW == 64 ? 0 : (lane >> 5);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tx = t % a.tiles_x, rest = t / a.tiles_x;
   const int ty = rest % a.tiles_y, tz = rest / a.tiles_y;
-  const int i = tx * 64 + lane;
-  const int j0 = (ty * WAVES + wv) * RY;
-  if (j0 >= g.n[1]) return;
+  const int jw = (ty * WAVES + wv) * (64 / XW) * RY;
+  if (jw >= g.n[1]) return;
+  const int i = tx * XW + xl;
+  const int j0 = jw + yl * RY;
   const int z0 = tz * a.zc, z1 = min(z0 + a.zc, g.n[2]);
   const unsigned N = (unsigned)g.N, nx = g.n[0], ny = g.n[1];
-  const T* __restrict__ Hx = (const T*)a.H;
-  const T* __restrict__ Hy = Hx + N;
-  const T* __restrict__ Hz = Hx + 2 * N;
-  T* __restrict__ E = (T*)a.E;
+  const gptr<const T> Hg = as_global((const T*)a.H);
+  const gptr<T> Eg = as_global((T*)a.E);
+  const gptr<const float> cbdg = as_global(a.cbd);
+  const gptr<const uint32_t> midg = as_global(a.mid);
+  gptr<T> psi[3][3];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) psi[ax][c] = as_global((T*)a.pml[ax].psi[c]);
   const bool inx = i < g.n[0];
-  const int il = tx * 64 - 1;
-  const bool xl_ok = (lane == 0) && il >= 0;
-  const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  const float4 px = (a.pml[0].prof && inx) ? a.pml[0].prof[i] : no_pml();
+  const int il = tx * XW - 1;
+  const bool xl_ok = (xl == 0) && il >= 0;
+  // lanes of this lane's x row below it
+  const unsigned long long below = xl == 0 ? 0ull : (((~0ull) >> (64 - xl)) << (yl * 32));
+  const float4 px = (a.pml[0].prof && inx) ? ((const float4*)as_global(a.pml[0].prof))[i] : no_pml();
   float4 py[RY];
 #pragma unroll
-  for (int r = 0; r < RY; ++r) py[r] = (a.pml[1].prof && j0 + r < g.n[1]) ? ld_uniform(a.pml[1].prof, j0 + r) : no_pml();
+  for (int r = 0; r < RY; ++r) py[r] = row_prof<XW, RY>(a.pml[1].prof, jw, yl, r, g.n[1]);
 
   // H rows r = -1 .. RY-1 of H_x and H_z are stored at r + 1
   T cHx[RY + 1], cHy[RY], cHz[RY + 1], xHy[RY], xHz[RY];
@@ -312,18 +358,30 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
   // material of the current / next plane: packed 8-bit ids (MAT 1) or the three coefficients (MAT 0)
   uint32_t id_c[RY], id_n[RY];
   float cbd_c[RY][MAT == 0 ? 3 : 1], cbd_n[RY][MAT == 0 ? 3 : 1];
-  int ab_c[RY][3], ab_n[RY][3];  // first D-form slot of each row's 64-cell chunk per component (wave-uniform)
+  int ab_c[RY][3], ab_n[RY][3];  // first D-form slot of each row's x chunk per component
+  const unsigned cq = (unsigned)tx * (XW / 32);
   auto load_bases = [&](int k, int (*ab)[3]) {
 #pragma unroll
     for (int r = 0; r < RY; ++r)
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
-        ab[r][c] = (NP > 0 && j0 + r < g.n[1] && k < g.n[2])
-                       ? ld_uniform(a.chunk_base, (((unsigned)c * g.n[2] + (unsigned)k) * ny + (unsigned)(j0 + r)) * a.nchunk + tx)
-                       : 0;
+      for (int c = 0; c < 3; ++c) {
+        int va = 0, vb = 0;
+        if (NP > 0 && k < g.n[2]) {
+          const unsigned rowc = ((unsigned)c * g.n[2] + (unsigned)k) * ny;
+          if (jw + r < g.n[1]) va = ld_uniform(a.chunk_base, (rowc + (unsigned)(jw + r)) * a.nchunk + cq);
+          if (XW == 32 && jw + RY + r < g.n[1])
+            vb = ld_uniform(a.chunk_base, (rowc + (unsigned)(jw + RY + r)) * a.nchunk + cq);
+        }
+        ab[r][c] = (XW == 32 && yl) ? vb : va;
+      }
   };
 
-  auto load_plane = [&](int k, T* xH, T* yH, T* zH, T* hy, T* hz, uint32_t* ids, float (*cbd)[MAT == 0 ? 3 : 1]) {
+  auto load_plane = [&](int k, T* xH, T* yH, T* zH, T* hy, T* hz, uint32_t* ids, float (*cb)[MAT == 0 ? 3 : 1]) {
+    const T* __restrict__ Hx = (const T*)Hg;
+    const T* __restrict__ Hy = Hx + N;
+    const T* __restrict__ Hz = Hx + 2 * N;
+    const uint32_t* __restrict__ mid = (const uint32_t*)midg;
+    const float* __restrict__ cbd = (const float*)cbdg;
 #pragma unroll
     for (int r = -1; r < RY; ++r) {
       const int j = j0 + r;
@@ -337,10 +395,10 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
         hz[r] = lok ? load_lo<PER>(Hz, g, il, j, k) : zero<T>();
         const unsigned cell = cell32(nx, ny, i, ok ? j : 0, ok ? k : 0);
         if (MAT == 1) {
-          ids[r] = ok ? a.mid[cell] : 0u;
+          ids[r] = ok ? mid[cell] : 0u;
         } else {
 #pragma unroll
-          for (int c = 0; c < (MAT == 0 ? 3 : 1); ++c) cbd[r][c] = ok ? a.cbd[(unsigned)c * N + cell] : 0.f;
+          for (int c = 0; c < (MAT == 0 ? 3 : 1); ++c) cb[r][c] = ok ? cbd[(unsigned)c * N + cell] : 0.f;
         }
       }
     }
@@ -354,14 +412,17 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
   for (int r = 0; r < RY; ++r) {
     const int j = j0 + r, k = z0 - 1;
     const bool ok = inx && j < g.n[1] && k >= 0;
+    const T* __restrict__ Hx = (const T*)Hg;
     pHx[r] = ok ? load_lo<PER>(Hx, g, i, j, k) : zero<T>();
-    pHy[r] = ok ? load_lo<PER>(Hy, g, i, j, k) : zero<T>();
+    pHy[r] = ok ? load_lo<PER>(Hx + N, g, i, j, k) : zero<T>();
   }
   load_plane(z0, cHx, cHy, cHz, xHy, xHz, id_c, cbd_c);
   load_bases(z0, ab_c);
 
-  Pair<T>* P = (Pair<T>*)a.P;
+  const gptr<Pair<T>> Pg = as_global((Pair<T>*)a.P);
   for (int k = z0; k < z1; ++k) {
+    T* __restrict__ E = (T*)Eg;
+    Pair<T>* __restrict__ P = (Pair<T>*)Pg;
     const float4 pz = a.pml[2].prof ? ld_uniform(a.pml[2].prof, k) : no_pml();
     // ---- loads: next H plane and material (prefetch), old E, psi, pole state
     if (k + 1 < z1) {
@@ -388,11 +449,11 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
           const int p = (int)pr.w;
           ps[r][c][s] = zero<T>();
           if (st[r][c] && p >= 0)
-            ps[r][c][s] = ((const T*)a.pml[ax].psi[c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)];
+            ps[r][c][s] = ((const T*)psi[ax][c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)];
         }
         aq[r][c] = -1;
         if (NP > 0) {
-          // D-form node: its slot in the compact pole arrays (chunk base + ballot rank in the wavefront)
+          // D-form node: its slot in the compact pole arrays (chunk base + rank among its row's lanes)
           const bool dsp = st[r][c] && coef(r, c) < 0.f;
           const unsigned long long m = __ballot(dsp);
           if (m && dsp) aq[r][c] = ab_c[r][c] + __popcll(m & below);
@@ -410,7 +471,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
     for (int r = 0; r < RY; ++r) {
       hzm[r] = shfl_up(cHz[r + 1]);
       hym[r] = shfl_up(cHy[r]);
-      if (lane == 0) {
+      if (xl == 0) {
         hzm[r] = xHz[r];
         hym[r] = xHy[r];
       }
@@ -479,7 +540,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
           const int ax = (c + 1 + s) % 3;
           const float4 pr = ax == 0 ? px : (ax == 1 ? py[r] : pz);
           const int p = (int)pr.w;
-          if (p >= 0) ((T*)a.pml[ax].psi[c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)] = ps[r][c][s];
+          if (p >= 0) ((T*)psi[ax][c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)] = ps[r][c][s];
         }
         if (NP > 0 && aq[r][c] >= 0) {
 #pragma unroll
@@ -508,6 +569,43 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(StepArgs a) {
         xHz[r] = nxHz[r];
       }
     }
+  }
+}
+
+// Each member's tile 0 advances its step counter: counter[1] = n, counter[0] = n + 1 (engine.py:678). The LDS
+// tables (indexed material, D-form media) are loaded again whenever the block moves to another member.
+template <class T, int MAT, int RY, int WAVES, int XW, bool PER, int NP, bool LOOP>
+__global__ void __launch_bounds__(64 * WAVES) k_update_e(const StepArgs* __restrict__ args, Sched sc) {
+  __shared__ float tab[MAT == 1 ? 256 : 1];
+  __shared__ float med_tab[NP > 0 ? MAX_MEDIA_FLOATS : 1];
+  int cur = -1;
+  for (int b = blockIdx.x; b < sc.n_pad; b += gridDim.x) {
+    const int t = xcd_tile(b, sc.n_pad);
+    if (t >= sc.n_tiles) {
+      if (!LOOP) break;
+      continue;
+    }
+    int m, lt;
+    member_of(sc, t, m, lt);
+    const StepArgs& a = args[m];
+    if (a.tick && lt == 0 && threadIdx.x == 0) {
+      long long* cnt = (long long*)as_global(a.counter);
+      const long long n = cnt[0];
+      cnt[1] = n;
+      cnt[0] = n + 1;
+    }
+    if ((MAT == 1 || NP > 0) && m != cur) {
+      if (cur >= 0) __syncthreads();  // the previous member's readers are done
+      if (MAT == 1)
+        for (int q = threadIdx.x; q < 256; q += 64 * WAVES) tab[q] = ((const float*)as_global(a.mtab))[q];
+      if (NP > 0)
+        for (int q = threadIdx.x; q < a.n_media_floats; q += 64 * WAVES)
+          med_tab[q] = ((const float*)as_global(a.media))[q];
+      __syncthreads();
+      cur = m;
+    }
+    e_tile<T, MAT, RY, WAVES, XW, PER, NP>(a, lt, tab, med_tab);
+    if (!LOOP) break;
   }
 }
 

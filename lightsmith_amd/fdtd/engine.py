@@ -156,14 +156,17 @@ def to_reference(native, shape) -> np.ndarray:
     return ((c * n0 + i) * n1 + j) * n2 + k
 
 
+CHUNK = 32  # x cells per chunk of the D-form slot table (the narrowest wavefront row of the E kernel)
+
+
 def ade_chunk_base(nodes, shape) -> np.ndarray:
-    """First compact slot of each 64-cell chunk (c, k, j, x // 64) for D-form nodes sorted by native index: the
-    E kernel finds a node's slot as chunk_base + (D-form lanes below it in its wavefront)."""
+    """First compact slot of each 32-cell chunk (c, k, j, x // 32) for D-form nodes sorted by native index: the
+    E kernel finds a node's slot as the base of its row's chunk + (D-form lanes below it in that row)."""
     n0, n1, n2 = shape
     N = n0 * n1 * n2
-    nchunk = (n0 + 63) // 64
+    nchunk = (n0 + CHUNK - 1) // CHUNK
     c, cell = np.divmod(np.asarray(nodes, dtype=np.int64), N)
-    keys = (c * (n1 * n2) + cell // n0) * nchunk + (cell % n0) // 64
+    keys = (c * (n1 * n2) + cell // n0) * nchunk + (cell % n0) // CHUNK
     return np.searchsorted(keys, np.arange(3 * n1 * n2 * nchunk), side="left").astype(np.int32)
 
 
@@ -227,8 +230,9 @@ def _device_of(device) -> torch.device:
 class Engine:
     """FDTD engine whose stepping runs in the HIP library (engine.py:230-315 constructor contract)."""
 
-    # launch tiling (rows per lane, waves per block, z planes per block); see lsf_set_tiling
-    tiling = (1, 8, 16)
+    # launch tiling (rows per lane, wavefronts per block, x cells per wavefront row, z planes per block);
+    # see lsf_set_tiling
+    tiling = (1, 8, 64, 16)
 
     def __init__(self, device=None, *, shape, dx, dt: float, single: bool = True, inv_eps: np.ndarray,
                  groups=(), pml: PMLSpec | None = None, sources=(), monitors=(), dft_stride: int = 1,
@@ -263,6 +267,8 @@ class Engine:
         n0, n1, n2 = self.shape
         self.N = n0 * n1 * n2
         self._keep = []  # device tables whose lifetime is the engine's
+        self._subs = []  # sub-engines stepped inside this one
+        self._held = 0  # batches stepping this engine
         self.E_flat = torch.zeros(3 * self.N, dtype=self.fdt, device=self._dev)
         self.H_flat = torch.zeros(3 * self.N, dtype=self.fdt, device=self._dev)
         self.E = self.E_flat.view(3, n2, n1, n0).permute(0, 3, 2, 1)
@@ -289,8 +295,8 @@ class Engine:
         handle = ctypes.c_void_p()
         check(self.lib.lsf_create(ctypes.byref(g), ctypes.byref(handle)), "lsf_create")
         self._h = handle
-        ry, waves, zc = self.tiling
-        check(self.lib.lsf_set_tiling(self._h, ry, waves, zc), "lsf_set_tiling")
+        ry, waves, xw, zc = self.tiling
+        check(self.lib.lsf_set_tiling(self._h, ry, waves, xw, zc), "lsf_set_tiling")
 
         inv_eps = np.asarray(inv_eps, dtype=float)
         if inv_eps.shape != (3,) + self.shape:
@@ -745,7 +751,32 @@ class Engine:
             check(self.lib.lsf_graph_reset(self._h), "lsf_graph_reset")
             self._graphed = False
 
+    def add_subengine(self, sub: "Engine"):
+        """Step `sub` (created with _sub=True on this engine's counter) inside this engine's H and E phases."""
+        check(self.lib.lsf_add_subengine(self._h, sub._h), "lsf_add_subengine")
+        self._subs.append(sub)
+
+    def state_tensors(self) -> list:
+        """Every device tensor the engine reads or writes while stepping (fields, counter, psi, pole and group
+        state, source lines, monitor accumulators, sub-engines'), each once: a snapshot of these restores the
+        run exactly."""
+        out, seen = [], set()
+        ts = [self.E_flat, self.H_flat, self._counter, self.energy, *self._keep]
+        for m in self.monitors:
+            ts += [getattr(m, "acc_e", None), getattr(m, "acc_h", None)]
+        for m in self.time_monitors:
+            ts += [getattr(m, "buf", None), getattr(m, "pos", None)]
+        for sub in self._subs:
+            ts += sub.state_tensors()
+        for t in ts:
+            if isinstance(t, torch.Tensor) and t.data_ptr() not in seen:
+                seen.add(t.data_ptr())
+                out.append(t)
+        return out
+
     def close(self):
+        if self._held:
+            raise RuntimeError("the engine is stepped by a Batch: close the batch first")
         h = getattr(self, "_h", None)
         if h is not None and h.value:
             torch.cuda.synchronize(self._dev)
@@ -754,7 +785,8 @@ class Engine:
 
     def __del__(self):
         try:
-            self.close()
+            if not self._held:
+                self.close()
         except Exception:  # noqa: BLE001 (interpreter shutdown)
             pass
 

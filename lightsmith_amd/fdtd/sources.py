@@ -338,7 +338,7 @@ class BlochPlaneSource(_Native):
                      monitors=[], dft_stride=1, periodic=(True, True, False), bloch=(*self.kpar, 0.0), _sub=True,
                      _counter=eng._counter)
         self.col = col
-        check(eng.lib.lsf_add_subengine(eng._h, col._h), "lsf_add_subengine")
+        eng.add_subengine(col)
         nx, ny = self.shape[0] - 2, self.shape[1] - 2
         k = self.k_src
         cE = self.dt / (self.eps_b * self.spacing[2])

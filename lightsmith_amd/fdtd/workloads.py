@@ -120,3 +120,101 @@ def throughput_engine(scene: dict, device="cuda:0", max_steps: int = 10000, tili
     finally:
         Engine.tiling = old
     return eng
+
+
+# --- batched oblique sweep (periodic.py:392-459, 503-519) --------------------------------------------
+
+def oblique_sweep_scene(n_inst: int = 64, nz: int = 256, size_lo: int = 48, size_hi: int = 96, seed: int = 2,
+                        pattern_seed: int = 3, d: float = 10.0, pml: int = 16, gap: int = 40, pad: int = 8,
+                        layer: int = 40, eps_pillar: float = 12.1, eps_sub: float = 2.1, courant: float = 0.99,
+                        n_angles: int = 8, n_wl: int = 8) -> list:
+    """Independent runs of an oblique sweep over periodic metasurface cells: angles linspace(0, 85, n_angles)
+    deg x wavelengths linspace(500, 1500, n_wl) nm, polarisation alternating s/p by run index. Each run has
+    its own unit cell of nx, ny ~ U{size_lo..size_hi} real cells (seed) plus ghost planes, nz nodes along z
+    with a `pml`-cell CPML at both ends, a binary pillar pattern (eps 1.0 / eps_pillar on 4 x 4-cell blocks,
+    p = 0.5, pattern_seed) in a `layer`-cell layer on an eps_sub substrate, the reference's z layout
+    (k_r = pml + gap, k_src = k_r + 2, k_top = k_src + pad, k_t = k_bot + 3; periodic.py:181-186), its time
+    step, DFT stride and pulse half-width (periodic.py:304-331, 418-421). Angle 0 runs at normal incidence
+    (real fields, a 1D-fed TFSF plane); the others at fixed k_par with a Bloch plane source (complex
+    fields)."""
+    rng = np.random.default_rng(seed)
+    prng = np.random.default_rng(pattern_seed)
+    angles = np.linspace(0.0, 85.0, n_angles)
+    wls = np.linspace(500.0, 1500.0, n_wl)
+    k_r = pml + gap
+    k_src = k_r + 2
+    k_top = k_src + pad
+    k_bot = k_top + layer
+    k_t = k_bot + 3
+    if k_t + 2 >= nz - pml:
+        raise ValueError("nz too small for the layout")
+    runs = []
+    for q in range(n_inst):
+        theta, lam = float(angles[(q // n_wl) % n_angles]), float(wls[q % n_wl])
+        pol = "s" if q % 2 == 0 else "p"
+        nx, ny = (int(v) for v in rng.integers(size_lo, size_hi + 1, size=2))
+        shape = (nx + 2, ny + 2, nz)
+        blocks = prng.random(((nx + 3) // 4, (ny + 3) // 4)) < 0.5
+        pattern = np.kron(blocks, np.ones((4, 4), bool))[:nx, :ny]
+        eps = np.ones(shape)
+        eps[1:nx + 1, 1:ny + 1, k_top:k_bot] = np.where(pattern[:, :, None], eps_pillar, 1.0)
+        eps[:, :, k_bot:] = eps_sub
+        inv_eps = np.broadcast_to(1.0 / eps, (3,) + shape).copy()
+        omega = 2 * np.pi / lam
+        k = omega * math.sin(math.radians(theta))  # superstrate eps = 1
+        kpar = (k, 0.0)
+        inv2 = 3.0 / d ** 2  # both periodic axes have more than one cell (periodic.py:304-314)
+        dt = courant / math.sqrt(inv2)
+        gaps = [abs(omega - k / math.sqrt(e)) for e in (1.0, eps_sub)]
+        half = float(np.clip(min(gaps) / 4, 0.01 * omega, 0.15 * omega))  # periodic.py:324-331
+        stride = max(1, int(2 * np.pi / (2 * omega + 4 * half) / dt))  # periodic.py:421
+        runs.append({"shape": shape, "spacing": (d, d, d), "inv_eps": inv_eps, "theta": theta, "wavelength": lam,
+                     "pol": pol, "kpar": kpar, "dt": dt, "omega": omega, "omega_half": half, "stride": stride,
+                     "k_r": k_r, "k_src": k_src, "k_top": k_top, "k_t": k_t, "pml": pml, "eps_sub": eps_sub,
+                     "n_cells": (nx, ny)})
+    return runs
+
+
+def common_stride(runs: list) -> int:
+    """A DFT stride every run of a batch can use: the smallest run stride (each run's stride is its
+    aliasing-safe maximum, so a smaller one is valid for it)."""
+    return min(r["stride"] for r in runs)
+
+
+def oblique_engine(run: dict, device="cuda:0", max_steps: int = 4000, stride: int | None = None):
+    """The HIP engine of one sweep run with its source and the reflection, transmission and incident
+    monitors (periodic.py:392-459)."""
+    from .engine import DFTMonitor, Engine, PMLSpec
+    from .run import plane_points
+    from .sources import BlochPlaneSource, TFSFPlaneWave
+
+    shape, d = run["shape"], run["spacing"]
+    lam_max = run["wavelength"]
+    pml = PMLSpec(cells=((0, 0), (0, 0), (run["pml"], run["pml"])), alpha_max=0.05 * 2 * np.pi / lam_max,
+                  eps_ref=1.0, eps_ref_high=run["eps_sub"])
+    stride = stride or run["stride"]
+    om = np.array([run["omega"]])
+    mon_r = DFTMonitor(om, *plane_points(shape, run["n_cells"], run["k_r"]), energy=1)
+    mon_t = DFTMonitor(om, *plane_points(shape, run["n_cells"], run["k_t"]), energy=1)
+    if run["theta"] == 0.0:
+        phi = 0.0
+        ehat = np.array([-np.sin(phi), np.cos(phi), 0.0]) if run["pol"] == "s" else np.array([1.0, 0.0, 0.0])
+        origin = np.array([-(shape[0] - 2) * d[0] / 2 - d[0], -(shape[1] - 2) * d[1] / 2 - d[1], -run["k_top"] * d[2]])
+        r0 = origin + np.array([1, 1, run["k_top"]]) * np.asarray(d)
+        big = 1 << 30
+        src = TFSFPlaneWave(shape=shape, origin=origin, dx=d, dt=run["dt"], r0=r0, lo=np.array([-big, -big, run["k_src"]]),
+                            hi=np.array([big, big, big]), khat=np.array([0.0, 0.0, 1.0]), ehat=ehat, eps_b=1.0,
+                            omega_c=run["omega"], omega_half=run["omega_half"], max_steps=max_steps,
+                            periodic=(True, True, False))
+        mon_i = DFTMonitor(om, *src.probe(), source=src)
+        bloch = (0.0, 0.0, 0.0)
+    else:
+        col_pml = PMLSpec(cells=pml.cells, alpha_max=pml.alpha_max, eps_ref=1.0)
+        src = BlochPlaneSource(shape=shape, spacing=d, dt=run["dt"], k_src=run["k_src"], k_launch=run["k_src"] - 1,
+                               kpar=run["kpar"], pol=run["pol"], eps_b=1.0, pml=col_pml, omega_c=run["omega"],
+                               omega_half=run["omega_half"], max_steps=max_steps)
+        mon_i = DFTMonitor(om, *src.probe(run["k_top"]), source=src)
+        bloch = (run["kpar"][0], run["kpar"][1], 0.0)
+    return Engine(device, shape=shape, dx=d, dt=run["dt"], single=True, inv_eps=run["inv_eps"], groups=[], pml=pml,
+                  sources=[src], monitors=[mon_r, mon_t, mon_i], dft_stride=stride, periodic=(True, True, False),
+                  bloch=bloch)

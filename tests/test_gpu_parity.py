@@ -45,7 +45,8 @@ def test_pec_box_random_fields(gpu):
     _compare(o, g, "PEC box 60 steps")
 
 
-@pytest.mark.parametrize("tiling", [(1, 4, 5), (2, 4, 16), (2, 8, 7), (1, 8, 3)])
+@pytest.mark.parametrize("tiling", [(1, 4, 64, 5), (2, 4, 64, 16), (2, 8, 64, 7), (1, 8, 64, 3), (1, 4, 32, 6),
+                                    (2, 8, 32, 4)])
 def test_tilings_agree(gpu, tiling):
     shape = (67, 45, 23)
     eps = np.random.default_rng(3).uniform(1, 4, size=(3,) + shape)

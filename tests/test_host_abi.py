@@ -43,7 +43,9 @@ def test_abi_argument_errors_without_a_gpu():
     g = _lib.LsfGrid()
     g.shape[:] = [2, 8, 8]
     assert lib.lsf_create(ctypes.byref(g), ctypes.byref(out)) == _lib.LSF_EINVAL
-    assert lib.lsf_set_tiling(None, 2, 4, 16) == _lib.LSF_EINVAL
+    assert lib.lsf_set_tiling(None, 2, 4, 64, 16) == _lib.LSF_EINVAL
+    assert lib.lsf_batch_create(None, 0, ctypes.byref(out)) == _lib.LSF_EINVAL
+    assert lib.lsf_batch_step(None, 1, None) == _lib.LSF_EINVAL
     with pytest.raises(ValueError):
         _lib.check(_lib.LSF_EINVAL, "x")
     with pytest.raises(RuntimeError):
@@ -74,21 +76,27 @@ def test_native_index_round_trip_and_layout():
     assert np.array_equal(native[nat], A.reshape(-1)[ref])
 
 
-@pytest.mark.parametrize("shape", [(70, 9, 6), (64, 3, 4), (130, 5, 5)])
-def test_ade_chunk_base_reproduces_the_compact_slots(shape):
-    """Emulate the E kernel: a node's slot = chunk_base[chunk] + number of D-form nodes at lower x in the
-    same 64-cell chunk (the wavefront ballot). It must equal the node's position in native order."""
+@pytest.mark.parametrize("xw", [64, 32])
+@pytest.mark.parametrize("shape", [(70, 9, 6), (64, 3, 4), (130, 5, 5), (33, 4, 3)])
+def test_ade_chunk_base_reproduces_the_compact_slots(shape, xw):
+    """Emulate the E kernel: a wavefront row covers xw x cells; a node's slot = the base of the row's first
+    32-cell chunk + number of D-form nodes at lower x in the same row (the ballot restricted to the row). It
+    must equal the node's position in native order."""
     rng = np.random.default_rng(1)
     N = int(np.prod(shape))
     nodes = np.sort(rng.choice(3 * N, size=3 * N // 7, replace=False))
     base = peng.ade_chunk_base(nodes, shape)
     n0, n1, n2 = shape
+    assert len(base) == 3 * n1 * n2 * ((n0 + 31) // 32)
     c, cell = np.divmod(nodes, N)
     i = cell % n0
-    chunk = (c * (n1 * n2) + cell // n0) * ((n0 + 63) // 64) + i // 64
+    row = c * (n1 * n2) + cell // n0
+    tile = i // xw
+    chunk = row * ((n0 + 31) // 32) + tile * (xw // 32)
+    group = row * ((n0 + xw - 1) // xw) + tile
     rank = np.zeros(len(nodes), int)
-    for q in np.unique(chunk):
-        sel = np.flatnonzero(chunk == q)
+    for q in np.unique(group):
+        sel = np.flatnonzero(group == q)
         rank[sel] = np.argsort(np.argsort(i[sel]))
     assert np.array_equal(base[chunk] + rank, np.arange(len(nodes)))
 
