@@ -169,6 +169,32 @@ int lsf_graph_capture(lsf_engine* eng, int stride, void* stream);
 int lsf_graph_replay(lsf_engine* eng, int n, void* stream);
 int lsf_graph_reset(lsf_engine* eng);
 
+/* ---- Adjoint (gradient) support -------------------------------------------------------------------
+ * No reference counterpart: the reference FDTD has no adjoint (its inverse design runs on FDFD). An adjoint
+ * engine is an engine created like its forward twin (same grid, material and CPML profiles; its own field,
+ * counter and CPML state arrays, which hold the adjoint variables e~ = cb dL/dE, h~ = -cH dL/dH and the
+ * transposed psi terms) and switched to the transposed, time-reversed update. Each lsf_step of it reverses one
+ * forward step, last step first. Real fields, no periodic axes, no D-form nodes. */
+int lsf_set_adjoint(lsf_engine* eng, int on);
+/* Forward tape: during the H phase of each forward step n, record X = curl- H^{n+1/2} (the term cb multiplies
+ * in the E update) at n_nodes design E nodes (native flat indices, interior) into dev_tape[n - base][node]
+ * (float), for 0 <= n - dev_base[0] < n_slots. */
+int lsf_add_tape(lsf_engine* eng, int64_t n_nodes, const int64_t* dev_nodes, float* dev_tape, int64_t n_slots,
+                 const int64_t* dev_base);
+/* Adjoint engine: after each adjoint H half-step reversing forward step n = n_steps - 1 - steps done,
+ * dev_g[q] += (e~/cb)[node q] * tape[n - base][q] (double), i.e. dL/dcb at each design node. */
+int lsf_add_adjoint_gradient(lsf_engine* eng, int64_t n_nodes, const int64_t* dev_nodes, const float* dev_cb,
+                             const float* dev_tape, int64_t n_slots, const int64_t* dev_base, int64_t n_steps,
+                             double* dev_g);
+/* Adjoint engine: the source of a DFT monitor objective. dev_W[nf][np] complex64 = conj(dL/dacc) (torch's
+ * gradient of the monitor's E or H accumulator). E side: after the adjoint E half-step reaching E^n, for
+ * n > 0, n % stride == 0: F[t1[p]] += scale[p] * sum_f Re(W[f,p] exp(i omega_f n dt)) (scale = cb at the node).
+ * H side: after the adjoint H half-step reaching H^{n-1/2}: both t1[p] and t2[p] get the term with
+ * t = n dt - dt/2 (scale = -cH / 2). The source of E^{n_steps} itself is the caller's initial condition. */
+int lsf_add_adjoint_source(lsf_engine* eng, int h_side, int nf, int64_t n_pts, const double* dev_omega, double dt,
+                           const void* dev_W, const int64_t* dev_t1, const int64_t* dev_t2, const float* dev_scale,
+                           int64_t n_steps, int stride);
+
 /* ---- Batches: many independent engines stepped together -----------------------------------------
  * No reference counterpart: the reference runs the angles/wavelengths/polarisations of an oblique sweep
  * one Engine after another (periodic.py:503-519). A batch steps its members with shared launches: per

@@ -31,7 +31,10 @@ int hip_check(hipError_t e, const char* what) {
   return fail(LSF_EHIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-enum OpKind { OP_GATHER, OP_MODAL, OP_TABLE, OP_VIEW, OP_LINE_H, OP_LINE_E, OP_SUB_H, OP_SUB_E, OP_MIXED };
+enum OpKind {
+  OP_GATHER, OP_MODAL, OP_TABLE, OP_VIEW, OP_LINE_H, OP_LINE_E, OP_SUB_H, OP_SUB_E, OP_MIXED,
+  OP_TAPE, OP_ADJ_GRAD, OP_ADJ_SRC
+};
 
 struct Op {
   int kind = 0, phase = 0, field = 0;
@@ -62,6 +65,13 @@ struct Op {
   int S = 0, maxp = 0, has_h = 0, has_a = 0;
   const float *f = nullptr, *wmix = nullptr, *h0 = nullptr, *inv_a = nullptr, *sm = nullptr;
   void* state = nullptr;
+  // adjoint: tape rows, forward step count, DFT stride, gradient accumulator, second H target, H side
+  long long n_slots = 0, n_steps = 0;
+  int dft_stride = 1, h_side = 0;
+  double* gacc = nullptr;
+  const long long* t2 = nullptr;
+  const double* omega = nullptr;
+  double dt = 0.0;
 };
 
 struct Monitor {
@@ -119,6 +129,7 @@ struct lsf_engine {
   bool dirty = true;                   // dev_args out of date
   bool launched = false;               // kernels may be reading dev_args
   int held = 0;                        // batches holding this engine (its configuration is then frozen)
+  int adjoint = 0;                     // steps the adjoint (transposed, time-reversed) update
 };
 
 namespace {
@@ -140,7 +151,10 @@ StepArgs make_args(const lsf_engine* e, int field, const Tiling& t) {
   a.cbd = e->cbd;
   a.mid = e->mid;
   a.mtab = e->mtab;
-  for (int ax = 0; ax < 3; ++ax) a.pml[ax] = e->pml[field][ax];
+  for (int ax = 0; ax < 3; ++ax) {
+    a.pml[ax] = e->pml[field][ax];
+    a.xpml[ax] = e->pml[1 - field][ax];
+  }
   a.chunk_base = e->chunk_base;
   a.P = e->P;
   a.n_ade = (int)e->n_ade;
@@ -208,12 +222,14 @@ lsf::KLaunch own_launch(lsf_engine* e, int field) {
 }
 
 void launch_h(lsf_engine* e, hipStream_t s) {
-  lsf::launch_update_h(own_launch(e, 1), s);
+  if (e->adjoint) lsf::launch_adjoint_h(own_launch(e, 1), s);
+  else lsf::launch_update_h(own_launch(e, 1), s);
   e->launched = true;
 }
 
 void launch_e(lsf_engine* e, hipStream_t s) {
-  lsf::launch_update_e(own_launch(e, 0), s);
+  if (e->adjoint) lsf::launch_adjoint_e(own_launch(e, 0), s);
+  else lsf::launch_update_e(own_launch(e, 0), s);
   e->launched = true;
 }
 
@@ -274,6 +290,24 @@ void run_op(lsf_engine* e, const Op& o, hipStream_t s) {
       launch_e(o.sub, s);
       for (int ph = LSF_PH_OFFDIAG; ph <= LSF_PH_ADVANCE_E; ++ph) run_phase(o.sub, ph, s);
       break;
+    case OP_TAPE: {
+      lsf::TapeJob J{o.n, o.target, (const float*)e->H, e->g.n[0], e->g.n[1], e->g.N,
+                     {e->g.s[0], e->g.s[1], e->g.s[2]}, o.out, o.n_slots, o.base, (const long long*)e->counter};
+      if (o.n > 0) hipLaunchKernelGGL(lsf::k_tape, dim3(nblk(o.n, 256)), dim3(256), 0, s, J);
+      break;
+    }
+    case OP_ADJ_GRAD: {
+      lsf::GradJob J{o.n, o.target, (const float*)e->E, o.coef, o.table, o.n_slots, o.base,
+                     (const long long*)e->counter, o.n_steps, o.gacc};
+      if (o.n > 0) hipLaunchKernelGGL(lsf::k_adj_grad, dim3(nblk(o.n, 256)), dim3(256), 0, s, J);
+      break;
+    }
+    case OP_ADJ_SRC: {
+      lsf::AdjSrcJob J{o.h_side, o.width, o.n, o.omega, o.dt, (const c32*)o.w, o.target, o.t2, o.scale,
+                       (float*)(o.h_side ? e->H : e->E), (const long long*)e->counter, o.n_steps, o.dft_stride};
+      if (o.n > 0) hipLaunchKernelGGL(lsf::k_adj_src, dim3(nblk(o.n, 256)), dim3(256), 0, s, J);
+      break;
+    }
     case OP_MIXED:
       if (o.n > 0)
         hipLaunchKernelGGL(lsf::k_mixed<T>, dim3(nblk(o.n, 256)), dim3(256), 0, s, o.n, o.S, o.maxp, o.target, o.f,
@@ -632,6 +666,82 @@ int lsf_add_mixed_group(lsf_engine* e, int64_t n, int S, int max_poles, const in
   return add_op(e, o);
 }
 
+int lsf_set_adjoint(lsf_engine* e, int on) {
+  if (int rc0 = configurable(e)) return rc0;
+  if (on) {
+    if (e->cplx || any_periodic(e)) return fail(LSF_EINVAL, "the adjoint update takes real fields without periodic axes");
+    if (e->n_ade > 0) return fail(LSF_EINVAL, "the adjoint update has no D-form (ADE) nodes");
+    if (e->xw != 64) return fail(LSF_EINVAL, "the adjoint kernels take 64 x cells per wavefront row");
+    for (const Op& o : e->ops)
+      if (o.kind == OP_MIXED || o.kind == OP_SUB_H || o.kind == OP_SUB_E || o.kind == OP_LINE_H)
+        return fail(LSF_EINVAL, "the adjoint engine takes only adjoint sources and gradient operations");
+  }
+  e->adjoint = on ? 1 : 0;
+  return LSF_OK;
+}
+
+int lsf_add_tape(lsf_engine* e, int64_t n, const int64_t* nodes, float* tape, int64_t n_slots, const int64_t* base) {
+  if (valid(e)) return LSF_EINVAL;
+  if (n < 1 || !nodes || !tape || n_slots < 1 || !base) return fail(LSF_EINVAL, "bad tape");
+  if (e->cplx || e->adjoint) return fail(LSF_EINVAL, "the tape records a real forward engine");
+  Op o;
+  o.kind = OP_TAPE;
+  o.phase = LSF_PH_ADVANCE_H;
+  o.field = 1;
+  o.n = n;
+  o.target = (const long long*)nodes;
+  o.out = tape;
+  o.n_slots = n_slots;
+  o.base = (const long long*)base;
+  return add_op(e, o);
+}
+
+int lsf_add_adjoint_gradient(lsf_engine* e, int64_t n, const int64_t* nodes, const float* cb, const float* tape,
+                             int64_t n_slots, const int64_t* base, int64_t n_steps, double* g) {
+  if (valid(e)) return LSF_EINVAL;
+  if (!e->adjoint) return fail(LSF_ESTATE, "gradient operations belong to an adjoint engine (lsf_set_adjoint)");
+  if (n < 1 || !nodes || !cb || !tape || n_slots < 1 || !base || n_steps < 1 || !g)
+    return fail(LSF_EINVAL, "bad adjoint gradient");
+  Op o;
+  o.kind = OP_ADJ_GRAD;
+  o.phase = LSF_PH_ADVANCE_H;
+  o.field = 0;
+  o.n = n;
+  o.target = (const long long*)nodes;
+  o.coef = cb;
+  o.table = tape;
+  o.n_slots = n_slots;
+  o.base = (const long long*)base;
+  o.n_steps = n_steps;
+  o.gacc = g;
+  return add_op(e, o);
+}
+
+int lsf_add_adjoint_source(lsf_engine* e, int h_side, int nf, int64_t np, const double* omega, double dt,
+                           const void* W, const int64_t* t1, const int64_t* t2, const float* scale, int64_t n_steps,
+                           int stride) {
+  if (valid(e)) return LSF_EINVAL;
+  if (!e->adjoint) return fail(LSF_ESTATE, "adjoint sources belong to an adjoint engine (lsf_set_adjoint)");
+  if (nf < 1 || np < 1 || !omega || !W || !t1 || !scale || n_steps < 1 || stride < 1 || (h_side && !t2))
+    return fail(LSF_EINVAL, "bad adjoint source");
+  Op o;
+  o.kind = OP_ADJ_SRC;
+  o.phase = h_side ? LSF_PH_CORRECT_H : LSF_PH_CORRECT_E;
+  o.field = h_side ? 1 : 0;
+  o.h_side = h_side ? 1 : 0;
+  o.width = nf;
+  o.n = np;
+  o.omega = omega;
+  o.dt = dt;
+  o.w = W;
+  o.target = (const long long*)t1;
+  o.t2 = (const long long*)t2;
+  o.scale = scale;
+  o.n_steps = n_steps;
+  o.dft_stride = stride;
+  return add_op(e, o);
+}
+
 int lsf_set_energy(lsf_engine* e, float* energy) {
   if (int rc0 = configurable(e)) return rc0;
   e->energy = energy;
@@ -702,6 +812,7 @@ int lsf_set_tiling(lsf_engine* e, int ry, int waves, int xw, int zc) {
   if (!lsf::tile_shape_ok(ry, waves, xw))
     return fail(LSF_EINVAL, "unsupported tile shape: rows per lane 1 or 2, waves 4 or 8, x cells per row 64 or 32");
   if (zc < 1) return fail(LSF_EINVAL, "z chunk must be >= 1");
+  if (e->adjoint && xw != 64) return fail(LSF_EINVAL, "the adjoint kernels take 64 x cells per wavefront row");
   if (e->exec) return fail(LSF_ESTATE, "the engine has a captured graph; reset it before retiling");
   e->ry = ry;
   e->waves = waves;
@@ -1165,6 +1276,7 @@ int lsf_batch_create(lsf_engine* const* members, int n, lsf_batch** out) {
     if (!e) return fail(LSF_EINVAL, "null engine in batch");
     if (!e->tick) return fail(LSF_EINVAL, "batch members are top-level engines (tick = 1)");
     if (!e->cbd && !e->mid) return fail(LSF_ESTATE, "a batch member has no material");
+    if (e->adjoint) return fail(LSF_EINVAL, "adjoint engines are stepped alone");
     for (int r = 0; r < q; ++r)
       if (members[r] == e || members[r]->counter == e->counter)
         return fail(LSF_EINVAL, "batch members must be distinct engines with their own step counters");

@@ -39,6 +39,10 @@ void launch_update_h_c(const KLaunch& L, hipStream_t s);
 LSF_E_LAUNCHERS(LSF_DECLARE_E)
 #undef LSF_DECLARE_E
 
+// adjoint kernels (real fields, no periodic axes, no D-form nodes; one tile per block): lsf_k_adj.hip
+void launch_adjoint_h(const KLaunch& L, hipStream_t s);
+void launch_adjoint_e(const KLaunch& L, hipStream_t s);
+
 inline void launch_update_h(const KLaunch& L, hipStream_t s) {
   if (L.cplx) launch_update_h_c(L, s);
   else launch_update_h_f(L, s);

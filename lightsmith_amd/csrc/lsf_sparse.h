@@ -343,4 +343,103 @@ __global__ void k_bump(long long* pos) {
   if (threadIdx.x == 0 && blockIdx.x == 0) pos[0] += 1;
 }
 
+// ---------------------------------------------------------------------------------------------
+// adjoint operations (the bulk adjoint kernels: lsf_adjoint.h)
+// ---------------------------------------------------------------------------------------------
+
+// Forward tape: X = curl- H at design E nodes (the factor multiplying cb in the E update), one row per step.
+// slot = counter[0] - base[0] (the forward step n of the update that follows); rows outside [0, n_slots) are
+// not recorded.
+struct TapeJob {
+  long long n;           // design nodes
+  const long long* node; // native flat E indices
+  const float* H;
+  int n0, n1;
+  long long N;
+  float s[3];
+  float* tape;           // [n_slots][n]
+  long long n_slots;
+  const long long* base;
+  const long long* counter;
+};
+
+__global__ void k_tape(TapeJob J) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= J.n) return;
+  const long long slot = J.counter[0] - J.base[0];
+  if (slot < 0 || slot >= J.n_slots) return;
+  const long long nd = J.node[q];
+  const int c = (int)(nd / J.N);
+  const long long cell = nd - (long long)c * J.N;
+  const long long st[3] = {1, J.n0, (long long)J.n0 * J.n1};
+  const int d1 = (c + 1) % 3, d2 = (c + 2) % 3;
+  const float* Hd2 = J.H + (long long)d2 * J.N;
+  const float* Hd1 = J.H + (long long)d1 * J.N;
+  const float da = Hd2[cell] - Hd2[cell - st[d1]];
+  const float db = Hd1[cell] - Hd1[cell - st[d2]];
+  J.tape[slot * J.n + q] = J.s[d1] * da - J.s[d2] * db;
+}
+
+// Gradient accumulation of an adjoint step: g[q] += (e~ / cb)[node q] * tape[n][q], n = n_steps - 1 -
+// counter[0] the forward step being reversed (tape row n - base).
+struct GradJob {
+  long long n;
+  const long long* node;
+  const float* E;       // e~
+  const float* cb;      // [n] cb of each design node
+  const float* tape;
+  long long n_slots;
+  const long long* base;
+  const long long* counter;
+  long long n_steps;
+  double* g;
+};
+
+__global__ void k_adj_grad(GradJob J) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= J.n) return;
+  const long long n = J.n_steps - 1 - J.counter[0];
+  const long long slot = n - J.base[0];
+  if (n < 0 || slot < 0 || slot >= J.n_slots) return;
+  J.g[q] += (double)(J.E[J.node[q]] / J.cb[q]) * (double)J.tape[slot * J.n + q];
+}
+
+// Adjoint DFT source: F[target p] += scale_p * sum_f Re(W[f, p] exp(i omega_f t)) at sampling steps, where
+// W = conj(dL/dacc) in torch's gradient convention for the monitor accumulator acc[f, p]. E side (h_side = 0):
+// after an adjoint E half-step the state is E^n, n = n_steps - 1 - counter[1]; sampled when n > 0 and n % stride
+// == 0, t = n dt. H side (h_side = 1): after an adjoint H half-step the state is H^{n-1/2} with
+// n = n_steps - counter[0], sampled at n with t = n dt - dt/2; each H point is the average of two nodes.
+struct AdjSrcJob {
+  int h_side;
+  int nf;
+  long long np;
+  const double* omega;
+  double dt;
+  const c32* W;          // [nf][np]
+  const long long* t1;   // targets (E node, or first H node)
+  const long long* t2;   // second H node (H side)
+  const float* scale;    // [np] cb at the E node, or -cH / 2 for H
+  float* F;
+  const long long* counter;
+  long long n_steps;
+  int stride;
+};
+
+__global__ void k_adj_src(AdjSrcJob J) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= J.np) return;
+  const long long n = J.h_side ? J.n_steps - J.counter[0] : J.n_steps - 1 - J.counter[1];
+  if (n <= 0 || n % J.stride != 0) return;
+  const double t = (double)n * J.dt - (J.h_side ? 0.5 * J.dt : 0.0);
+  double v = 0.0;
+  for (int f = 0; f < J.nf; ++f) {
+    const double ph = J.omega[f] * t;
+    const c32 w = J.W[(long long)f * J.np + p];
+    v += (double)w.x * cos(ph) - (double)w.y * sin(ph);
+  }
+  const float add = (float)v * J.scale[p];
+  atomicAdd(J.F + J.t1[p], add);  // points may share a node (edges of a flux box)
+  if (J.h_side) atomicAdd(J.F + J.t2[p], add);
+}
+
 }  // namespace lsf

@@ -33,6 +33,7 @@ struct StepArgs {
   const uint32_t* mid;   // indexed material [N], or null
   const float* mtab;     // indexed material table [256]
   PmlAx pml[3];          // this kernel's field
+  PmlAx xpml[3];         // the other field's CPML (read by the adjoint kernels)
   const int* chunk_base; // first D-form slot of each 32-cell chunk [3][N2][N1][nchunk]
   void* P;
   int n_ade;

@@ -1,0 +1,220 @@
+"""Gradients of DFT-monitor objectives with respect to a design region's permittivity, by the adjoint method.
+
+The reference FDTD has no adjoint (docs/solvers/fdtd.md:314; its inverse design runs on FDFD,
+inverse/__init__.py:14-40); this module adds one behind ``torch.autograd``:
+
+    prob = AdjointProblem(device, shape=..., design=DesignRegion(...), monitor=DFTMonitor(...), ...)
+    acc_e, acc_h = monitor_spectra(rho, prob)          # forward FDTD run, differentiable in rho
+    loss = objective(acc_e, acc_h); loss.backward()    # rho.grad by one adjoint run
+
+Density rho in [0, 1] per design voxel sets eps = eps_lo + rho (eps_hi - eps_lo) at the E_x, E_y and E_z nodes of
+that cell index (staircase). The forward engine records, per step, X = curl- H at the design nodes (a tape); the
+adjoint engine (lsf_set_adjoint) runs the transposed update backwards from the monitor's adjoint sources and
+accumulates dL/dcb = sum_n Ebar^{n+1} X^{n+1/2} per node; dL/drho follows from cb = (dt/dx) / eps(rho).
+
+Memory: the tape holds 4 B per design node and step. When all steps fit in `tape_budget` bytes the adjoint needs
+no recomputation. Otherwise the forward run keeps a checkpoint of its state every `checkpoint_every` steps, and
+the adjoint pass recomputes each interval from its checkpoint (recording that interval's tape) just before
+reversing it: memory stays at the checkpoints plus one interval of tape.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .._lib import check, ptr
+from .engine import DFTMonitor, Engine, PMLSpec
+
+
+@dataclass
+class DesignRegion:
+    """Cells [lo, hi) (index boxes per axis); eps = eps_lo + rho (eps_hi - eps_lo) at their E nodes."""
+
+    lo: tuple
+    hi: tuple
+    eps_lo: float
+    eps_hi: float
+
+    @property
+    def shape(self) -> tuple:
+        return tuple(int(h - l) for l, h in zip(self.lo, self.hi))
+
+    @property
+    def n_voxels(self) -> int:
+        return int(np.prod(self.shape))
+
+    def nodes(self, shape) -> tuple[np.ndarray, np.ndarray]:
+        """(native flat E indices, voxel index) of the 3 * n_voxels design nodes, component-major; voxels in
+        C order over the box (i slowest, k fastest, as rho.reshape(shape))."""
+        n0, n1, n2 = shape
+        N = n0 * n1 * n2
+        ii, jj, kk = np.meshgrid(*[np.arange(l, h) for l, h in zip(self.lo, self.hi)], indexing="ij")
+        cell = ((kk * n1 + jj) * n0 + ii).reshape(-1)
+        nodes = np.concatenate([c * N + cell for c in range(3)]).astype(np.int64)
+        vox = np.tile(np.arange(self.n_voxels), 3)
+        return nodes, vox
+
+
+class AdjointProblem:
+    """A real-field FDTD run (CPML, sources, one DFT monitor) whose design-region permittivity follows rho."""
+
+    def __init__(self, device, *, shape, dx, dt: float, inv_eps: np.ndarray, pml: PMLSpec, sources, monitor: DFTMonitor,
+                 design: DesignRegion, n_steps: int, dft_stride: int, checkpoint_every: int = 100,
+                 tape_budget: int = 4 << 30, use_graphs: bool = True):
+        self.design = design
+        self.shape = tuple(int(v) for v in shape)
+        self.stride = int(dft_stride)
+        self.n_steps = int(np.ceil(n_steps / self.stride)) * self.stride
+        K = max(self.stride, int(np.ceil(checkpoint_every / self.stride)) * self.stride)
+        self.K = K
+        self.use_graphs = use_graphs
+        for a in range(3):
+            cells = pml.cells[a]
+            if design.lo[a] < max(1, cells[0]) or design.hi[a] > self.shape[a] - max(1, cells[1]) - 1:
+                raise ValueError("the design region must lie inside the CPML-free interior")
+        self.fwd = Engine(device, shape=shape, dx=dx, dt=dt, single=True, inv_eps=inv_eps, pml=pml, sources=sources,
+                          monitors=[monitor], dft_stride=self.stride, material="dense")
+        self.monitor = monitor
+        dev = self.fwd._dev
+        self._dev = dev
+        nodes, vox = design.nodes(self.shape)
+        self.n_nodes = len(nodes)
+        self._nodes = torch.as_tensor(nodes, device=dev)
+        self._vox = torch.as_tensor(vox, device=dev)
+        self.cb_design = self.fwd._mat[self._nodes].clone()
+        per_step = 4 * self.n_nodes
+        self.store_all = per_step * self.n_steps <= tape_budget
+        self.n_slots = self.n_steps if self.store_all else K
+        self.tape = torch.zeros(self.n_slots * self.n_nodes, dtype=torch.float32, device=dev)
+        self.base = torch.zeros(1, dtype=torch.int64, device=dev)
+        lib = self.fwd.lib
+        check(lib.lsf_add_tape(self.fwd._h, self.n_nodes, ptr(self._nodes), ptr(self.tape), self.n_slots,
+                               ptr(self.base)), "lsf_add_tape")
+
+        # the adjoint engine: same grid, material (shared) and CPML profiles; its own fields and chi arrays
+        self.adj = Engine(device, shape=shape, dx=dx, dt=dt, single=True, inv_eps=inv_eps, pml=pml, sources=[],
+                          monitors=[], dft_stride=1, material="dense")
+        check(lib.lsf_set_material_dense(self.adj._h, ptr(self.fwd._mat)), "lsf_set_material_dense")
+        check(lib.lsf_set_adjoint(self.adj._h, 1), "lsf_set_adjoint")
+        self.g = torch.zeros(self.n_nodes, dtype=torch.float64, device=dev)
+        check(lib.lsf_add_adjoint_gradient(self.adj._h, self.n_nodes, ptr(self._nodes), ptr(self.cb_design),
+                                           ptr(self.tape), self.n_slots, ptr(self.base), self.n_steps, ptr(self.g)),
+              "lsf_add_adjoint_gradient")
+        m = monitor
+        nf = len(np.asarray(m.omega))
+        self._e_nodes = m._e
+        self._cb_e = self.fwd._mat[self._e_nodes].clone()
+        self.W_E = torch.zeros((nf, len(m._e)), dtype=torch.complex64, device=dev)
+        check(lib.lsf_add_adjoint_source(self.adj._h, 0, nf, len(m._e), ptr(m._omega), float(dt), ptr(self.W_E),
+                                         ptr(m._e), None, ptr(self._cb_e), self.n_steps, self.stride),
+              "lsf_add_adjoint_source")
+        self.W_H = None
+        if m.acc_h is not None:
+            self.W_H = torch.zeros((nf, len(m._h1)), dtype=torch.complex64, device=dev)
+            self._h_scale = torch.full((len(m._h1),), -0.5 * self.fwd.cH, dtype=torch.float32, device=dev)
+            check(lib.lsf_add_adjoint_source(self.adj._h, 1, nf, len(m._h1), ptr(m._omega), float(dt), ptr(self.W_H),
+                                             ptr(m._h1), ptr(m._h2), ptr(self._h_scale), self.n_steps, self.stride),
+                  "lsf_add_adjoint_source")
+        self._omega = torch.as_tensor(np.asarray(m.omega, dtype=float), device=dev)
+        self.dt = float(dt)
+        self._run = None
+        self.checkpoints = []
+        self.timings = {}
+
+    # --- density -> material --------------------------------------------------------------------------
+    def eps(self, rho: torch.Tensor) -> torch.Tensor:
+        d = self.design
+        return d.eps_lo + rho.to(torch.float64).reshape(-1) * (d.eps_hi - d.eps_lo)
+
+    def set_density(self, rho: torch.Tensor):
+        if rho.numel() != self.design.n_voxels:
+            raise ValueError(f"rho has {rho.numel()} values, the design region {self.design.n_voxels}")
+        eps = self.eps(rho.to(self._dev))
+        cb = (self.fwd.cdx / eps)[self._vox].to(torch.float32)
+        self.fwd._mat[self._nodes] = cb
+        self.cb_design.copy_(cb)
+        self._cb_e.copy_(self.fwd._mat[self._e_nodes])
+
+    # --- runs -----------------------------------------------------------------------------------------
+    def _zero(self, eng: Engine):
+        for t in eng.state_tensors():
+            t.zero_()
+
+    def forward(self, rho: torch.Tensor):
+        """Forward run for density rho: the monitor's (acc_e, acc_h) after n_steps (copies)."""
+        self.set_density(rho)
+        self._zero(self.fwd)
+        fwd = self.fwd
+        self.base.fill_(0 if self.store_all else -(1 << 62))
+        self.checkpoints = []
+        if self._run is None:
+            self._run, _ = fwd.make_runner(self.use_graphs)
+        for step in range(0, self.n_steps, self.stride):
+            if not self.store_all and step % self.K == 0:
+                self.checkpoints.append([t.clone() for t in fwd.state_tensors()])
+            self._run()
+        m = self.monitor
+        acc_h = m.acc_h.clone() if m.acc_h is not None else torch.zeros(0, dtype=torch.complex64, device=self._dev)
+        return m.acc_e.clone(), acc_h
+
+    def backward(self, grad_e: torch.Tensor | None, grad_h: torch.Tensor | None) -> torch.Tensor:
+        """dL/drho (float64, the shape of rho's flattening) for the objective whose gradients with respect to the
+        monitor's accumulators are grad_e, grad_h (torch's convention for complex tensors)."""
+        adj, fwd = self.adj, self.fwd
+        self._zero(adj)
+        self.g.zero_()
+        self.W_E.copy_(torch.conj(grad_e) if grad_e is not None else torch.zeros_like(self.W_E))
+        if self.W_H is not None:
+            self.W_H.copy_(torch.conj(grad_h) if grad_h is not None and grad_h.numel() else torch.zeros_like(self.W_H))
+        # the sample of E^{n_steps}: the adjoint's initial condition
+        t = self.n_steps * self.dt
+        ph = torch.exp(1j * self._omega * t).to(torch.complex128)
+        src = torch.real(self.W_E.to(torch.complex128) * ph[:, None]).sum(0)
+        adj.E_flat.index_add_(0, self._e_nodes, (src * self._cb_e.to(torch.float64)).to(torch.float32))
+        if self.store_all:
+            self.base.fill_(0)
+            adj.steps(self.n_steps)
+        else:
+            for seg in reversed(range(len(self.checkpoints))):
+                n_seg = min(self.K, self.n_steps - seg * self.K)
+                for tt, snap in zip(fwd.state_tensors(), self.checkpoints[seg]):
+                    tt.copy_(snap)
+                self.base.fill_(seg * self.K)
+                fwd.steps(n_seg)
+                adj.steps(n_seg)
+        d = self.design
+        cb = self.cb_design.to(torch.float64)
+        # cb = C / eps  ->  dcb/deps = -cb^2 / C ; deps/drho = eps_hi - eps_lo
+        dcb_drho = -(cb * cb) / self.fwd.cdx * (d.eps_hi - d.eps_lo)
+        grad = torch.zeros(d.n_voxels, dtype=torch.float64, device=self._dev)
+        grad.index_add_(0, self._vox, self.g * dcb_drho)
+        return grad
+
+    def close(self):
+        if self._run is not None:
+            self.fwd.release()
+        self.fwd.close()
+        self.adj.close()
+
+
+class _MonitorSpectra(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rho, prob):
+        ctx.prob = prob
+        ctx.shape = rho.shape
+        ctx.dtype = rho.dtype
+        return prob.forward(rho.detach())
+
+    @staticmethod
+    def backward(ctx, grad_e, grad_h):
+        g = ctx.prob.backward(grad_e, grad_h)
+        return g.reshape(ctx.shape).to(ctx.dtype), None
+
+
+def monitor_spectra(rho: torch.Tensor, prob: AdjointProblem):
+    """(acc_e, acc_h) of the problem's DFT monitor after a forward run with density rho; differentiable in rho
+    through the adjoint run."""
+    return _MonitorSpectra.apply(rho, prob)

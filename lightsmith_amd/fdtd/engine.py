@@ -267,6 +267,7 @@ class Engine:
         n0, n1, n2 = self.shape
         self.N = n0 * n1 * n2
         self._keep = []  # device tables whose lifetime is the engine's
+        self._state = []  # device arrays the stepping writes (psi, pole and group state, source lines)
         self._subs = []  # sub-engines stepped inside this one
         self._held = 0  # batches stepping this engine
         self.E_flat = torch.zeros(3 * self.N, dtype=self.fdt, device=self._dev)
@@ -351,9 +352,11 @@ class Engine:
         return t
 
     def zeros(self, n, complex_=None):
+        """A zeroed device state array kept for the engine's lifetime."""
         t = torch.zeros(int(n), dtype=self.fdt if complex_ is None else (self.cdt if complex_ else self.rdt),
                         device=self._dev)
         self._keep.append(t)
+        self._state.append(t)
         return t
 
     def field_values(self, vals) -> torch.Tensor:
@@ -424,6 +427,7 @@ class Engine:
         chunk_base = ade_chunk_base(self.ade_nodes, self.shape)
         self._ade_P = torch.zeros(max_p * n * 2, dtype=self.fdt, device=self._dev)
         self._keep.append(self._ade_P)
+        self._state.append(self._ade_P)
         self._ade_tab = self.to_device(tab.astype(np.float32))
         self._ade_chunk = self.to_device(chunk_base)
         self._ade_inv_eps_inf = np.array([1.0 / m.eps_inf for m in self.ade_media])
@@ -761,7 +765,7 @@ class Engine:
         state, source lines, monitor accumulators, sub-engines'), each once: a snapshot of these restores the
         run exactly."""
         out, seen = [], set()
-        ts = [self.E_flat, self.H_flat, self._counter, self.energy, *self._keep]
+        ts = [self.E_flat, self.H_flat, self._counter, self.energy, *self._state]
         for m in self.monitors:
             ts += [getattr(m, "acc_e", None), getattr(m, "acc_h", None)]
         for m in self.time_monitors:

@@ -231,6 +231,7 @@ class TFSFPlaneWave(_Native):
         ca, cb, da, db = f32(self.ca), f32(self.cb), f32(self.da), f32(self.db)
         wave = f32(self.wave)
         eng._keep += [self.E_flat, self.H_flat]
+        eng._state += [self.E_flat, self.H_flat]
         for kind, phase, src in (("H", PH_CORRECT_H, self.E_flat), ("E", PH_CORRECT_E, self.H_flat)):
             tg, si, ww = [], [], []
             for k, c, region, coef, i0, w, shape_ in self.slabs:

@@ -218,3 +218,74 @@ def oblique_engine(run: dict, device="cuda:0", max_steps: int = 4000, stride: in
     return Engine(device, shape=shape, dx=d, dt=run["dt"], single=True, inv_eps=run["inv_eps"], groups=[], pml=pml,
                   sources=[src], monitors=[mon_r, mon_t, mon_i], dft_stride=stride, periodic=(True, True, False),
                   bloch=bloch)
+
+
+# --- adjoint inverse design (an SOI-like waveguide with a design region) ---------------------------------
+
+def adjoint_scene(shape=(256, 192, 96), dx: float = 20.0, design=(80, 80, 11), seed: int = 4, pml: int = 16,
+                  n_steps: int = 3000, stride: int = 25, wavelength: float = 1550.0, courant: float = 0.99) -> dict:
+    """A silicon strip waveguide (eps 3.48^2, 500 x 220 nm: 25 x 11 cells) along x in oxide cladding
+    (eps 1.44^2), broken in the middle by a design region of `design` cells (the waveguide's thickness) whose
+    density rho ~ U[0, 1] (seed) sets eps = 1.44^2 + rho (3.48^2 - 1.44^2); 16-cell CPML on every face; a soft
+    mode-injection source of E_y on an x plane near the input (a Gaussian approximation of the fundamental TE
+    profile over the core, Gaussian pulse at `wavelength` with half-width 0.1 omega); a DFT monitor of E_y, E_z
+    and face-averaged H on an x plane near the output at `wavelength`. The objective is the power in the same
+    profile at the output, |sum_p w_p E_y(p)|^2. DFT stride 25 (finer than the reference rule's 90 at this dt;
+    it divides the 3000 steps and the 100-step checkpoint interval)."""
+    n0, n1, n2 = shape
+    e_clad, e_core = 1.44 ** 2, 3.48 ** 2
+    eps = np.full(shape, e_clad)
+    jc, kc = n1 // 2, n2 // 2
+    wy, wz = 25, 11
+    j_lo, k_lo = jc - wy // 2, kc - wz // 2
+    eps[:, j_lo:j_lo + wy, k_lo:k_lo + wz] = e_core
+    di, dj, dk = design
+    lo = (n0 // 2 - di // 2, jc - dj // 2, k_lo)
+    hi = (lo[0] + di, lo[1] + dj, lo[2] + dk)
+    rho = np.random.default_rng(seed).uniform(0.0, 1.0, size=design)
+    inv_eps = np.broadcast_to(1.0 / eps, (3,) + shape).copy()
+    dt = courant * dx / math.sqrt(3)
+    omega = 2 * np.pi / wavelength
+    # E_y window around the core: +-1 core width in y, +-2 thicknesses in z
+    jj, kk = np.meshgrid(np.arange(j_lo - wy, j_lo + 2 * wy), np.arange(k_lo - 2 * wz, k_lo + 3 * wz), indexing="ij")
+    jj, kk = jj.ravel(), kk.ravel()
+    prof = np.exp(-((jj - (j_lo + wy / 2 - 0.5)) / (0.45 * wy)) ** 2 - ((kk - (k_lo + wz / 2 - 0.5)) / (0.6 * wz)) ** 2)
+    i_src, i_mon = pml + 14, n0 - pml - 14
+    e_src = flat_index(1, np.full(jj.size, i_src), jj, kk, shape)
+    tau = 2.0 / (0.1 * omega)
+    t = (np.arange(n_steps + 2) + 0.5) * dt
+    pulse = np.exp(-((t - 3.5 * tau) / tau) ** 2) * np.sin(omega * (t - 3.5 * tau))
+    im = np.full(jj.size, i_mon)
+    e_mon = np.concatenate([flat_index(1, im, jj, kk, shape), flat_index(2, im, jj, kk, shape)])
+    h1 = np.concatenate([flat_index(2, im - 1, jj, kk, shape), flat_index(1, im - 1, jj, kk, shape)])
+    h2 = np.concatenate([flat_index(2, im, jj, kk, shape), flat_index(1, im, jj, kk, shape)])
+    w = np.concatenate([prof, np.zeros_like(prof)]) / np.linalg.norm(prof)
+    return {"shape": shape, "dx": dx, "dt": dt, "inv_eps": inv_eps, "pml": pml, "lo": lo, "hi": hi, "eps_lo": e_clad,
+            "eps_hi": e_core, "rho": rho, "n_steps": n_steps, "stride": stride, "omega": np.array([omega]),
+            "e_src": e_src, "m_src": prof[:, None], "t_src": pulse[:, None], "e_mon": e_mon, "h1": h1, "h2": h2,
+            "w": w, "n_cells": int(np.prod(shape))}
+
+
+def adjoint_problem(scene: dict, device="cuda:0", **kw):
+    """The AdjointProblem of an adjoint scene (CPML alpha from the source frequency)."""
+    from .adjoint import AdjointProblem, DesignRegion
+    from .engine import DFTMonitor, PMLSpec
+    from .sources import ModeInjection
+
+    p = scene["pml"]
+    src = ModeInjection(np.zeros(0, np.int64), np.zeros((0, 1)), np.zeros((scene["n_steps"] + 2, 1)), scene["e_src"],
+                        scene["m_src"], scene["t_src"])
+    mon = DFTMonitor(scene["omega"], scene["e_mon"], scene["h1"], scene["h2"])
+    pml = PMLSpec(cells=((p, p),) * 3, alpha_max=0.05 * float(scene["omega"][0]), eps_ref=scene["eps_lo"])
+    return AdjointProblem(device, shape=scene["shape"], dx=scene["dx"], dt=scene["dt"], inv_eps=scene["inv_eps"],
+                          pml=pml, sources=[src], monitor=mon,
+                          design=DesignRegion(scene["lo"], scene["hi"], scene["eps_lo"], scene["eps_hi"]),
+                          n_steps=scene["n_steps"], dft_stride=scene["stride"], **kw)
+
+
+def mode_power(acc_e, w):
+    """|sum_p w_p acc_e[0, p]|^2 (the adjoint scene's objective; numpy or torch)."""
+    if isinstance(acc_e, np.ndarray):
+        return abs(np.sum(w * acc_e[0])) ** 2
+    import torch
+    return (torch.as_tensor(w, device=acc_e.device).to(acc_e.dtype) * acc_e[0]).sum().abs() ** 2

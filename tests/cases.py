@@ -193,3 +193,96 @@ def synthetic_table_source(ns, shape, seed=0, m=3, n_rows=40, n_pts=None):
         points[kind] = tuple(np.concatenate(v) for v in (nodes, base, i0, u0, coef))
     inc = (np.arange(3) * L - 2, np.full(3, 6 * m + 1), np.full(3, 0.25))
     return ns.TableSource(T.reshape(-1), m, cmax, points, inc=inc, T_inc=T[:3].reshape(-1))
+
+
+# --- adjoint ------------------------------------------------------------------------------------------
+
+def adjoint_case(shape=(34, 28, 22), pml=6, steps=240, stride=8, seed=4, design=((13, 9, 8), (21, 19, 14)),
+                 wavelength=400.0):
+    """A small SOI-like adjoint problem: clad eps 1.44^2 with a core bar along x (eps 3.48^2), CPML with kappa
+    and alpha, a mode-injection soft source on an x plane (E_y, Gaussian profile, Gaussian pulse), a design box
+    of the given cells, and a two-frequency DFT monitor (E_y, E_z and face-averaged H_z, H_y) on an x plane
+    downstream. Returns a dict of host inputs shared by the oracle and the HIP problem."""
+    rng = np.random.default_rng(seed)
+    n0, n1, n2 = shape
+    dx = 20.0
+    dt = 0.95 * dx / np.sqrt(3)
+    e_clad, e_core = 1.44 ** 2, 3.48 ** 2
+    eps = np.full(shape, e_clad)
+    jc, kc = n1 // 2, n2 // 2
+    eps[:, jc - 3:jc + 3, kc - 2:kc + 2] = e_core
+    inv_eps = np.broadcast_to(1.0 / eps, (3,) + shape).copy()
+    lo, hi = design
+    # source: E_y on plane i = pml + 2 over a window around the core
+    i_src = pml + 2
+    jj, kk = np.meshgrid(np.arange(jc - 6, jc + 6), np.arange(kc - 5, kc + 5), indexing="ij")
+    from oracle.fdtd import flat_index
+    e_src = flat_index(1, np.full(jj.size, i_src), jj.ravel(), kk.ravel(), shape)
+    prof = np.exp(-((jj.ravel() - jc + 0.5) / 4.0) ** 2 - ((kk.ravel() - kc + 0.5) / 3.0) ** 2)
+    omega_c = 2 * np.pi / wavelength
+    tau = 2.0 / (0.2 * omega_c)
+    t = (np.arange(steps + 2) + 0.5) * dt
+    pulse = np.exp(-((t - 3.5 * tau) / tau) ** 2) * np.sin(omega_c * (t - 3.5 * tau))
+    # monitor on plane i = n0 - pml - 3
+    i_mon = n0 - pml - 3
+    jm, km = np.meshgrid(np.arange(jc - 7, jc + 7), np.arange(kc - 6, kc + 6), indexing="ij")
+    jm, km = jm.ravel(), km.ravel()
+    im = np.full(jm.size, i_mon)
+    e_mon = np.concatenate([flat_index(1, im, jm, km, shape), flat_index(2, im, jm, km, shape)])
+    h1 = np.concatenate([flat_index(2, im - 1, jm, km, shape), flat_index(1, im - 1, jm, km, shape)])
+    h2 = np.concatenate([flat_index(2, im, jm, km, shape), flat_index(1, im, jm, km, shape)])
+    omega = omega_c * np.array([0.95, 1.05])
+    rho = rng.uniform(0, 1, size=tuple(h - l for l, h in zip(lo, hi)))
+    w_e = rng.standard_normal(len(e_mon)) + 1j * rng.standard_normal(len(e_mon))
+    w_h = rng.standard_normal(len(h1)) + 1j * rng.standard_normal(len(h1))
+    return dict(shape=shape, dx=dx, dt=dt, inv_eps=inv_eps, pml=pml, steps=steps, stride=stride, lo=lo, hi=hi,
+                eps_lo=e_clad, eps_hi=e_core, e_src=e_src, m_src=prof[:, None], t_src=pulse[:, None], omega=omega,
+                e_mon=e_mon, h1=h1, h2=h2, rho=rho, w_e=w_e, w_h=w_h)
+
+
+def adjoint_objective(acc_e, acc_h, case):
+    """|sum w_e acc_e[0]|^2 + |sum w_h acc_h[1]|^2 (numpy or torch)."""
+    if isinstance(acc_e, np.ndarray):
+        return abs(np.sum(case["w_e"] * acc_e[0])) ** 2 + abs(np.sum(case["w_h"] * acc_h[1])) ** 2
+    import torch
+    we = torch.as_tensor(case["w_e"], device=acc_e.device).to(acc_e.dtype)
+    wh = torch.as_tensor(case["w_h"], device=acc_h.device).to(acc_h.dtype)
+    return (we * acc_e[0]).sum().abs() ** 2 + (wh * acc_h[1]).sum().abs() ** 2
+
+
+def adjoint_inv_eps(case, rho):
+    """inv_eps of the case with the design box set from rho (the staircase rule of DesignRegion)."""
+    inv = case["inv_eps"].copy()
+    box = tuple(slice(l, h) for l, h in zip(case["lo"], case["hi"]))
+    eps = case["eps_lo"] + np.asarray(rho) * (case["eps_hi"] - case["eps_lo"])
+    for c in range(3):
+        inv[c][box] = 1.0 / eps
+    return inv
+
+
+def adjoint_oracle(case, rho):
+    """The FP64 oracle Engine of the case at density rho (not stepped)."""
+    from oracle import fdtd, sources
+    p = case["pml"]
+    src = sources.ModeInjection(np.zeros(0, np.int64), np.zeros((0, 1)), np.zeros((case["steps"] + 2, 1)), case["e_src"],
+                                case["m_src"], case["t_src"])
+    mon = fdtd.DFTMonitor(case["omega"], case["e_mon"], case["h1"], case["h2"])
+    return fdtd.Engine(shape=case["shape"], dx=case["dx"], dt=case["dt"], inv_eps=adjoint_inv_eps(case, rho),
+                       pml=fdtd.PMLSpec(cells=((p, p),) * 3, kappa_max=2.0, alpha_max=0.02, eps_ref=case["eps_lo"]),
+                       sources=[src], monitors=[mon], dft_stride=case["stride"])
+
+
+def adjoint_problem(case, device, **kw):
+    """The HIP AdjointProblem of the case."""
+    from lightsmith_amd.fdtd import DFTMonitor, PMLSpec
+    from lightsmith_amd.fdtd.adjoint import AdjointProblem, DesignRegion
+    from lightsmith_amd.fdtd.sources import ModeInjection
+    p = case["pml"]
+    src = ModeInjection(np.zeros(0, np.int64), np.zeros((0, 1)), np.zeros((case["steps"] + 2, 1)), case["e_src"],
+                        case["m_src"], case["t_src"])
+    mon = DFTMonitor(case["omega"], case["e_mon"], case["h1"], case["h2"])
+    return AdjointProblem(device, shape=case["shape"], dx=case["dx"], dt=case["dt"], inv_eps=case["inv_eps"],
+                          pml=PMLSpec(cells=((p, p),) * 3, kappa_max=2.0, alpha_max=0.02, eps_ref=case["eps_lo"]),
+                          sources=[src], monitor=mon,
+                          design=DesignRegion(case["lo"], case["hi"], case["eps_lo"], case["eps_hi"]),
+                          n_steps=case["steps"], dft_stride=case["stride"], **kw)
