@@ -14,6 +14,7 @@ measured with HIP events around each launch on the launch stream over a second p
 cpu_baseline: the FP64 NumPy oracle (one thread) timed on a bounded sample of the same scene.
 batched: the oblique sweep (64 uneven Bloch-periodic runs) stepped as one autotuned Batch against the same runs
 stepped one after another on the plain engine; aggregate Gcell-updates/s of both and their ratio.
+adjoint: time of dL/drho (forward + adjoint run) over one forward run on the 256x192x96 adjoint scene.
 """
 
 from __future__ import annotations
@@ -163,6 +164,64 @@ def batched_leg(dev, steps: int, n_inst: int = 64, tune: bool = True) -> dict:
             "autotune_top5": top, "finite": finite}
 
 
+def adjoint_leg(dev, repeats: int = 2) -> dict:
+    """Cost of dL/drho by the adjoint (forward run + adjoint run, lightsmith_amd.fdtd.adjoint) against one forward
+    run, on the adjoint scene (workloads.adjoint_scene: 256x192x96 cells, 80x80x11 design region, 3000 steps):
+    with the stored tape and with checkpoint recompute every 100 steps. Wall time between device
+    synchronisations, best of `repeats`."""
+    import torch
+
+    from lightsmith_amd.fdtd import workloads
+    from lightsmith_amd.fdtd.adjoint import monitor_spectra
+
+    t0 = time.perf_counter()
+    scene = workloads.adjoint_scene()
+    prob = workloads.adjoint_problem(scene, dev)
+    rho0 = torch.tensor(scene["rho"], dtype=torch.float64, device=dev)
+    prob.forward(rho0)  # warm-up (captures the chunk graph)
+    torch.cuda.synchronize()
+    log(f"adjoint scene: {scene['shape']} cells, design {prob.design.shape}, {prob.n_steps} steps, tape "
+        f"{prob.tape.numel() * 4 / 1e9:.2f} GB, set-up {time.perf_counter() - t0:.1f} s")
+
+    def timed(fn):
+        best = float("inf")
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - a)
+        return best, out
+
+    t_fwd, _ = timed(lambda: prob.forward(rho0))
+
+    def grad():
+        rho = rho0.clone().requires_grad_(True)
+        acc_e, _ = monitor_spectra(rho, prob)
+        loss = workloads.mode_power(acc_e, scene["w"])
+        loss.backward()
+        return rho.grad, float(loss)
+
+    t_grad, (g, loss) = timed(grad)
+    prob.use_checkpoints(True)
+    t_ck, (g_ck, _) = timed(grad)
+    prob.use_checkpoints(False)
+    same = bool(torch.equal(g, g_ck))
+    log(f"adjoint: forward {t_fwd * 1e3:.1f} ms, forward + adjoint {t_grad * 1e3:.1f} ms "
+        f"({t_grad / t_fwd:.2f}x), with checkpoint recompute {t_ck * 1e3:.1f} ms ({t_ck / t_fwd:.2f}x)")
+    cells = scene["n_cells"]
+    prob.close()
+    return {"workload": "adjoint inverse design: 256x192x96 cells at 20 nm, Si strip waveguide (eps 3.48^2) in "
+                        "oxide (1.44^2), 80x80x11 design region rho~U[0,1] (seed 4), mode-injection source and "
+                        "mode-power objective at 1550 nm, 16-cell CPML, 3000 steps, float32",
+            "forward_ms": round(t_fwd * 1e3, 2), "gradient_ms": round(t_grad * 1e3, 2),
+            "gradient_vs_forward": round(t_grad / t_fwd, 3),
+            "gradient_checkpointed_ms": round(t_ck * 1e3, 2),
+            "gradient_checkpointed_vs_forward": round(t_ck / t_fwd, 3), "checkpoint_every": prob.K,
+            "checkpointed_equals_stored": same, "forward_gcell_per_s": round(cells * prob.n_steps / t_fwd / 1e9, 3),
+            "loss": loss, "grad_norm": float(torch.linalg.vector_norm(g))}
+
+
 def read_traffic():
     if not os.path.exists(TRAFFIC_FILE):
         return None
@@ -183,6 +242,8 @@ def main() -> int:
     ap.add_argument("--batched-steps", type=int, default=300,
                     help="steps per run of the batched-vs-sequential oblique sweep (0: skip)")
     ap.add_argument("--batched-only", action="store_true", help="run only the oblique sweep comparison")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint cost measurement")
     ap.add_argument("--no-tune", action="store_true", help="batched sweep: keep the default schedule")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB (for scaling rocprof FETCH/WRITE counters)")
@@ -212,6 +273,9 @@ def main() -> int:
 
     if args.batched_only:
         print(json.dumps(batched_leg(dev, args.batched_steps, tune=not args.no_tune)), flush=True)
+        return 0
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(dev)), flush=True)
         return 0
 
     t_setup = time.perf_counter()
@@ -284,6 +348,7 @@ def main() -> int:
     batched = None
     if args.batched_steps > 0:
         batched = batched_leg(dev, args.batched_steps, tune=not args.no_tune)
+    adjoint = None if args.no_adjoint else adjoint_leg(dev)
 
     out = {"metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": K,
            "warmup": W, "ms_per_step": round(elapsed * 1e3 / K, 4), "higher_is_better": True, "scaling": "weak",
@@ -295,7 +360,7 @@ def main() -> int:
                       "dft_stride": stride, "dispersive_nodes": int(eng.n_ade), "psi_values": int(eng.n_psi),
                       "material": eng.material_mode, "tiling": list(eng.tiling if tiling is None else tiling),
                       "hip_graphs": graphs, "parallelism": "single GPU"},
-           "roofline": roofline, "cpu_baseline": cpu, "batched": batched}
+           "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "adjoint": adjoint}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -146,6 +146,22 @@ int lsf_add_dft_monitor(lsf_engine* eng, int n_freq, int64_t n_pts, const double
 int lsf_add_time_monitor(lsf_engine* eng, int64_t n_pts, const int64_t* dev_e_idx, void* dev_buf, int64_t n_max,
                          int64_t* dev_pos);
 
+/* Conformal vertex tensor (TensorMesh, engine.py:140-181, 796-923): the D-form update of n_nodes interface E
+ * nodes (native flat indices dev_nodes, components dev_comp) from n_vert tensor vertices. dev_vert_nodes[v][d][2]:
+ * node numbers of vertex v's minus/plus d-nodes; dev_vert_phase (complex64 [v][d][2], complex fields only):
+ * exp(i k . shift L) of those node positions; dev_normal[v][3]. Slots q = k * n_nodes + node (k = 0: the vertex at
+ * the node's index, 1: at index + e_c): dev_slot_vert[q] = tensor vertex or -1 (plain slot). Per slot the filter
+ * inputs dev_frac[S][2 n_nodes] (medium fractions; one-hot for plain slots), dev_h0 (sum of f_s / eps_inf_s over
+ * non-dispersive media), dev_inv_a (1 / sum f_s eps_inf_s); dev_slot_media[S][2 + 3 max_poles] = {1/eps_inf,
+ * n_poles, (c1, c2, c3) per pole}. dev_state (field type, zeroed by the caller):
+ * D[n] | E_prev[n] | out[2n] | Ea[2n] | Eh[S][2n] | Ph[S][max_poles][2][2n] | Pa[S][max_poles][2][2n], n = n_nodes.
+ * Runs after the E update and the E corrections, with the other groups (engine.py:672-676). */
+int lsf_add_tensor_group(lsf_engine* eng, int64_t n_nodes, const int64_t* dev_nodes, const int32_t* dev_comp,
+                         int64_t n_vert, const int32_t* dev_vert_nodes, const void* dev_vert_phase,
+                         const float* dev_normal, const int32_t* dev_slot_vert, int S, int max_poles,
+                         const float* dev_frac, const float* dev_h0, const float* dev_inv_a,
+                         const float* dev_slot_media, void* dev_state);
+
 /* Energy proxy slots (engine.py:315 "energy"), float[2] (or a field-precision pair). */
 int lsf_set_energy(lsf_engine* eng, float* dev_energy);
 

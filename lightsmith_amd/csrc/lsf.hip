@@ -9,6 +9,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "lsf.h"
@@ -33,7 +34,7 @@ int hip_check(hipError_t e, const char* what) {
 
 enum OpKind {
   OP_GATHER, OP_MODAL, OP_TABLE, OP_VIEW, OP_LINE_H, OP_LINE_E, OP_SUB_H, OP_SUB_E, OP_MIXED,
-  OP_TAPE, OP_ADJ_GRAD, OP_ADJ_SRC
+  OP_TAPE, OP_ADJ_GRAD, OP_ADJ_SRC, OP_TENSOR
 };
 
 struct Op {
@@ -72,6 +73,9 @@ struct Op {
   const long long* t2 = nullptr;
   const double* omega = nullptr;
   double dt = 0.0;
+  // conformal vertex tensor
+  lsf::TensorJob<float> tjf{};
+  lsf::TensorJob<c32> tjc{};
 };
 
 struct Monitor {
@@ -306,6 +310,16 @@ void run_op(lsf_engine* e, const Op& o, hipStream_t s) {
       lsf::AdjSrcJob J{o.h_side, o.width, o.n, o.omega, o.dt, (const c32*)o.w, o.target, o.t2, o.scale,
                        (float*)(o.h_side ? e->H : e->E), (const long long*)e->counter, o.n_steps, o.dft_stride};
       if (o.n > 0) hipLaunchKernelGGL(lsf::k_adj_src, dim3(nblk(o.n, 256)), dim3(256), 0, s, J);
+      break;
+    }
+    case OP_TENSOR: {
+      lsf::TensorJob<T> J;
+      if constexpr (sizeof(T) == sizeof(float)) J = o.tjf;
+      else J = o.tjc;
+      J.E = (T*)e->E;
+      hipLaunchKernelGGL(lsf::k_tensor_d<T>, dim3(nblk(J.nN, 256)), dim3(256), 0, s, J);
+      hipLaunchKernelGGL(lsf::k_tensor_slot<T>, dim3(nblk(2 * J.nN, 256)), dim3(256), 0, s, J);
+      hipLaunchKernelGGL(lsf::k_tensor_e<T>, dim3(nblk(J.nN, 256)), dim3(256), 0, s, J);
       break;
     }
     case OP_MIXED:
@@ -673,7 +687,7 @@ int lsf_set_adjoint(lsf_engine* e, int on) {
     if (e->n_ade > 0) return fail(LSF_EINVAL, "the adjoint update has no D-form (ADE) nodes");
     if (e->xw != 64) return fail(LSF_EINVAL, "the adjoint kernels take 64 x cells per wavefront row");
     for (const Op& o : e->ops)
-      if (o.kind == OP_MIXED || o.kind == OP_SUB_H || o.kind == OP_SUB_E || o.kind == OP_LINE_H)
+      if (o.kind == OP_MIXED || o.kind == OP_TENSOR || o.kind == OP_SUB_H || o.kind == OP_SUB_E || o.kind == OP_LINE_H)
         return fail(LSF_EINVAL, "the adjoint engine takes only adjoint sources and gradient operations");
   }
   e->adjoint = on ? 1 : 0;
@@ -739,6 +753,51 @@ int lsf_add_adjoint_source(lsf_engine* e, int h_side, int nf, int64_t np, const 
   o.scale = scale;
   o.n_steps = n_steps;
   o.dft_stride = stride;
+  return add_op(e, o);
+}
+
+int lsf_add_tensor_group(lsf_engine* e, int64_t n_nodes, const int64_t* nodes, const int32_t* comp, int64_t n_vert,
+                         const int32_t* vert_nodes, const void* vert_phase, const float* normal,
+                         const int32_t* slot_vert, int S, int max_poles, const float* frac, const float* h0,
+                         const float* inv_a, const float* slot_media, void* state) {
+  if (valid(e)) return LSF_EINVAL;
+  if (n_nodes < 1 || n_vert < 1 || !nodes || !comp || !vert_nodes || !normal || !slot_vert || !frac || !h0 ||
+      !inv_a || !slot_media || !state || S < 1 || max_poles < 0)
+    return fail(LSF_EINVAL, "missing tensor-group arrays");
+  if (e->cplx != (vert_phase != nullptr))
+    return fail(LSF_EINVAL, "a tensor group carries vertex phases exactly when the fields are complex");
+  Op o;
+  o.kind = OP_TENSOR;
+  o.phase = LSF_PH_GROUPS;
+  o.n = n_nodes;
+  auto fill = [&](auto& J, auto* st) {
+    using TT = std::remove_pointer_t<decltype(st)>;
+    const long long nN = n_nodes, ns = 2 * nN;
+    J.nN = nN;
+    J.node = (const long long*)nodes;
+    J.comp = comp;
+    J.vn = vert_nodes;
+    J.vph = (const TT*)vert_phase;
+    J.normal = normal;
+    J.slot_v = slot_vert;
+    J.S = S;
+    J.maxp = max_poles;
+    J.ms = 2 + 3 * max_poles;
+    J.f = frac;
+    J.h0 = h0;
+    J.inv_a = inv_a;
+    J.sm = slot_media;
+    J.D = st;
+    J.Ep = st + nN;
+    J.out = st + 2 * nN;
+    J.Ea = st + 2 * nN + ns;
+    J.Eh = J.Ea + ns;
+    J.Ph = J.Eh + (long long)S * ns;
+    J.Pa = J.Ph + (long long)S * max_poles * 2 * ns;
+    J.E = nullptr;
+  };
+  if (e->cplx) fill(o.tjc, (c32*)state);
+  else fill(o.tjf, (float*)state);
   return add_op(e, o);
 }
 

@@ -27,6 +27,9 @@ __device__ __forceinline__ c32 mul(c32 a, c32 b) { return mk(a.x * b.x - a.y * b
 __device__ __forceinline__ float rot(float a, c32) { return a; }
 __device__ __forceinline__ c32 rot(c32 a, c32 p) { return mul(a, p); }
 __device__ __forceinline__ c32 conjg(c32 a) { return mk(a.x, -a.y); }
+// complex conjugate on the field type (identity for real values)
+__device__ __forceinline__ float conjg_t(float a) { return a; }
+__device__ __forceinline__ c32 conjg_t(c32 a) { return conjg(a); }
 __device__ __forceinline__ c32 to_c(float a) { return mk(a, 0.f); }
 __device__ __forceinline__ c32 to_c(c32 a) { return a; }
 __device__ __forceinline__ float norm2(float a) { return a * a; }

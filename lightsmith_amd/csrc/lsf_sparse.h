@@ -221,6 +221,147 @@ __global__ void k_mixed(long long n, int S, int maxp, const long long* __restric
   Ep[r] = out;
 }
 
+// Conformal vertex tensor (engine.py:839-923). Filters of a slot r (of n slots) for a medium table sm[S][ms]:
+// harmonic y = h0 u + sum_s f_s E_s, E_s = (u - sum_p P_s,p) / eps_inf_s with each medium's poles driven by its
+// own previous E_s (engine.py:796-816); arithmetic y = (u - sum_s f_s sum_p P_s,p) inv_a with all poles driven
+// by the previous y (engine.py:819-836).
+template <class T>
+__device__ __forceinline__ T tensor_harmonic(T u, long long r, long long n, int S, int maxp, const float* sm, int ms,
+                                             const float* f, float h0, T* Eh, T* Ph) {
+  T y = scl(h0, u);
+  for (int s = 0; s < S; ++s) {
+    const float* mp = sm + s * ms;
+    const int np = (int)mp[1];
+    const float fs = f[s * n + r];
+    if (np == 0 || fs == 0.f) continue;
+    const T eh = Eh[s * n + r];
+    T tot = zero<T>();
+    for (int p = 0; p < np; ++p) {
+      T* P0 = Ph + ((long long)(s * maxp + p) * 2) * n;
+      T* P1 = P0 + n;
+      const T pn = add(add(scl(mp[2 + 3 * p], P0[r]), scl(mp[3 + 3 * p], P1[r])), scl(mp[4 + 3 * p], eh));
+      P1[r] = P0[r];
+      P0[r] = pn;
+      tot = p == 0 ? pn : add(tot, pn);
+    }
+    const T ehn = scl(mp[0], sub(u, tot));
+    Eh[s * n + r] = ehn;
+    y = add(y, scl(fs, ehn));
+  }
+  return y;
+}
+
+template <class T>
+__device__ __forceinline__ T tensor_arithmetic(T u, long long r, long long n, int S, int maxp, const float* sm,
+                                               int ms, const float* f, float inv_a, T* Ea, T* Pa) {
+  T acc = u;
+  const T ea = Ea[r];
+  for (int s = 0; s < S; ++s) {
+    const float* mp = sm + s * ms;
+    const int np = (int)mp[1];
+    const float fs = f[s * n + r];
+    if (np == 0 || fs == 0.f) continue;
+    T tot = zero<T>();
+    for (int p = 0; p < np; ++p) {
+      T* P0 = Pa + ((long long)(s * maxp + p) * 2) * n;
+      T* P1 = P0 + n;
+      const T pn = add(add(scl(mp[2 + 3 * p], P0[r]), scl(mp[3 + 3 * p], P1[r])), scl(mp[4 + 3 * p], ea));
+      P1[r] = P0[r];
+      P0[r] = pn;
+      tot = p == 0 ? pn : add(tot, pn);
+    }
+    acc = sub(acc, scl(fs, tot));
+  }
+  const T y = scl(inv_a, acc);
+  Ea[r] = y;
+  return y;
+}
+
+template <class T>
+struct TensorJob {
+  long long nN;             // nodes; slots 2 nN (slot k nN + node: k = 0 the vertex at the node index, 1 at + e_c)
+  const long long* node;    // native E index of each node
+  const int* comp;          // component of each node
+  const int* vn;            // [nV][3][2] node numbers of each vertex's (minus, plus) d-nodes
+  const T* vph;             // [nV][3][2] Bloch phase of those node positions, or null
+  const float* normal;      // [nV][3]
+  const int* slot_v;        // [2 nN] tensor vertex of each slot, -1 for a plain slot
+  int S, maxp, ms;
+  const float* f;           // [S][2 nN] medium fractions (one-hot for plain slots)
+  const float* h0;          // [2 nN]
+  const float* inv_a;       // [2 nN]
+  const float* sm;          // [S][ms] {1/eps_inf, n_poles, (c1, c2, c3) per pole}
+  T* D;                     // [nN]
+  T* Ep;                    // [nN]
+  T* out;                   // [2 nN]
+  T* Ea;                    // [2 nN]
+  T* Eh;                    // [S][2 nN]
+  T* Ph;                    // [S][maxp][2][2 nN]
+  T* Pa;                    // [S][maxp][2][2 nN]
+  T* E;
+};
+
+// D += E_now - E_prev per node
+template <class T>
+__global__ void k_tensor_d(TensorJob<T> J) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= J.nN) return;
+  J.D[q] = add(J.D[q], sub(J.E[J.node[q]], J.Ep[q]));
+}
+
+// one slot: the tensor vertex's share, or half the plain medium's
+template <class T>
+__global__ void k_tensor_slot(TensorJob<T> J) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = 2 * J.nN;
+  if (q >= n) return;
+  const int k = q >= J.nN ? 1 : 0;
+  const long long node = q - k * J.nN;
+  const int v = J.slot_v[q];
+  T contrib;
+  if (v >= 0) {
+    T dv[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+      dv[t] = J.D[J.vn[(long long)v * 6 + t]];
+      if (J.vph) dv[t] = mul(dv[t], J.vph[(long long)v * 6 + t]);
+    }
+    T dbar[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) dbar[d] = scl(0.5f, add(dv[2 * d], dv[2 * d + 1]));
+    const float* nv = J.normal + (long long)v * 3;
+    const T sv = add(add(scl(nv[0], dbar[0]), scl(nv[1], dbar[1])), scl(nv[2], dbar[2]));
+    const int c = J.comp[node];
+    const int side = 1 - k;
+    T dp = dv[0], db = dbar[0];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (d == c) {
+        dp = side ? dv[2 * d + 1] : dv[2 * d];
+        db = dbar[d];
+      }
+    const float nc = nv[c];
+    const T sigma = add(sv, scl(nc, sub(dp, db)));
+    const T b = tensor_arithmetic(sub(dp, scl(nc, sigma)), q, n, J.S, J.maxp, J.sm, J.ms, J.f, J.inv_a[q], J.Ea, J.Pa);
+    const T a = tensor_harmonic(sigma, q, n, J.S, J.maxp, J.sm, J.ms, J.f, J.h0[q], J.Eh, J.Ph);
+    contrib = scl(0.5f, add(b, scl(nc, a)));
+    if (J.vph) contrib = mul(contrib, conjg_t(J.vph[(long long)v * 6 + c * 2 + side]));
+  } else {
+    contrib = scl(0.5f, tensor_harmonic(J.D[node], q, n, J.S, J.maxp, J.sm, J.ms, J.f, J.h0[q], J.Eh, J.Ph));
+  }
+  J.out[q] = contrib;
+}
+
+// E_p = the sum of its two slots
+template <class T>
+__global__ void k_tensor_e(TensorJob<T> J) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= J.nN) return;
+  const T e = add(J.out[q], J.out[J.nN + q]);
+  J.E[J.node[q]] = e;
+  J.Ep[q] = e;
+}
+
 // Running DFT monitor (engine.py:564-592). Block size 256; phases for all frequencies in LDS, computed in
 // double from the step counter and cast to complex64 as the reference does.
 template <class S>

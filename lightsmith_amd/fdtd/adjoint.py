@@ -124,6 +124,15 @@ class AdjointProblem:
         self.checkpoints = []
         self.timings = {}
 
+    def use_checkpoints(self, on: bool):
+        """Switch between the stored tape (no recompute) and checkpoint recompute (one interval of tape in use);
+        the tape allocated at construction must hold at least one checkpoint interval."""
+        if not on and self.n_slots < self.n_steps:
+            raise ValueError("the tape was sized for checkpoint recompute (tape_budget); it cannot hold every step")
+        if on and self.n_slots < self.K:
+            raise ValueError("the tape holds less than one checkpoint interval")
+        self.store_all = not on
+
     # --- density -> material --------------------------------------------------------------------------
     def eps(self, rho: torch.Tensor) -> torch.Tensor:
         d = self.design

@@ -13,8 +13,7 @@ flat indices (engine.py:103-105) to native ones. Periodic ghost planes are read 
 inside the kernels and are not materialised in the arrays.
 
 Precision: float32 fields (complex64 under a Bloch phase), as the reference's GPU path (single=True).
-Not supported here: mirror planes and graded grids (the reference's symmetry.py and grid.py paths), and
-the vertex-tensor interface groups (TensorMesh).
+Not supported here: mirror planes and graded grids (the reference's symmetry.py and grid.py paths).
 """
 
 from __future__ import annotations
@@ -98,6 +97,34 @@ class DFTMonitor:
         e = self.acc_e.cpu().numpy().astype(complex)
         h = None if self.acc_h is None else self.acc_h.cpu().numpy().astype(complex)
         return e, h
+
+
+@dataclass
+class TensorMesh:
+    """engine.py:140-181: interface nodes updated with the smoothed 1/eps tensor applied at grid vertices."""
+
+    media: list
+    frac: np.ndarray  # (S, nV)
+    normal: np.ndarray  # (nV, 3)
+    node_comp: np.ndarray  # (nN,)
+    node_idx3: np.ndarray  # (nN, 3)
+    vert_nodes: np.ndarray  # (nV, 3, 2)
+    shift: np.ndarray  # (nV, 3, 2, 3)
+    slot_vert: np.ndarray  # (nN, 2)
+    slot_medium: np.ndarray  # (nN, 2)
+    shape: tuple
+    sign: np.ndarray | None = None
+    vert_idx3: np.ndarray | None = None
+
+    @property
+    def idx(self) -> np.ndarray:
+        i = self.node_idx3
+        return np.asarray(flat_index(self.node_comp, i[:, 0], i[:, 1], i[:, 2], self.shape), dtype=np.int64)
+
+
+def _is_tensor(g) -> bool:
+    """A TensorMesh of this package or of the reference (the same fields)."""
+    return all(hasattr(g, a) for a in ("slot_vert", "vert_nodes", "node_comp", "normal"))
 
 
 @dataclass
@@ -303,14 +330,17 @@ class Engine:
         if inv_eps.shape != (3,) + self.shape:
             raise ValueError(f"inv_eps must have shape {(3,) + self.shape}, got {inv_eps.shape}")
         groups = list(groups)
-        for grp in groups:
+        tensors = [g for g in groups if _is_tensor(g)]
+        dgroups = [g for g in groups if not _is_tensor(g)]
+        for grp in dgroups:
             if not isinstance(grp, DispersiveGroup):
-                raise ValueError(f"unsupported dispersive group type {type(grp).__name__} (vertex tensors: "
-                                 "mesh_refinement other than 'conformal')")
-        self._build_ade(groups)
+                raise ValueError(f"unsupported dispersive group type {type(grp).__name__}")
+        self._build_ade(dgroups)
         self._set_material(inv_eps, material)
         self._build_pml(pml or PMLSpec(cells=((0, 0),) * 3), inv_eps)
-        self._build_mixed(groups)
+        self._build_mixed(dgroups)
+        for tm in tensors:
+            self._build_tensor(tm)
         self._build_offdiag(offdiag, groups)
         self.energy = torch.zeros(2, dtype=torch.float32, device=self._dev)
         check(self.lib.lsf_set_energy(self._h, ptr(self.energy)))
@@ -528,6 +558,49 @@ class Engine:
                     self.to_device(sm.astype(np.float32))]
             check(self.lib.lsf_add_mixed_group(self._h, n, S, maxp, *(ptr(a) for a in arrs), ptr(state),
                                                int(np.any(w > 0)), int(np.any(w < 1))), "lsf_add_mixed_group")
+
+    # --- conformal vertex tensor (engine.py:839-923) ---------------------------------------------
+    def _build_tensor(self, g):
+        if getattr(g, "sign", None) is not None:
+            raise ValueError("vertex tensors next to mirror planes are not supported (run symmetry='off')")
+        nN, nV = len(g.node_comp), len(g.normal)
+        nat = self.native_index(g.idx)
+        self.check_index(nat, 3 * self.N, "tensor node index")
+        media = list(g.media)
+        S = len(media)
+        maxp = max([len(m.poles) for m in media] + [1])
+        sv = np.asarray(g.slot_vert, dtype=np.int64)
+        slot_v = np.concatenate([sv[:, 0], sv[:, 1]])  # slot k * nN + node
+        slot_m = np.concatenate([g.slot_medium[:, 0], g.slot_medium[:, 1]])
+        frac = np.asarray(g.frac, dtype=float)
+        F = np.zeros((S, 2 * nN))
+        tensor = slot_v >= 0
+        F[:, tensor] = frac[:, slot_v[tensor]]
+        F[slot_m[~tensor], np.flatnonzero(~tensor)] = 1.0
+        h0 = sum(F[s] / m.eps_inf for s, m in enumerate(media) if not m.dispersive) + 0 * F[0]
+        inv_a = 1.0 / sum(F[s] * m.eps_inf for s, m in enumerate(media))
+        sm = np.zeros((S, 2 + 3 * maxp))
+        for s, m in enumerate(media):
+            sm[s, 0], sm[s, 1] = 1.0 / m.eps_inf, len(m.poles)
+            for p, pole in enumerate(m.poles):
+                sm[s, 2 + 3 * p:5 + 3 * p] = pole_coefficients(pole, self.dt)
+        vph = None
+        if self.complex:
+            L = np.array([(n - 2) * d for n, d in zip(self.shape, self.spacing)])
+            ph = np.exp(1j * (np.asarray(g.shift) * (np.asarray(self.bloch) * L)).sum(-1))  # (nV, 3, 2)
+            vph = self.to_device(ph.reshape(-1).astype(np.complex128))
+        state = self.zeros((6 + 2 * S + 8 * S * maxp) * nN)
+        arrs = dict(nodes=self.to_device(nat), comp=self.to_device(np.asarray(g.node_comp, np.int32)),
+                    vn=self.to_device(np.asarray(g.vert_nodes, np.int32).reshape(-1)),
+                    normal=self.to_device(np.asarray(g.normal, np.float32).reshape(-1)),
+                    slot_v=self.to_device(slot_v.astype(np.int32)), f=self.to_device(F.astype(np.float32)),
+                    h0=self.to_device(np.asarray(h0, np.float32)), inv_a=self.to_device(inv_a.astype(np.float32)),
+                    sm=self.to_device(sm.astype(np.float32)))
+        check(self.lib.lsf_add_tensor_group(self._h, nN, ptr(arrs["nodes"]), ptr(arrs["comp"]), nV, ptr(arrs["vn"]),
+                                            ptr(vph), ptr(arrs["normal"]), ptr(arrs["slot_v"]), S, maxp,
+                                            ptr(arrs["f"]), ptr(arrs["h0"]), ptr(arrs["inv_a"]), ptr(arrs["sm"]),
+                                            ptr(state)), "lsf_add_tensor_group")
+        self.n_tensor_nodes = getattr(self, "n_tensor_nodes", 0) + nN
 
     # --- off-diagonal tensor terms (engine.py:421-518, uniform grid) ------------------------------
     def _build_offdiag(self, od, groups):

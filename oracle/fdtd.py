@@ -149,7 +149,7 @@ class Engine:
         self.cH = cdx
         self._build_pml(pml or PMLSpec(cells=((0, 0),) * 3), inv_eps)
         self._build_offdiag(offdiag, list(groups))
-        self.groups = [PureGroup(self, g) if g.w is None else MixedGroup(self, g) for g in groups]
+        self.groups = [_group(self, g) for g in groups]
         self.sources = list(sources)
         for s in self.sources:
             s.bind(self)
@@ -408,6 +408,14 @@ def _advance(pairs, coefs, e):
         pair[0] = new
         tot = new if tot is None else tot + new
     return tot
+
+
+def _group(eng, g):
+    """The group of a DispersiveGroup (pure or mixed) or of a TensorMesh (oracle/tensor.py)."""
+    from .tensor import TensorGroup, TensorMesh
+    if isinstance(g, TensorMesh):
+        return TensorGroup(eng, g)
+    return PureGroup(eng, g) if g.w is None else MixedGroup(eng, g)
 
 
 class PureGroup:

@@ -28,14 +28,25 @@ def _cat_set(F: torch.Tensor, R, v: torch.Tensor) -> torch.Tensor:
 class TorchRun:
     """Forward run of an oracle Engine's configuration in torch. `eng` must be freshly built (fields zero)."""
 
-    def __init__(self, eng: Engine, lo, hi, eps_lo: float, eps_hi: float):
+    def __init__(self, eng: Engine, lo, hi, eps_lo: float, eps_hi: float, device="cpu"):
         if eng.complex or any(eng.periodic) or eng.groups or eng.od is not None:
             raise ValueError("TorchRun restates real fields without periodic axes, groups or off-diagonal terms")
         self.eng = eng
+        self.dev = torch.device(device)
         self.lo, self.hi = tuple(lo), tuple(hi)
         self.eps_lo, self.eps_hi = float(eps_lo), float(eps_hi)
         self.cdx = eng.dt / eng.dx if eng.uniform else eng.dt
         self.box = tuple(slice(a, b) for a, b in zip(self.lo, self.hi))
+        d = self.dev
+        conv = lambda a: None if a is None else torch.as_tensor(a, dtype=T, device=d)  # noqa: E731
+        self.pml_H = [(c, s_, dd, Rs, conv(bb), conv(cp), conv(kp)) for c, s_, dd, Rs, bb, cp, kp, _ in eng.pml_H]
+        self.pml_E = [(c, s_, dd, Rs, conv(bb), conv(cp), conv(kp)) for c, s_, dd, Rs, bb, cp, kp, _ in eng.pml_E]
+        self.src = []
+        for s in eng.sources:
+            self.src.append((torch.as_tensor(np.asarray(s.h_idx, dtype=np.int64), device=d), conv(s.mH @ s.tE.T).T
+                             if len(s.h_idx) else None,
+                             torch.as_tensor(np.asarray(s.e_idx, dtype=np.int64), device=d), conv(s.mE @ s.tH.T).T
+                             if len(s.e_idx) else None))
 
     def cb(self, rho: torch.Tensor):
         """Full-grid E coefficient per component with the design box from rho (each E_c over its stepped box)."""
@@ -43,28 +54,31 @@ class TorchRun:
         out = []
         eps = self.eps_lo + rho * (self.eps_hi - self.eps_lo)
         for c in range(3):
-            full = torch.zeros(eng.shape, dtype=T)
-            full[eng.RE[c]] = torch.as_tensor(eng.cbE[c], dtype=T)
+            full = torch.zeros(eng.shape, dtype=T, device=self.dev)
+            full[eng.RE[c]] = torch.as_tensor(eng.cbE[c], dtype=T, device=self.dev)
             out.append(_cat_set(full, self.box, self.cdx / eps))
         return out
 
     def run(self, rho: torch.Tensor, n_steps: int):
-        """(acc_e, acc_h) of the engine's first DFT monitor after n_steps (sampled every eng.stride steps)."""
-        eng = self.eng
-        cb = self.cb(rho)
-        E = torch.zeros((3,) + eng.shape, dtype=T)
-        H = torch.zeros((3,) + eng.shape, dtype=T)
-        psiE = [torch.zeros(t[5].shape, dtype=T) for t in eng.pml_E]
-        psiH = [torch.zeros(t[5].shape, dtype=T) for t in eng.pml_H]
+        """(acc_e, acc_h) of the engine's first DFT monitor after n_steps (sampled every eng.stride steps).
+        Differentiable in rho; under torch.no_grad() the updates run in place."""
+        eng, d = self.eng, self.dev
+        inplace = not torch.is_grad_enabled()
+        put = (lambda F, R, v: F.__setitem__(R, v) or F) if inplace else _cat_set  # noqa: E731
+        cb = self.cb(rho.to(d))
+        E = torch.zeros((3,) + eng.shape, dtype=T, device=d)
+        H = torch.zeros((3,) + eng.shape, dtype=T, device=d)
+        psiE = [torch.zeros(t[5].shape, dtype=T, device=d) for t in self.pml_E]
+        psiH = [torch.zeros(t[5].shape, dtype=T, device=d) for t in self.pml_H]
         m = eng.monitors[0]
-        omega = torch.as_tensor(np.asarray(m.omega, dtype=float), dtype=T)
-        e_idx = torch.as_tensor(np.asarray(m.e_idx))
-        acc_e = torch.zeros((len(omega), len(e_idx)), dtype=torch.complex128)
+        omega = torch.as_tensor(np.asarray(m.omega, dtype=float), dtype=T, device=d)
+        e_idx = torch.as_tensor(np.asarray(m.e_idx), device=d)
+        acc_e = torch.zeros((len(omega), len(e_idx)), dtype=torch.complex128, device=d)
         has_h = m.h_idx1 is not None
         if has_h:
-            h1, h2 = torch.as_tensor(np.asarray(m.h_idx1)), torch.as_tensor(np.asarray(m.h_idx2))
-            acc_h = torch.zeros((len(omega), len(h1)), dtype=torch.complex128)
-        src = [s for s in eng.sources]
+            h1 = torch.as_tensor(np.asarray(m.h_idx1), device=d)
+            h2 = torch.as_tensor(np.asarray(m.h_idx2), device=d)
+            acc_h = torch.zeros((len(omega), len(h1)), dtype=torch.complex128, device=d)
         for n in range(n_steps):
             # H update (engine.py:630-643)
             Hn = []
@@ -75,17 +89,16 @@ class TorchRun:
                 b = E[d1][_moved(R, d2, 1)] - E[d1][R]
                 if not eng.uniform:
                     a, b = a * eng.inv_d[d1], b * eng.inv_d[d2]
-                Hn.append(_cat_set(H[c], R, H[c][R] - eng.cH * (a - b)))
-            for q, (c, s_, d, Rs, bb, cp, kp, _) in enumerate(eng.pml_H):
-                der = E[s_][_moved(Rs, d, 1)] - E[s_][Rs]
-                psiH[q] = torch.as_tensor(bb, dtype=T) * psiH[q] + torch.as_tensor(cp, dtype=T) * der
-                inc = psiH[q] + (torch.as_tensor(kp, dtype=T) * der if kp is not None else 0.0)
-                Hn[c] = _cat_set(Hn[c], Rs, Hn[c][Rs] + inc)
-            H = torch.stack(Hn)
-            for s in src:
-                if len(s.h_idx):
-                    H = H.reshape(-1).index_add(0, torch.as_tensor(s.h_idx),
-                                                torch.as_tensor(s.mH @ s.tE[n], dtype=T)).reshape(H.shape)
+                Hn.append(put(H[c], R, H[c][R] - eng.cH * (a - b)))
+            for q, (c, s_, dd, Rs, bb, cp, kp) in enumerate(self.pml_H):
+                der = E[s_][_moved(Rs, dd, 1)] - E[s_][Rs]
+                psiH[q] = bb * psiH[q] + cp * der
+                inc = psiH[q] + (kp * der if kp is not None else 0.0)
+                Hn[c] = put(Hn[c], Rs, Hn[c][Rs] + inc)
+            H = H if inplace else torch.stack(Hn)
+            for h_idx, mh, _, _ in self.src:
+                if mh is not None:
+                    H = H.reshape(-1).index_add(0, h_idx, mh[n]).reshape(H.shape)
             # E update (engine.py:645-663)
             En = []
             for c in range(3):
@@ -95,17 +108,16 @@ class TorchRun:
                 b = H[d1][R] - H[d1][_moved(R, d2, -1)]
                 if not eng.uniform:
                     a, b = a * eng.inv_d[d1], b * eng.inv_d[d2]
-                En.append(_cat_set(E[c], R, E[c][R] + cb[c][R] * (a - b)))
-            for q, (c, s_, d, Rs, bb, cp, kp, _) in enumerate(eng.pml_E):
-                der = H[s_][Rs] - H[s_][_moved(Rs, d, -1)]
-                psiE[q] = torch.as_tensor(bb, dtype=T) * psiE[q] + torch.as_tensor(cp, dtype=T) * der
-                inc = psiE[q] + (torch.as_tensor(kp, dtype=T) * der if kp is not None else 0.0)
-                En[c] = _cat_set(En[c], Rs, En[c][Rs] + inc)
-            E = torch.stack(En)
-            for s in src:
-                if len(s.e_idx):
-                    E = E.reshape(-1).index_add(0, torch.as_tensor(s.e_idx),
-                                                torch.as_tensor(s.mE @ s.tH[n], dtype=T)).reshape(E.shape)
+                En.append(put(E[c], R, E[c][R] + cb[c][R] * (a - b)))
+            for q, (c, s_, dd, Rs, bb, cp, kp) in enumerate(self.pml_E):
+                der = H[s_][Rs] - H[s_][_moved(Rs, dd, -1)]
+                psiE[q] = bb * psiE[q] + cp * der
+                inc = psiE[q] + (kp * der if kp is not None else 0.0)
+                En[c] = put(En[c], Rs, En[c][Rs] + inc)
+            E = E if inplace else torch.stack(En)
+            for _, _, e_src, me in self.src:
+                if me is not None:
+                    E = E.reshape(-1).index_add(0, e_src, me[n]).reshape(E.shape)
             # DFT sample after each chunk (engine.py:564-592)
             if (n + 1) % eng.stride == 0:
                 te = (n + 1) * eng.dt

@@ -19,7 +19,13 @@ def oracle_ns():
                                  TFSFPlaneWave=sources.TFSFPlaneWave, TableSource=sources.TableSource,
                                  BlochPlaneSource=sources.BlochPlaneSource, ModeInjection=sources.ModeInjection,
                                  Medium=media.Medium, Pole=media.Pole, flat_index=fdtd.flat_index,
-                                 e_region=fdtd.e_region, h_region=fdtd.h_region, oracle=True)
+                                 e_region=fdtd.e_region, h_region=fdtd.h_region, TensorMesh=_oracle_tensor_mesh(),
+                                 oracle=True)
+
+
+def _oracle_tensor_mesh():
+    from oracle.tensor import TensorMesh
+    return TensorMesh
 
 
 def product_ns():
@@ -31,7 +37,8 @@ def product_ns():
                                  TFSFPlaneWave=sources.TFSFPlaneWave, TableSource=sources.TableSource,
                                  BlochPlaneSource=sources.BlochPlaneSource, ModeInjection=sources.ModeInjection,
                                  Medium=fdtd.Medium, Pole=fdtd.Pole, flat_index=fdtd.flat_index,
-                                 e_region=fdtd.e_region, h_region=fdtd.h_region, oracle=False)
+                                 e_region=fdtd.e_region, h_region=fdtd.h_region, TensorMesh=fdtd.TensorMesh,
+                                 oracle=False)
 
 
 def make(ns, device=None, **kw):
@@ -268,7 +275,8 @@ def adjoint_oracle(case, rho):
                                 case["m_src"], case["t_src"])
     mon = fdtd.DFTMonitor(case["omega"], case["e_mon"], case["h1"], case["h2"])
     return fdtd.Engine(shape=case["shape"], dx=case["dx"], dt=case["dt"], inv_eps=adjoint_inv_eps(case, rho),
-                       pml=fdtd.PMLSpec(cells=((p, p),) * 3, kappa_max=2.0, alpha_max=0.02, eps_ref=case["eps_lo"]),
+                       pml=fdtd.PMLSpec(cells=((p, p),) * 3, kappa_max=case.get("kappa_max", 2.0),
+                                        alpha_max=case.get("alpha_max", 0.02), eps_ref=case["eps_lo"]),
                        sources=[src], monitors=[mon], dft_stride=case["stride"])
 
 
@@ -286,3 +294,57 @@ def adjoint_problem(case, device, **kw):
                           sources=[src], monitor=mon,
                           design=DesignRegion(case["lo"], case["hi"], case["eps_lo"], case["eps_hi"]),
                           n_steps=case["steps"], dft_stride=case["stride"], **kw)
+
+
+# --- conformal vertex tensor ---------------------------------------------------------------------------
+
+def tensor_sphere(ns, device=None, shape=(24, 24, 24), steps=300, periodic=(False, False, False), bloch=(0.0, 0.0, 0.0),
+                  pml_cells=4, center=(0.3, 0.1, -0.2), radius=6.4):
+    """A Drude-metal sphere (eps_inf 9.5, omega_p = 9 eV) in water (eps 1.77), meshed with the conformal vertex
+    tensor (the oracle's restatement of setup.vertex_tensor) around the interface: staircase water / D-form metal
+    nodes elsewhere, tensor nodes at inv_eps = 1. A z dipole next to the sphere, CPML on the non-periodic axes,
+    a DFT monitor on an x plane."""
+    from oracle import media as omed
+    from oracle.tensor import vertex_tensor
+    o_media = [omed.Medium(1.77, ()), omed.Medium(9.5, (omed.Pole(0.00208, 0.0, 1.8e-4),))]
+    media = [ns.Medium(1.77, ()), ns.Medium(9.5, (ns.Pole(0.00208, 0.0, 1.8e-4),))]
+    c = np.asarray(center)
+    dx = 1.0
+    origin = -np.asarray(shape, float) / 2 * dx
+    slot_fn = lambda p: (np.linalg.norm(p - c, axis=-1) < radius).astype(int)  # noqa: E731
+    roi_lo, roi_hi = np.full(3, 2), np.asarray(shape) - 2
+    tm = vertex_tensor(slot_fn, o_media, shape, origin, dx, roi_lo, roi_hi, periodic=periodic)
+    assert tm is not None and len(tm.normal) > 50
+    n0, n1, n2 = shape
+    inv_eps = np.empty((3,) + shape)
+    metal = []
+    for comp in range(3):
+        off = np.zeros(3)
+        off[comp] = 0.5
+        ii, jj, kk = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+        pos = origin + (np.stack([ii, jj, kk], -1) + off) * dx
+        m = slot_fn(pos.reshape(-1, 3)).reshape(shape)
+        inv_eps[comp] = np.where(m == 1, 1.0, 1 / 1.77)
+        mi = np.nonzero(m == 1)
+        metal.append(ns.flat_index(comp, *mi, shape))
+    metal = np.concatenate(metal)
+    taken = tm.idx
+    inv_eps.reshape(-1)[taken] = 1.0
+    metal = metal[~np.isin(metal, taken)]
+    tm_ns = ns.TensorMesh(media=media, frac=tm.frac, normal=tm.normal, node_comp=tm.node_comp,
+                          node_idx3=tm.node_idx3, vert_nodes=tm.vert_nodes, shift=tm.shift, slot_vert=tm.slot_vert,
+                          slot_medium=tm.slot_medium, shape=tm.shape, vert_idx3=tm.vert_idx3)
+    groups = [ns.DispersiveGroup(idx=metal, media=[media[1]], frac=np.ones((1, len(metal))), w=None), tm_ns]
+    dt = 0.5
+    pos = c + np.array([radius + 2.3, 0.4, 0.2])
+    src = ns.PointDipole(shape=shape, origin=origin, dx=dx, dt=dt, pos=pos, pdir=(0.1, 0.2, 1.0), inv_eps=inv_eps,
+                         omega_c=2 * np.pi / 25.0, omega_half=0.3 * 2 * np.pi / 25.0, max_steps=steps + 10)
+    cells = tuple((0, 0) if p else (pml_cells, pml_cells) for p in periodic)
+    kx = n0 // 2 + 5
+    jj, kk = np.meshgrid(np.arange(3, n1 - 3), np.arange(3, n2 - 3), indexing="ij")
+    e_idx = np.concatenate([ns.flat_index(cc, np.full(jj.size, kx), jj.ravel(), kk.ravel(), shape) for cc in (1, 2)])
+    mon = ns.DFTMonitor(2 * np.pi / np.array([22.0, 28.0]), e_idx)
+    kw = dict(shape=shape, dx=dx, dt=dt, inv_eps=inv_eps, groups=groups,
+              pml=ns.PMLSpec(cells=cells, eps_ref=1.77), sources=[src], monitors=[mon], dft_stride=5,
+              periodic=periodic, bloch=bloch)
+    return make(ns, device, **kw)

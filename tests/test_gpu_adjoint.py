@@ -6,6 +6,8 @@ Tolerance: relative L2 error of dL/drho over the design region < 1e-3 (the float
 reference). Checkpoint recompute must reproduce the stored-tape gradient exactly.
 """
 
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -25,7 +27,8 @@ def _hip_grad(case, gpu, **kw):
     acc_e, acc_h = monitor_spectra(rho, prob)
     loss = cases.adjoint_objective(acc_e, acc_h, case)
     loss.backward()
-    out = rho.grad.detach().cpu().numpy(), float(loss), acc_e.cpu().numpy(), acc_h.cpu().numpy(), prob
+    out = (rho.grad.detach().cpu().numpy(), float(loss), acc_e.detach().cpu().numpy(), acc_h.detach().cpu().numpy(),
+           prob)
     return out
 
 
@@ -67,3 +70,41 @@ def test_adjoint_refuses_what_it_does_not_transpose(gpu):
     case = cases.adjoint_case()
     with pytest.raises(ValueError):
         cases.adjoint_problem(dict(case, lo=(2, 9, 8)), gpu)  # design region inside the CPML
+
+
+@pytest.mark.skipif(not os.environ.get("LSF_ADJOINT_FULL"),
+                    reason="full adjoint scene against float64 central differences takes minutes; LSF_ADJOINT_FULL=1")
+def test_full_scene_gradient_matches_float64_central_differences(gpu):
+    """The 256x192x96 adjoint scene (3000 steps): dL/drho on 32 seeded voxels (seed 5) against central
+    differences of the float64 torch restatement run on the GPU (h = 1e-3)."""
+    from lightsmith_amd.fdtd import workloads
+    scene = workloads.adjoint_scene()
+    prob = workloads.adjoint_problem(scene, gpu)
+    rho = torch.tensor(scene["rho"], dtype=torch.float64, device=gpu, requires_grad=True)
+    acc_e, _ = monitor_spectra(rho, prob)
+    workloads.mode_power(acc_e, scene["w"]).backward()
+    g = rho.grad.cpu().numpy()
+    prob.close()
+    case = dict(scene, steps=scene["n_steps"], kappa_max=1.0, alpha_max=0.05 * float(scene["omega"][0]))
+    o = cases.adjoint_oracle(case, scene["rho"])
+    run = TorchRun(o, scene["lo"], scene["hi"], scene["eps_lo"], scene["eps_hi"], device=gpu)
+    rng = np.random.default_rng(5)
+    picks = [tuple(int(rng.integers(0, s)) for s in scene["rho"].shape) for _ in range(32)]
+    h = 1e-3
+    fd, ad = [], []
+    with torch.no_grad():
+        for q, v in enumerate(picks):
+            vals = []
+            for sgn in (1, -1):
+                r = scene["rho"].copy()
+                r[v] += sgn * h
+                e, _ = run.run(torch.as_tensor(r), scene["n_steps"])
+                vals.append(float(workloads.mode_power(e.cpu().numpy(), scene["w"])))
+            fd.append((vals[0] - vals[1]) / (2 * h))
+            ad.append(g[v])
+            print(f"voxel {q} {v}: adjoint {ad[-1]:.6e}  central difference {fd[-1]:.6e}", flush=True)
+    fd, ad = np.array(fd), np.array(ad)
+    err = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
+    print(f"32 voxels: relative L2 error {err:.3e}; max per-voxel relative error "
+          f"{np.max(np.abs(ad - fd) / np.abs(fd)):.3e}")
+    assert err < GRAD_TOL

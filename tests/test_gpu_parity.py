@@ -284,3 +284,23 @@ def test_mixed_groups_and_offdiagonal(gpu):
         out.append(eng)
     o, o32, g = out
     _compare(o, g, "mixed groups + off-diagonal + pure", o32=o32)
+
+
+@pytest.mark.parametrize("bloch", [False, True])
+def test_vertex_tensor_sphere(gpu, bloch):
+    """Conformal vertex tensor around a Drude sphere (TensorMesh), with pure D-form metal nodes, a dipole and a DFT
+    plane; Bloch-periodic in x and y (complex64) in the second case."""
+    kw = dict(shape=(26, 26, 24), periodic=(True, True, False), bloch=(0.05, 0.02, 0.0)) if bloch else {}
+    steps = 300
+    o = cases.tensor_sphere(cases.oracle_ns(), **kw)
+    g = cases.tensor_sphere(cases.product_ns(), gpu, **kw)
+    for _ in range(steps // o.stride):
+        o.chunk()
+        g.chunk()
+    mask = cases.real_mask(o.shape, o.periodic) if bloch else None
+    _compare(o, g, f"vertex tensor sphere ({'Bloch' if bloch else 'real'}) {steps} steps", mask)
+    eo, _ = o.monitors[0].results()
+    eg, _ = g.monitors[0].results()
+    d = cases.rel_l2(eg, eo)
+    print(f"  DFT rel L2 {d:.2e}")
+    assert d < DFT_TOL

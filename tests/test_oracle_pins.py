@@ -374,3 +374,66 @@ def test_offdiagonal_coupling_is_symmetric():
     assert len(K) > 100
     for (t, s), v in K.items():
         assert (s, t) in K and abs(K[(s, t)] - v) < 1e-15
+
+
+# --- conformal vertex tensor (engine.py:839-923; the reference's own test: test_fdtd.py:254-284) --------------
+
+def test_vertex_tensor_map_is_symmetric_and_bounded():
+    """The D -> E map of the vertex tensor is a sum of congruences: symmetric, every Rayleigh quotient between
+    1/eps_inf,max and 1/eps_min (the reference's test_fdtd_metal_vertex_tensor_is_symmetric_and_bounded)."""
+    from oracle import media as omed
+    from oracle.tensor import TensorGroup, vertex_tensor
+    media = [omed.Medium(1.77, ()), omed.Medium(9.5, (omed.Pole(0.00208, 0.0, 1.8e-4),))]
+    shape, c, r = (24, 24, 24), np.array([0.3, 0.1, -0.2]), 6.4
+    tm = vertex_tensor(lambda p: (np.linalg.norm(p - c, axis=-1) < r).astype(int), media, shape, np.full(3, -12.0),
+                       1.0, np.full(3, 2), np.full(3, 22))
+    assert tm is not None and len(tm.normal) > 100
+    eng = fdtd.Engine(shape=shape, dx=1.0, dt=0.5, inv_eps=np.ones((3,) + shape), groups=[], pml=None, sources=[],
+                     monitors=[], dft_stride=1)
+    idx = tm.idx
+
+    def apply(x):
+        g = TensorGroup(eng, tm)
+        eng.E_flat[idx] = x
+        g.step(eng)
+        return np.array(eng.E_flat[idx])
+
+    rng = np.random.default_rng(0)
+    x, y = rng.standard_normal((2, len(idx)))
+    mx, my = apply(x), apply(y)
+    assert abs(x @ my - y @ mx) < 1e-12 * abs(x @ mx)
+    for z in rng.standard_normal((5, len(idx))):
+        q = z @ apply(z) / (z @ z)
+        assert 1 / 9.5 <= q <= 1 / 1.77
+    # planted mistake: taking the slot's own D for sigma (dropping the pair mean) breaks the symmetry
+    g = TensorGroup(eng, tm)
+    g.t_vc = g.t_vc * 0
+    eng.E_flat[idx] = x
+    g.step(eng)
+    bx = np.array(eng.E_flat[idx])
+    g = TensorGroup(eng, tm)
+    g.t_vc = g.t_vc * 0
+    eng.E_flat[idx] = y
+    g.step(eng)
+    by = np.array(eng.E_flat[idx])
+    assert abs(x @ by - y @ bx) > 1e-6 * abs(x @ bx)
+
+
+def test_vertex_tensor_in_one_medium_is_the_plain_update():
+    """With every vertex and plain slot in one non-dispersive medium the map is D / eps for every node."""
+    from oracle import media as omed
+    from oracle.tensor import TensorGroup, vertex_tensor
+    media = [omed.Medium(1.77, ()), omed.Medium(9.5, (omed.Pole(0.00208, 0.0, 1.8e-4),))]
+    shape, c, r = (20, 20, 20), np.zeros(3), 5.2
+    tm = vertex_tensor(lambda p: (np.linalg.norm(p - c, axis=-1) < r).astype(int), media, shape, np.full(3, -10.0),
+                       1.0, np.full(3, 2), np.full(3, 18))
+    tm.frac[:] = 0.0
+    tm.frac[0] = 1.0
+    tm.slot_medium[:] = 0
+    eng = fdtd.Engine(shape=shape, dx=1.0, dt=0.5, inv_eps=np.ones((3,) + shape), groups=[], pml=None, sources=[],
+                     monitors=[], dft_stride=1)
+    g = TensorGroup(eng, tm)
+    x = np.random.default_rng(1).standard_normal(len(tm.idx))
+    eng.E_flat[tm.idx] = x
+    g.step(eng)
+    assert np.max(np.abs(eng.E_flat[tm.idx] - x / 1.77)) < 1e-14
