@@ -54,47 +54,121 @@ class RunOptions:
     max_time_fs: float = 2000.0
 
 
+class DecayTracker:
+    """The per-chunk decisions of run_until_decayed (setup.py:722-773) for one run: divergence, the decay level
+    over a sliding window of chunk energies, the stop test, the time-cap warning."""
+
+    def __init__(self, opts, *, max_steps: int, min_steps: int, window: int):
+        self.opts = opts
+        self.max_steps, self.min_steps = int(max_steps), int(min_steps)
+        self.win = deque(maxlen=max(3, int(window)))
+        self.peak = self.peak_ref = self.grown_ref = 0.0
+        self.level = float("nan")
+        self.ref_steps = min(self.min_steps, self.max_steps // 2)
+
+    def update(self, steps: int, e: float, e_ref: float) -> bool:
+        """Record the energies after `steps` steps; True when the run is over (decayed or at its cap). Raises
+        FloatingPointError on non-finite or runaway energy."""
+        finite = np.isfinite(e) and np.isfinite(e_ref)
+        if not finite or (steps > self.ref_steps and e > DIVERGED_GROWTH * self.grown_ref > 0):
+            why = "non-finite fields" if not finite else \
+                f"monitor energy grew to {e / self.grown_ref:.0e} times its peak while the pulse passed"
+            raise FloatingPointError(
+                f"FDTD diverged after {steps} steps ({why}); the results would be meaningless. "
+                "Use a finer mesh (smaller spacing_nm or larger cells_per_wavelength), lower courant, "
+                "or give lossy/dispersive materials as Drude/DrudeLorentz")
+        self.win.append(e)
+        self.peak, self.peak_ref = max(self.peak, e), max(self.peak_ref, e_ref)
+        if steps <= self.ref_steps:
+            self.grown_ref = self.peak
+        floor = max(self.peak, 1e-6 * self.peak_ref)
+        self.level = max(self.win) / floor if floor > 0 else float("nan")
+        if not self.opts.run_time_fs and steps >= self.min_steps and len(self.win) == self.win.maxlen and \
+                self.level < self.opts.decay_by:
+            return True
+        return steps >= self.max_steps
+
+    def cap_warning(self, steps: int):
+        if steps >= self.max_steps and not self.opts.run_time_fs and not (self.level < self.opts.decay_by):
+            return (f"fields had decayed only to {self.level:.1e} of peak at the time cap "
+                    f"({self.opts.max_time_fs} fs); raise max_time_fs or decay_by")
+        return None
+
+
+def _energies(en) -> tuple[float, float]:
+    en = en.detach().cpu().numpy() if hasattr(en, "detach") else np.asarray(en)
+    return float(en[0]), float(en[1])
+
+
 def run_until_decayed(eng, opts, *, max_steps: int, min_steps: int, window: int, warnings: list):
     """Chunks until the energy of the monitors with energy=1 has fallen below opts.decay_by of its peak over a
     window of `window` chunks (after min_steps), or until max_steps / a fixed run time (setup.py:722-773).
     Monitors with energy=2 give a reference: the peak is taken as at least 1e-6 of theirs. Raises
     FloatingPointError on non-finite or runaway energy. Returns (steps, decay_level, used_graphs)."""
     run, graphs = eng.make_runner(opts.cuda_graphs)
-    win = deque(maxlen=max(3, int(window)))
-    peak, peak_ref, grown_ref, level, steps = 0.0, 0.0, 0.0, float("nan"), 0
-    ref_steps = min(min_steps, max_steps // 2)
+    tr = DecayTracker(opts, max_steps=max_steps, min_steps=min_steps, window=window)
+    steps = 0
     try:
-        while steps < max_steps:
+        while True:
             run()
             steps += eng.stride
-            en = eng.energy
-            en = en.detach().cpu().numpy() if hasattr(en, "detach") else np.asarray(en)
-            e, e_ref = (float(v) for v in en)  # one sync per chunk
-            finite = np.isfinite(e) and np.isfinite(e_ref)
-            if not finite or (steps > ref_steps and e > DIVERGED_GROWTH * grown_ref > 0):
-                why = "non-finite fields" if not finite else \
-                    f"monitor energy grew to {e / grown_ref:.0e} times its peak while the pulse passed"
-                raise FloatingPointError(
-                    f"FDTD diverged after {steps} steps ({why}); the results would be meaningless. "
-                    "Use a finer mesh (smaller spacing_nm or larger cells_per_wavelength), lower courant, "
-                    "or give lossy/dispersive materials as Drude/DrudeLorentz")
-            win.append(e)
-            peak, peak_ref = max(peak, e), max(peak_ref, e_ref)
+            e, e_ref = _energies(eng.energy)  # one sync per chunk
+            done = tr.update(steps, e, e_ref)
             report("fdtd_steps", 0.95 * steps / max_steps if opts.run_time_fs else None, steps=steps,
                    max_steps=max_steps)
-            if steps <= ref_steps:
-                grown_ref = peak
-            floor = max(peak, 1e-6 * peak_ref)
-            level = max(win) / floor if floor > 0 else float("nan")
-            if not opts.run_time_fs and steps >= min_steps and len(win) == win.maxlen and level < opts.decay_by:
+            if done:
                 break
-        else:
-            if not opts.run_time_fs:
-                warnings.append(f"fields had decayed only to {level:.1e} of peak at the time cap "
-                                f"({opts.max_time_fs} fs); raise max_time_fs or decay_by")
+        w = tr.cap_warning(steps)
+        if w:
+            warnings.append(w)
     finally:
         eng.release()
-    return steps, level, graphs
+    return steps, tr.level, graphs
+
+
+def run_batch_until_decayed(engines, opts, *, max_steps, min_steps, windows, warnings: list, schedule=None,
+                            tune: bool = False):
+    """run_until_decayed for many runs stepped as one Batch (one DFT stride): each run stops at its own decay
+    point or cap, exactly as when run alone -- a finished run leaves the batch (the batch is rebuilt over the
+    others), so its monitors hold what the sequential loop (periodic.py:503-519) would give. One readback of every
+    member's energy per chunk. Returns [(steps, decay_level, used_graphs)] in the order of `engines`."""
+    import torch
+
+    from .batch import Batch
+
+    n = len(engines)
+    trackers = [DecayTracker(opts, max_steps=max_steps[q], min_steps=min_steps[q], window=windows[q])
+                for q in range(n)]
+    steps = [0] * n
+    graphs = [False] * n
+    active = list(range(n))
+    stride = engines[0].stride
+    while active:
+        b = Batch([engines[q] for q in active], schedule=schedule)
+        if tune:
+            b.autotune()
+            schedule = b.schedule
+        run, used = b.make_runner(opts.cuda_graphs)
+        finished = []
+        try:
+            while not finished:
+                run()
+                en = torch.stack([engines[q].energy for q in active]).cpu().numpy()  # one sync per chunk
+                for idx, q in enumerate(active):
+                    steps[q] += stride
+                    graphs[q] = used
+                    if trackers[q].update(steps[q], float(en[idx, 0]), float(en[idx, 1])):
+                        finished.append(q)
+                report("fdtd_steps", None, runs_active=len(active), steps=max(steps))
+        finally:
+            b.release()
+            b.close()
+        for q in finished:
+            w = trackers[q].cap_warning(steps[q])
+            if w:
+                warnings.append(f"run {q}: {w}")
+        active = [q for q in active if q not in finished]
+    return [(steps[q], trackers[q].level, graphs[q]) for q in range(n)]
 
 
 def flux_points(lo, hi, shape, dx: float, open_lo=()):

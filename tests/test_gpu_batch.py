@@ -139,3 +139,25 @@ def test_batch_refuses_what_it_cannot_run(gpu):
     with pytest.raises(ValueError):
         Batch([engs[2], odd])  # different DFT strides
     np.testing.assert_equal(len(engs), 8)
+
+
+def test_batch_run_control_matches_sequential_runs(gpu):
+    """run_batch_until_decayed stops each member where run_until_decayed stops it alone (staggered caps and
+    minimum lengths), with bit-identical monitors."""
+    from lightsmith_amd.fdtd.run import RunOptions, run_batch_until_decayed, run_until_decayed
+    opts = RunOptions(decay_by=1e-2)
+    n = 4
+    max_steps = [200 + 40 * q for q in range(n)]
+    min_steps = [60 + 20 * q for q in range(n)]
+    windows = [3] * n
+    runs = workloads.oblique_sweep_scene(**dict(SMALL, n_inst=n))
+    alone = _sweep(gpu, runs, 10, 400)
+    seq = [run_until_decayed(e, opts, max_steps=max_steps[q], min_steps=min_steps[q], window=windows[q],
+                             warnings=[]) for q, e in enumerate(alone)]
+    members = _sweep(gpu, runs, 10, 400)
+    bat = run_batch_until_decayed(members, opts, max_steps=max_steps, min_steps=min_steps, windows=windows,
+                                  warnings=[])
+    print("steps alone", [s[0] for s in seq], "batched", [s[0] for s in bat])
+    assert [s[0] for s in seq] == [s[0] for s in bat]
+    torch.cuda.synchronize()
+    _assert_same(alone, members)
