@@ -93,7 +93,8 @@ int lsf_set_ade(lsf_engine* eng, const int32_t* dev_chunk_base, void* dev_P, int
 /* Gather-add rows: F[target[r]] += sum_k w[r, k] * src[src_idx[r, k]] (field 0 = E, 1 = H).
  * Replaces the off-diagonal rows (engine.py:653-655), the 1D-fed TFSF corrections (tfsf.py:195-204)
  * and the Bloch plane corrections (tfsf.py:274-286). dev_scale (or NULL = 1) scales each row's sum: 1/eps_inf
- * for a pure D-form E node, where the reference's correction enters through D before the group update. */
+ * for a pure D-form E node, where the reference's correction enters through D before the group update.
+ * dev_src NULL reads the engine's own H (its current buffer, see lsf_set_fused). */
 int lsf_add_gather(lsf_engine* eng, int phase, int field, int64_t n_rows, int width, const int64_t* dev_target,
                    const void* dev_src, const int64_t* dev_src_idx, const void* dev_w, const float* dev_scale);
 
@@ -237,6 +238,16 @@ int lsf_batch_chunk(lsf_batch* batch, int n_chunks, int stride, void* stream);
 int lsf_batch_graph_capture(lsf_batch* batch, int stride, void* stream);
 int lsf_batch_graph_replay(lsf_batch* batch, int n, void* stream);
 int lsf_batch_graph_reset(lsf_batch* batch);
+
+/* Single-pass stepping (no reference counterpart: a launch choice). dev_E2, dev_H2: second field buffers (zeroed,
+ * like dev_E/dev_H); dev_psiH2[axis]: a second set of the H-side psi arrays of each axis with CPML (null where
+ * none), laid out like the first, whose two component arrays must be contiguous (the second component's array
+ * right after the first's). Where the engine allows it -- real fields, no periodic axes, one row per lane and 64 x
+ * cells per row, no operations between the H and the E update -- each step is then one kernel that reads one buffer
+ * set and writes the other; the E-phase operations, monitors and graphs follow the current set. lsf_parity gives
+ * the current set (0: dev_E/dev_H, 1: dev_E2/dev_H2). Results are bit-identical to the two-kernel step. */
+int lsf_set_fused(lsf_engine* eng, void* dev_E2, void* dev_H2, void* const* dev_psiH2);
+int lsf_parity(lsf_engine* eng);
 
 /* Message of the last failing call on this thread. */
 const char* lsf_last_error(void);

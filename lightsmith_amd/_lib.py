@@ -54,6 +54,8 @@ _SIGNATURES = {
     "lsf_graph_reset": [c_vp],
     "lsf_add_tensor_group": [c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp,
                              c_vp, c_vp],
+    "lsf_set_fused": [c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)],
+    "lsf_parity": [c_vp],
     "lsf_set_adjoint": [c_vp, c_int],
     "lsf_add_tape": [c_vp, c_i64, c_vp, c_vp, c_i64, c_vp],
     "lsf_add_adjoint_gradient": [c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp],

@@ -3,6 +3,7 @@
 Translation units (compiled by hipcc for gfx950 in parallel, then linked into lightsmith_amd/_lsf.so):
   csrc/lsf.hip     host API, batches, sparse per-step kernels
   csrc/lsf_k_adj.hip the adjoint bulk kernels
+  csrc/lsf_k_fused.hip the single-pass step kernel
   csrc/lsf_k_h.hip the bulk H kernel instantiations, once per field type (float, complex64)
   csrc/lsf_k_e.hip the bulk E kernel instantiations, once per field type, material mode (dense, indexed)
                    and periodicity
@@ -26,7 +27,7 @@ OUT = os.path.join(HERE, "_lsf.so")
 ARCH = "gfx950"
 
 # (object name, source, extra defines)
-UNITS = [("lsf", "lsf.hip", []), ("k_adj", "lsf_k_adj.hip", [])] + [
+UNITS = [("lsf", "lsf.hip", []), ("k_adj", "lsf_k_adj.hip", []), ("k_fused", "lsf_k_fused.hip", [])] + [
     (f"k_h_{t}", "lsf_k_h.hip", [f"-DLSF_H_CPLX={1 if t == 'c' else 0}", f"-DLSF_H_ENTRY=launch_update_h_{t}"])
     for t in ("f", "c")
 ] + [

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "lsf.h"
+#include "lsf_fused.h"
 #include "lsf_launch.h"
 #include "lsf_sparse.h"
 
@@ -134,6 +135,17 @@ struct lsf_engine {
   bool launched = false;               // kernels may be reading dev_args
   int held = 0;                        // batches holding this engine (its configuration is then frozen)
   int adjoint = 0;                     // steps the adjoint (transposed, time-reversed) update
+  // single-pass stepping: both field buffers and H-side psi sets, the parity of the current ones
+  void* Ebuf[2] = {nullptr, nullptr};
+  void* Hbuf[2] = {nullptr, nullptr};
+  void* psiH[2][3][3] = {};            // [parity][axis][component]
+  int parity = 0;
+  bool fused_set = false;              // second buffers given (lsf_set_fused)
+  bool use_fused = false;              // the single-pass kernel steps this engine (decided at sync_args)
+  lsf::FusedArgs* dev_fused = nullptr; // library scratch: [parity]
+  hipGraph_t graph2 = nullptr;         // the chunk graph starting at the other parity (odd strides)
+  hipGraphExec_t exec2 = nullptr;
+  int graph_parity = 0, graph_stride = 0;
 };
 
 namespace {
@@ -187,6 +199,81 @@ int pole_regs(const lsf_engine* e) {
   return e->max_poles <= 2 ? 2 : (e->max_poles == 3 ? 3 : 4);
 }
 
+void set_parity(lsf_engine* e, int p) {
+  e->parity = p;
+  e->E = e->Ebuf[p];
+  e->H = e->Hbuf[p];
+  for (int ax = 0; ax < 3; ++ax)
+    for (int c = 0; c < 3; ++c) e->pml[1][ax].psi[c] = e->psiH[p][ax][c];
+}
+
+bool has_h_phase_ops(const lsf_engine* e) {
+  for (const Op& o : e->ops)
+    if (o.phase <= LSF_PH_ADVANCE_H) return true;
+  return false;
+}
+
+// elements from the first to the second psi array of (field, axis) when they are contiguous, else -1
+long long psi_gap(const lsf_engine* e, int field, int ax, int parity_set) {
+  const int c1 = ax == 0 ? 1 : 0, c2 = ax == 2 ? 1 : 2;
+  void* const* ps = field == 0 ? e->pml[0][ax].psi : e->psiH[parity_set][ax];
+  const long long ns = e->pml[field][ax].ns;
+  long long size = ns;
+  for (int a = 0; a < 3; ++a)
+    if (a != ax) size *= e->g.n[a];
+  if (!ps[c1] || !ps[c2]) return -1;
+  return ((const char*)ps[c2] - (const char*)ps[c1]) == size * (long long)sizeof(float) ? size : -1;
+}
+
+bool fused_eligible(const lsf_engine* e) {
+  if (!e->fused_set || e->cplx || any_periodic(e) || e->adjoint || e->ry != 1 || e->xw != 64 || has_h_phase_ops(e))
+    return false;
+  for (int f = 0; f < 2; ++f)
+    for (int ax = 0; ax < 3; ++ax)
+      if (e->pml[f][ax].prof)
+        for (int p = 0; p < (f == 1 ? 2 : 1); ++p)
+          if (psi_gap(e, f, ax, p) < 0) return false;
+  return true;
+}
+
+lsf::FusedArgs fused_args(const lsf_engine* e, int p) {
+  lsf::FusedArgs a{};
+  a.g = e->g;
+  a.E_in = (const float*)e->Ebuf[p];
+  a.H_in = (const float*)e->Hbuf[p];
+  a.E_out = (float*)e->Ebuf[p ^ 1];
+  a.H_out = (float*)e->Hbuf[p ^ 1];
+  a.cbd = e->cbd;
+  a.mid = e->mid;
+  a.mtab = e->mtab;
+  for (int ax = 0; ax < 3; ++ax) {
+    const int c1 = ax == 0 ? 1 : 0;
+    a.prof[0][ax] = e->pml[0][ax].prof;
+    a.prof[1][ax] = e->pml[1][ax].prof;
+    a.psiE[ax] = (float*)e->pml[0][ax].psi[c1];
+    a.psiHi[ax] = (const float*)e->psiH[p][ax][c1];
+    a.psiHo[ax] = (float*)e->psiH[p ^ 1][ax][c1];
+    for (int f = 0; f < 2; ++f) {
+      a.psz[f][ax] = e->pml[f][ax].prof ? psi_gap(e, f, ax, 0) : 0;
+      a.ns[f][ax] = e->pml[f][ax].ns;
+      a.has_k[f][ax] = e->pml[f][ax].has_k;
+    }
+  }
+  a.chunk_base = e->chunk_base;
+  a.P = e->P;
+  a.nchunk = (e->g.n[0] + 31) / 32;
+  a.max_poles = e->max_poles;
+  a.media_stride = 2 + 3 * e->max_poles;
+  a.n_media_floats = e->n_ade > 0 ? e->n_media * (2 + 3 * e->max_poles) : 0;
+  a.media = e->media;
+  a.counter = e->counter;
+  a.tick = e->tick;
+  a.tiles_x = (e->g.n[0] + 63) / 64;
+  a.tiles_y = (e->g.n[1] + e->waves - 1) / e->waves;
+  a.zc = e->zc;
+  return a;
+}
+
 // Upload the engine's kernel arguments (and its sub-engines') when its configuration changed. Not called
 // during graph capture.
 int sync_args(lsf_engine* e) {
@@ -206,8 +293,28 @@ int sync_args(lsf_engine* e) {
   if (rc) return rc;
   e->nb = h[0].nb;
   e->nbp = h[0].nbp;
+  e->use_fused = fused_eligible(e);
+  if (e->use_fused) {
+    if (!e->dev_fused) rc = hip_check(hipMalloc(&e->dev_fused, 2 * sizeof(lsf::FusedArgs)), "hipMalloc fused arguments");
+    if (rc) return rc;
+    const lsf::FusedArgs f[2] = {fused_args(e, 0), fused_args(e, 1)};
+    rc = hip_check(hipMemcpy(e->dev_fused, f, sizeof f, hipMemcpyHostToDevice), "fused argument upload");
+    if (rc) return rc;
+  }
   e->dirty = false;
   return LSF_OK;
+}
+
+void launch_fused_step(lsf_engine* e, hipStream_t s) {
+  lsf::FusedLaunch L{};
+  L.args = e->dev_fused + e->parity;
+  L.sc = lsf::Sched{nullptr, nullptr, e->nb, e->nbp};
+  L.waves = e->waves;
+  L.mat = e->mid ? 1 : 0;
+  L.np = pole_regs(e);
+  lsf::launch_fused(L, s);
+  e->launched = true;
+  set_parity(e, e->parity ^ 1);
 }
 
 lsf::KLaunch own_launch(lsf_engine* e, int field) {
@@ -244,7 +351,8 @@ void run_phase(lsf_engine* e, int phase, hipStream_t s);
 template <class T>
 lsf::GatherJob<T> gather_job(lsf_engine* e, const Op& o) {
   T* F = (T*)(o.field == 0 ? e->E : e->H);
-  return lsf::GatherJob<T>{o.n, o.width, o.target, F, (const T*)o.src, o.sidx, (const T*)o.w, o.scale};
+  const T* src = o.src ? (const T*)o.src : (const T*)e->H;  // null: the engine's own (current) H
+  return lsf::GatherJob<T>{o.n, o.width, o.target, F, src, o.sidx, (const T*)o.w, o.scale};
 }
 
 template <class T>
@@ -339,6 +447,11 @@ void run_phase(lsf_engine* e, int phase, hipStream_t s) {
 }
 
 void one_step(lsf_engine* e, hipStream_t s) {
+  if (e->use_fused) {
+    launch_fused_step(e, s);
+    for (int ph = LSF_PH_OFFDIAG; ph <= LSF_PH_ADVANCE_E; ++ph) run_phase(e, ph, s);
+    return;
+  }
   launch_h(e, s);
   run_phase(e, LSF_PH_CORRECT_H, s);
   run_phase(e, LSF_PH_ADVANCE_H, s);
@@ -468,6 +581,8 @@ int lsf_create(const lsf_grid* gd, lsf_engine** out) {
   e->dt = gd->dt;
   e->E = gd->dev_E;
   e->H = gd->dev_H;
+  e->Ebuf[0] = e->E;
+  e->Hbuf[0] = e->H;
   e->counter = (long long*)gd->dev_counter;
   e->tick = gd->tick ? 1 : 0;
   *out = e;
@@ -481,6 +596,9 @@ void lsf_destroy(lsf_engine* e) {
   if (e->partial) (void)hipFree(e->partial);
   if (e->mon_tab) (void)hipFree(e->mon_tab);
   if (e->dev_args) (void)hipFree(e->dev_args);
+  if (e->dev_fused) (void)hipFree(e->dev_fused);
+  if (e->exec2) (void)hipGraphExecDestroy(e->exec2);
+  if (e->graph2) (void)hipGraphDestroy(e->graph2);
   delete e;
 }
 
@@ -516,6 +634,8 @@ int lsf_set_pml(lsf_engine* e, int field, int axis, const void* prof, int n_slab
   ax.has_k = has_k ? 1 : 0;
   for (int c = 0; c < 3; ++c)
     if (c != axis && prof && !ax.psi[c]) return fail(LSF_EINVAL, "missing psi array");
+  if (field == 1)
+    for (int c = 0; c < 3; ++c) e->psiH[e->parity][axis][c] = ax.psi[c];
   return LSF_OK;
 }
 
@@ -540,7 +660,7 @@ int lsf_set_ade(lsf_engine* e, const int32_t* chunk_base, void* P, int64_t n, in
 int lsf_add_gather(lsf_engine* e, int phase, int field, int64_t n, int width, const int64_t* target, const void* src,
                    const int64_t* sidx, const void* w, const float* scale) {
   if (valid(e)) return LSF_EINVAL;
-  if (n > 0 && (!target || !src || !sidx || !w || width < 1)) return fail(LSF_EINVAL, "missing gather arrays");
+  if (n > 0 && (!target || !sidx || !w || width < 1)) return fail(LSF_EINVAL, "missing gather arrays");
   Op o;
   o.kind = OP_GATHER;
   o.phase = phase;
@@ -902,12 +1022,13 @@ int lsf_step_timed(lsf_engine* e, int n, void* stream, double* ms) {
   if (!rc) {
     for (int q = 0; q < n; ++q) {
       (void)hipEventRecord(ev[4 * q + 0], s);
-      launch_h(e, s);
+      if (!e->use_fused) launch_h(e, s);
       (void)hipEventRecord(ev[4 * q + 1], s);
       run_phase(e, LSF_PH_CORRECT_H, s);
       run_phase(e, LSF_PH_ADVANCE_H, s);
       (void)hipEventRecord(ev[4 * q + 2], s);
-      launch_e(e, s);
+      if (e->use_fused) launch_fused_step(e, s);  // the single pass is reported as the E time
+      else launch_e(e, s);
       (void)hipEventRecord(ev[4 * q + 3], s);
       for (int ph = LSF_PH_OFFDIAG; ph <= LSF_PH_ADVANCE_E; ++ph) run_phase(e, ph, s);
     }
@@ -950,13 +1071,10 @@ int lsf_chunk(lsf_engine* e, int n_chunks, int stride, void* stream) {
   return hip_check(hipGetLastError(), "chunk launch");
 }
 
-int lsf_graph_capture(lsf_engine* e, int stride, void* stream) {
-  if (valid(e)) return LSF_EINVAL;
-  if (!stream) return fail(LSF_EINVAL, "graph capture needs a non-default stream");
-  if (e->exec) return fail(LSF_ESTATE, "a graph is already captured");
-  if (!e->cbd && !e->mid) return fail(LSF_ESTATE, "no material set");
-  if (int rc1 = sync_args(e)) return rc1;
-  hipStream_t s = (hipStream_t)stream;
+}  // extern "C"
+
+namespace {
+int capture_chunk(lsf_engine* e, int stride, hipStream_t s, hipGraph_t* graph, hipGraphExec_t* exec) {
   int rc = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
   if (rc) return rc;
   for (int q = 0; q < stride; ++q) one_step(e, s);
@@ -964,13 +1082,33 @@ int lsf_graph_capture(lsf_engine* e, int stride, void* stream) {
   hipGraph_t g = nullptr;
   rc = hip_check(hipStreamEndCapture(s, &g), "end capture");
   if (rc) return rc;
-  e->graph = g;
-  rc = hip_check(hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0), "graph instantiate");
+  *graph = g;
+  rc = hip_check(hipGraphInstantiate(exec, g, nullptr, nullptr, 0), "graph instantiate");
   if (rc) {
     (void)hipGraphDestroy(g);
-    e->graph = nullptr;
-    e->exec = nullptr;
+    *graph = nullptr;
+    *exec = nullptr;
   }
+  return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int lsf_graph_capture(lsf_engine* e, int stride, void* stream) {
+  if (valid(e)) return LSF_EINVAL;
+  if (!stream) return fail(LSF_EINVAL, "graph capture needs a non-default stream");
+  if (e->exec) return fail(LSF_ESTATE, "a graph is already captured");
+  if (!e->cbd && !e->mid) return fail(LSF_ESTATE, "no material set");
+  if (stride < 1) return fail(LSF_EINVAL, "stride must be >= 1");
+  if (int rc1 = sync_args(e)) return rc1;
+  hipStream_t s = (hipStream_t)stream;
+  const int p0 = e->parity;
+  int rc = capture_chunk(e, stride, s, &e->graph, &e->exec);  // the capture advances the host parity only
+  if (!rc && e->use_fused && (stride & 1)) rc = capture_chunk(e, stride, s, &e->graph2, &e->exec2);
+  set_parity(e, p0);
+  e->graph_parity = p0;
+  e->graph_stride = stride;
   return rc;
 }
 
@@ -978,9 +1116,34 @@ int lsf_graph_replay(lsf_engine* e, int n, void* stream) {
   if (valid(e)) return LSF_EINVAL;
   if (!e->exec) return fail(LSF_ESTATE, "no captured graph");
   for (int q = 0; q < n; ++q) {
-    int rc = hip_check(hipGraphLaunch(e->exec, (hipStream_t)stream), "graph launch");
+    hipGraphExec_t x = (e->exec2 && e->parity != e->graph_parity) ? e->exec2 : e->exec;
+    const int rc = hip_check(hipGraphLaunch(x, (hipStream_t)stream), "graph launch");
     if (rc) return rc;
+    if (e->use_fused && (e->graph_stride & 1)) set_parity(e, e->parity ^ 1);
   }
+  return LSF_OK;
+}
+
+int lsf_parity(lsf_engine* e) { return e ? e->parity : 0; }
+
+int lsf_set_fused(lsf_engine* e, void* E2, void* H2, void* const* psiH2) {
+  if (int rc0 = configurable(e)) return rc0;
+  if (!E2 || !H2) return fail(LSF_EINVAL, "missing second field buffers");
+  if (e->exec) return fail(LSF_ESTATE, "the engine has a captured graph; reset it first");
+  if (e->parity != 0) return fail(LSF_ESTATE, "second buffers are set before stepping");
+  e->Ebuf[1] = E2;
+  e->Hbuf[1] = H2;
+  for (int ax = 0; ax < 3; ++ax) {
+    for (int c = 0; c < 3; ++c) e->psiH[1][ax][c] = nullptr;
+    if (!e->pml[1][ax].prof) continue;
+    if (!psiH2 || !psiH2[ax]) return fail(LSF_EINVAL, "missing second H-side psi set");
+    const long long gap = psi_gap(e, 1, ax, 0);
+    if (gap < 0) return fail(LSF_EINVAL, "the H-side psi arrays of an axis must be contiguous");
+    const int c1 = ax == 0 ? 1 : 0, c2 = ax == 2 ? 1 : 2;
+    e->psiH[1][ax][c1] = psiH2[ax];
+    e->psiH[1][ax][c2] = (float*)psiH2[ax] + gap;
+  }
+  e->fused_set = true;
   return LSF_OK;
 }
 
@@ -990,8 +1153,12 @@ int lsf_graph_reset(lsf_engine* e) {
   if (e->exec || e->graph) rc = hip_check(hipDeviceSynchronize(), "synchronize before graph reset");
   if (e->exec) (void)hipGraphExecDestroy(e->exec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
+  if (e->exec2) (void)hipGraphExecDestroy(e->exec2);
+  if (e->graph2) (void)hipGraphDestroy(e->graph2);
   e->exec = nullptr;
   e->graph = nullptr;
+  e->exec2 = nullptr;
+  e->graph2 = nullptr;
   return rc;
 }
 

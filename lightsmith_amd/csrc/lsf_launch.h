@@ -39,6 +39,16 @@ void launch_update_h_c(const KLaunch& L, hipStream_t s);
 LSF_E_LAUNCHERS(LSF_DECLARE_E)
 #undef LSF_DECLARE_E
 
+// the single-pass step (real fields, no periodic axes, one row per wavefront, 64 x cells): lsf_k_fused.hip
+struct FusedArgs;
+struct FusedLaunch {
+  const FusedArgs* args;  // device, one entry
+  Sched sc;
+  int waves;
+  int mat, np;
+};
+void launch_fused(const FusedLaunch& L, hipStream_t s);
+
 // adjoint kernels (real fields, no periodic axes, no D-form nodes; one tile per block): lsf_k_adj.hip
 void launch_adjoint_h(const KLaunch& L, hipStream_t s);
 void launch_adjoint_e(const KLaunch& L, hipStream_t s);

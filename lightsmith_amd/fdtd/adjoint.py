@@ -76,7 +76,7 @@ class AdjointProblem:
             if design.lo[a] < max(1, cells[0]) or design.hi[a] > self.shape[a] - max(1, cells[1]) - 1:
                 raise ValueError("the design region must lie inside the CPML-free interior")
         self.fwd = Engine(device, shape=shape, dx=dx, dt=dt, single=True, inv_eps=inv_eps, pml=pml, sources=sources,
-                          monitors=[monitor], dft_stride=self.stride, material="dense")
+                          monitors=[monitor], dft_stride=self.stride, material="dense", fused=False)
         self.monitor = monitor
         dev = self.fwd._dev
         self._dev = dev
@@ -96,7 +96,7 @@ class AdjointProblem:
 
         # the adjoint engine: same grid, material (shared) and CPML profiles; its own fields and chi arrays
         self.adj = Engine(device, shape=shape, dx=dx, dt=dt, single=True, inv_eps=inv_eps, pml=pml, sources=[],
-                          monitors=[], dft_stride=1, material="dense")
+                          monitors=[], dft_stride=1, material="dense", fused=False)
         check(lib.lsf_set_material_dense(self.adj._h, ptr(self.fwd._mat)), "lsf_set_material_dense")
         check(lib.lsf_set_adjoint(self.adj._h, 1), "lsf_set_adjoint")
         self.g = torch.zeros(self.n_nodes, dtype=torch.float64, device=dev)

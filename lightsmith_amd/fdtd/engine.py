@@ -260,11 +260,15 @@ class Engine:
     # launch tiling (rows per lane, wavefronts per block, x cells per wavefront row, z planes per block);
     # see lsf_set_tiling
     tiling = (1, 8, 64, 16)
+    # default of the `fused` argument: step eligible engines (real fields, no periodic axes, one row per lane,
+    # 64-cell rows, no H-phase operations) with the single-pass kernel over a second buffer set
+    single_pass = False
 
     def __init__(self, device=None, *, shape, dx, dt: float, single: bool = True, inv_eps: np.ndarray,
                  groups=(), pml: PMLSpec | None = None, sources=(), monitors=(), dft_stride: int = 1,
                  periodic=(False, False, False), bloch=(0.0, 0.0, 0.0), offdiag: OffDiagonal | None = None,
-                 time_monitors=(), mirror=(0, 0, 0), widths=None, material: str = "auto", _sub: bool = False,
+                 time_monitors=(), mirror=(0, 0, 0), widths=None, material: str = "auto", fused: bool | None = None,
+                 _sub: bool = False,
                  _counter=None):
         if not single:
             raise ValueError("the HIP engine steps float32 fields (single=True); double precision runs on the CPU")
@@ -297,10 +301,16 @@ class Engine:
         self._state = []  # device arrays the stepping writes (psi, pole and group state, source lines)
         self._subs = []  # sub-engines stepped inside this one
         self._held = 0  # batches stepping this engine
-        self.E_flat = torch.zeros(3 * self.N, dtype=self.fdt, device=self._dev)
-        self.H_flat = torch.zeros(3 * self.N, dtype=self.fdt, device=self._dev)
-        self.E = self.E_flat.view(3, n2, n1, n0).permute(0, 3, 2, 1)
-        self.H = self.H_flat.view(3, n2, n1, n0).permute(0, 3, 2, 1)
+        # single-pass stepping (lsf_set_fused): a second buffer set for real non-periodic grids
+        self._two_sets = (fused if fused is not None else Engine.single_pass) and not self.complex and not any(self.periodic) \
+            and self.tiling[0] == 1 and self.tiling[2] == 64 and not _sub
+        nsets = 2 if self._two_sets else 1
+        self._E_bufs = [torch.zeros(3 * self.N, dtype=self.fdt, device=self._dev) for _ in range(nsets)]
+        self._H_bufs = [torch.zeros(3 * self.N, dtype=self.fdt, device=self._dev) for _ in range(nsets)]
+        view = lambda t: t.view(3, n2, n1, n0).permute(0, 3, 2, 1)  # noqa: E731
+        self._E_views = [view(t) for t in self._E_bufs]
+        self._H_views = [view(t) for t in self._H_bufs]
+        self._psiH_blocks = []  # (block, size): the H-side psi arrays of an axis, both sets
         self._counter = _counter if _counter is not None else torch.zeros(2, dtype=torch.int64, device=self._dev)
         self.n = self._counter[0:1]
         self.n_host = 0
@@ -318,7 +328,7 @@ class Engine:
         for a in range(3):
             kl = self.bloch[a] * (self.shape[a] - 2) * self.spacing[a] if self.periodic[a] else 0.0
             g.bloch_phase[a][0], g.bloch_phase[a][1] = np.cos(kl), np.sin(kl)
-        g.dev_E, g.dev_H, g.dev_counter = ptr(self.E_flat), ptr(self.H_flat), ptr(self._counter)
+        g.dev_E, g.dev_H, g.dev_counter = ptr(self._E_bufs[0]), ptr(self._H_bufs[0]), ptr(self._counter)
         g.tick = 0 if _sub else 1
         handle = ctypes.c_void_p()
         check(self.lib.lsf_create(ctypes.byref(g), ctypes.byref(handle)), "lsf_create")
@@ -357,6 +367,31 @@ class Engine:
             self._bind_time_monitor(m)
         self._graph_stream = None
         self._graphed = False
+        if self._two_sets:
+            arr = (ctypes.c_void_p * 3)(*[None if t is None else t.data_ptr() for t in self._psiH_second])
+            check(self.lib.lsf_set_fused(self._h, ptr(self._E_bufs[1]), ptr(self._H_bufs[1]), arr), "lsf_set_fused")
+
+    # --- fields: the current buffer set ------------------------------------------------------
+    @property
+    def parity(self) -> int:
+        """Which buffer set holds the current fields (the single-pass step alternates them)."""
+        return int(self.lib.lsf_parity(self._h)) if self._two_sets else 0
+
+    @property
+    def E_flat(self) -> torch.Tensor:
+        return self._E_bufs[self.parity]
+
+    @property
+    def H_flat(self) -> torch.Tensor:
+        return self._H_bufs[self.parity]
+
+    @property
+    def E(self) -> torch.Tensor:
+        return self._E_views[self.parity]
+
+    @property
+    def H(self) -> torch.Tensor:
+        return self._H_views[self.parity]
 
     # --- helpers ---------------------------------------------------------------------------
     def native_index(self, flat) -> np.ndarray:
@@ -505,6 +540,7 @@ class Engine:
     # --- CPML (engine.py:344-418) --------------------------------------------------------------
     def _build_pml(self, spec: PMLSpec, inv_eps):
         self.pml_cells = tuple(tuple(int(v) for v in c) for c in spec.cells)
+        self._psiH_second = [None, None, None]
         self.n_psi = 0  # psi values allocated (slab extent x full extent of the other two axes)
         self.n_psi_stepped = 0  # psi values the update touches (slab extent x the component's stepped box)
         for d in range(3):
@@ -521,9 +557,19 @@ class Engine:
                 size = ns * others[0] * others[1]
                 psis = [None, None, None]
                 regions = self.RE if fld == 0 else self.RH
+                # both components of an axis in one allocation (the single-pass kernel addresses them from one
+                # base); the H side of a two-set engine holds both sets
+                if fld == 1 and self._two_sets:
+                    block = torch.zeros(4 * size, dtype=self.fdt, device=self._dev)
+                    self._keep.append(block)
+                    self._psiH_blocks.append((block, size))
+                else:
+                    block = self.zeros(2 * size)
+                comps = [c for c in range(3) if c != d]
                 for c in range(3):
                     if c != d:
-                        psis[c] = self.zeros(size)
+                        slot = comps.index(c)
+                        psis[c] = block[slot * size:(slot + 1) * size]
                         self.n_psi += size
                         o1, o2 = (a for a in range(3) if a != d)
                         R = regions[c]
@@ -531,6 +577,8 @@ class Engine:
                 tprof = self.to_device(prof.astype(np.float32))
                 check(self.lib.lsf_set_pml(self._h, fld, d, ptr(tprof), ns, ptr(psis[0]), ptr(psis[1]),
                                            ptr(psis[2]), int(has_k)), "lsf_set_pml")
+                if fld == 1 and self._two_sets:
+                    self._psiH_second[d] = block[2 * size:]
 
     # --- mixed (interface) groups (engine.py:743-791) --------------------------------------------
     def _build_mixed(self, groups):
@@ -673,7 +721,7 @@ class Engine:
         src = self.native_index(hh.reshape(-1))
         self._od = [self.to_device(tgt), self.to_device(src), self.field_values(ww.reshape(-1))]
         check(self.lib.lsf_add_gather(self._h, PH_OFFDIAG, 0, len(tgt), 4 * width, ptr(self._od[0]),
-                                      ptr(self.H_flat), ptr(self._od[1]), ptr(self._od[2]), None),
+                                      None, ptr(self._od[1]), ptr(self._od[2]), None),
               "lsf_add_gather (off-diagonal)")
 
     # --- sparse work used by sources ---------------------------------------------------------------
@@ -839,6 +887,9 @@ class Engine:
         run exactly."""
         out, seen = [], set()
         ts = [self.E_flat, self.H_flat, self._counter, self.energy, *self._state]
+        par = self.parity
+        for block, size in self._psiH_blocks:
+            ts.append(block[par * 2 * size:(par + 1) * 2 * size])
         for m in self.monitors:
             ts += [getattr(m, "acc_e", None), getattr(m, "acc_h", None)]
         for m in self.time_monitors:

@@ -112,7 +112,8 @@ def medium(ns, eps_inf, poles):
 
 
 def dispersive_box(ns, device=None, shape=(40, 36, 32), dx=2.0, seed=0, pml_cells=6, n_steps_hint=400,
-                   with_dipole=True, frac=0.1, kappa_max=1.0, alpha=0.0, monitors=True, material="auto"):
+                   with_dipole=True, frac=0.1, kappa_max=1.0, alpha=0.0, monitors=True, material="auto", fused=None,
+                   stride=10):
     """CPML box with random dielectric values, a fraction of D-form 3-pole Lorentz nodes, a z dipole and a
     DFT plane: a small version of the 320^3 throughput configuration."""
     rng = np.random.default_rng(seed)
@@ -149,9 +150,11 @@ def dispersive_box(ns, device=None, shape=(40, 36, 32), dx=2.0, seed=0, pml_cell
         mons = [ns.DFTMonitor(omega, e, h1, h2, energy=1)]
     spec = ns.PMLSpec(cells=pml, kappa_max=kappa_max, alpha_max=alpha, eps_ref=1.0)
     kw = dict(shape=shape, dx=dx, dt=dt, inv_eps=inv_eps, groups=groups, pml=spec, sources=sources, monitors=mons,
-              dft_stride=10)
+              dft_stride=stride)
     if not ns.oracle:
         kw["material"] = material
+        if fused is not None:
+            kw["fused"] = fused
     return make(ns, device, **kw)
 
 

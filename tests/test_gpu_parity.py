@@ -304,3 +304,46 @@ def test_vertex_tensor_sphere(gpu, bloch):
     d = cases.rel_l2(eg, eo)
     print(f"  DFT rel L2 {d:.2e}")
     assert d < DFT_TOL
+
+
+def _same_state(a, b, label):
+    ta, tb = a.state_tensors(), b.state_tensors()
+    assert len(ta) == len(tb)
+    worst = 0.0
+    for x, y in zip(ta, tb):
+        if not torch.equal(x, y):
+            d = (x.double() - y.double()).abs().max().item() if x.dtype.is_floating_point or x.is_complex() else -1
+            worst = max(worst, d)
+    print(f"{label}: single pass vs two kernels, max |diff| {worst:.3e}")
+    assert worst == 0.0
+
+
+@pytest.mark.parametrize("material", ["indexed", "dense"])
+def test_single_pass_matches_two_kernels(gpu, material):
+    """The single-pass step (lsf_set_fused) against the two-kernel step on the dispersive CPML box (kappa and alpha,
+    D-form poles, dipole, DFT), eagerly and through graphs with an odd stride: bit-identical state."""
+    steps = 140
+    ns = cases.product_ns()
+    a = cases.dispersive_box(ns, gpu, shape=(70, 36, 33), n_steps_hint=steps, material=material, kappa_max=2.0,
+                             alpha=0.02, fused=True, stride=7)
+    b = cases.dispersive_box(ns, gpu, shape=(70, 36, 33), n_steps_hint=steps, material=material, kappa_max=2.0,
+                             alpha=0.02, fused=True, stride=7)
+    assert a._two_sets
+    cases.random_fields(a, 5)
+    cases.random_fields(b, 5)
+    ref = cases.dispersive_box(ns, gpu, shape=(70, 36, 33), n_steps_hint=steps, material=material, kappa_max=2.0,
+                               alpha=0.02, fused=False, stride=7)
+    assert not ref._two_sets
+    cases.random_fields(ref, 5)
+    for _ in range(steps // a.stride):
+        a.chunk()
+        ref.chunk()
+    torch.cuda.synchronize()
+    assert a.parity == (steps % 2)
+    _same_state(a, ref, f"eager, {material}")
+    run, used = b.make_runner(True)
+    for _ in range(steps // b.stride):
+        run()
+    b.release()
+    torch.cuda.synchronize()
+    _same_state(b, ref, f"graphs, {material}")
