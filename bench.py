@@ -11,7 +11,8 @@ Output: one JSON line on stdout. value = cells x steps / wall time of the timed 
 roofline: the bulk update (H kernel + E kernel) against the HBM roofline: algorithmic bytes per step
 (fields, coefficients, CPML psi, ADE state, each read and written once) / the two kernels' mean durations,
 measured with HIP events around each launch on the launch stream over a second pass of the same steps.
-cpu_baseline: the FP64 NumPy oracle (one thread) timed on a bounded sample of the same scene.
+cpu_baseline: the oracle restated in C (float64, OpenMP over the host's cores) on a bounded sample of the same
+scene; the FP64 NumPy oracle (one thread) alongside.
 batched: the oblique sweep (64 uneven Bloch-periodic runs) stepped as one autotuned Batch against the same runs
 stepped one after another on the plain engine; aggregate Gcell-updates/s of both and their ratio.
 adjoint: time of dL/drho (forward + adjoint run) over one forward run on the 256x192x96 adjoint scene.
@@ -56,8 +57,11 @@ def algorithmic_bytes(eng, scene) -> dict:
     return b
 
 
-def cpu_baseline(scene, steps: int) -> dict:
-    """The FP64 NumPy oracle on the same scene, one thread, `steps` timed steps after one warm-up step."""
+def cpu_baseline(scene, steps: int, c_steps: int) -> dict:
+    """The oracle on the same scene: the C restatement (oracle/cyee.c, float64, OpenMP over the host's cores)
+    timed over `c_steps` steps, and the FP64 NumPy oracle (one thread) over `steps` steps, each after two warm-up
+    steps. The C figure is the baseline reported; the NumPy one rides along."""
+    from oracle import cyee
     from oracle import fdtd as ofd
     from oracle import media as omed
     from oracle import sources as osrc
@@ -68,21 +72,33 @@ def cpu_baseline(scene, steps: int) -> dict:
     groups = [ofd.DispersiveGroup(idx=idx, media=[med], frac=np.ones((1, len(idx))), w=None)]
     src = osrc.PointDipole(shape=scene["shape"], origin=np.zeros(3), dx=scene["dx"], dt=scene["dt"],
                            pos=scene["dipole_pos"], pdir=scene["dipole_dir"], inv_eps=scene["inv_eps"],
-                           omega_c=scene["omega_c"], omega_half=scene["omega_half"], max_steps=steps + 10)
+                           omega_c=scene["omega_c"], omega_half=scene["omega_half"],
+                           max_steps=max(steps, c_steps) + 10)
     mon = ofd.DFTMonitor(scene["omega"], *scene["monitor"], energy=1)
     p = scene["pml"]
     pml = ofd.PMLSpec(cells=((p, p),) * 3, alpha_max=0.05 * float(np.min(scene["omega"])), eps_ref=1.0)
     eng = ofd.Engine(shape=scene["shape"], dx=scene["dx"], dt=scene["dt"], inv_eps=scene["inv_eps"], groups=groups,
                      pml=pml, sources=[src], monitors=[mon], dft_stride=scene["stride"])
-    eng.step()
+    cells = scene["n_cells"]
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count())
+    c = cyee.CYee(eng)
+    c.run(2)
+    t0 = time.perf_counter()
+    c.run(c_steps)
+    c_dt = time.perf_counter() - t0
+    for _ in range(2):
+        eng.step()
     t0 = time.perf_counter()
     for _ in range(steps):
         eng.step()
-    dt = time.perf_counter() - t0
-    return {"value": scene["n_cells"] * steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} steps of the same {scene['shape'][0]}^3 scene after 1 warm-up step; FP64 NumPy "
-                      "oracle (elementwise NumPy, one thread); DFT samples excluded (none fall in the sample)",
-            "seconds": dt}
+    np_dt = time.perf_counter() - t0
+    return {"value": cells * c_steps / c_dt / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{c_steps} steps of the same {scene['shape'][0]}^3 scene (H, E, CPML, ADE, dipole, DFT samples "
+                      f"every {scene['stride']} steps) after 2 warm-up steps; the oracle restated in C "
+                      f"(oracle/cyee.c: float64, OpenMP over {threads} threads, fields equal to the NumPy oracle's)",
+            "seconds": c_dt,
+            "numpy_one_thread": {"value": cells * steps / np_dt / 1e9, "cores": 1, "steps": steps,
+                                 "seconds": np_dt}}
 
 
 def batched_leg(dev, steps: int, n_inst: int = 64, tune: bool = True) -> dict:
@@ -235,7 +251,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=320, help="grid size per axis (the workload is 320)")
-    ap.add_argument("--cpu-steps", type=int, default=4, help="timed steps of the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="timed steps of the NumPy CPU oracle (0: skip the CPU legs)")
+    ap.add_argument("--cpu-c-steps", type=int, default=100, help="timed steps of the C (OpenMP) CPU oracle")
     ap.add_argument("--tiling", default=None, help="rows_per_lane,waves,x_cells,z_chunk")
     ap.add_argument("--no-roofline-pass", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch each chunk's kernels directly instead of a HIP graph")
@@ -340,9 +357,10 @@ def main() -> int:
 
     cpu = None
     if args.cpu_steps > 0:
-        log(f"CPU baseline: {args.cpu_steps} steps of the FP64 oracle ...")
-        cpu = cpu_baseline(scene, args.cpu_steps)
-        log(f"CPU baseline {cpu['value'] * 1e3:.2f} Mcell-updates/s ({cpu['seconds']:.1f} s)")
+        log(f"CPU baseline: {args.cpu_c_steps} steps of the C oracle, {args.cpu_steps} of the NumPy oracle ...")
+        cpu = cpu_baseline(scene, args.cpu_steps, args.cpu_c_steps)
+        log(f"CPU baseline {cpu['value'] * 1e3:.2f} Mcell-updates/s on {cpu['cores']} threads "
+            f"({cpu['seconds']:.1f} s); NumPy one thread {cpu['numpy_one_thread']['value'] * 1e3:.2f}")
 
     eng.close()
     batched = None
