@@ -1,0 +1,62 @@
+"""Mie cross sections of a homogeneous sphere (TEST INFRASTRUCTURE ONLY: the whole-solve check of the scattering
+driver).
+
+The reference's own Mie (backends/mie_native.py:216-262) cannot run here (array_api_compat is missing), so this
+restates the published series of Bohren & Huffman, "Absorption and Scattering of Light by Small Particles" (1983),
+chapter 4 and appendix A (BHMIE): the logarithmic derivative D_n(mx) by downward recurrence from n_stop + 15, the
+Riccati-Bessel functions psi_n, xi_n by upward recurrence, a_n = ((D_n/m + n/x) psi_n - psi_{n-1}) /
+((D_n/m + n/x) xi_n - xi_{n-1}), b_n likewise with m D_n, and
+Q_ext = 2/x^2 sum (2n+1) Re(a_n + b_n), Q_sca = 2/x^2 sum (2n+1)(|a_n|^2 + |b_n|^2), n_stop = x + 4 x^(1/3) + 2.
+
+Pinned in tests/test_oracle_pins.py by the book's worked example (m = 1.55, x = 5.213), the Rayleigh limit and the
+optical theorem for a lossless sphere.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def mie_q(m: complex, x: float) -> tuple[float, float, float]:
+    """(Q_ext, Q_sca, Q_back) for relative index m and size parameter x = k r (k in the host medium)."""
+    m = complex(m)
+    x = float(x)
+    mx = m * x
+    n_stop = int(x + 4.0 * x ** (1.0 / 3.0) + 2.0)
+    n_mx = int(max(n_stop, abs(mx))) + 15
+    D = np.zeros(n_mx + 1, dtype=complex)
+    for n in range(n_mx, 0, -1):
+        D[n - 1] = n / mx - 1.0 / (D[n] + n / mx)
+    psi0, psi1 = np.cos(x), np.sin(x)
+    chi0, chi1 = -np.sin(x), np.cos(x)
+    xi1 = complex(psi1, -chi1)
+    q_ext = q_sca = 0.0
+    back = 0j
+    for n in range(1, n_stop + 1):
+        psi = (2.0 * n - 1.0) * psi1 / x - psi0
+        chi = (2.0 * n - 1.0) * chi1 / x - chi0
+        xi = complex(psi, -chi)
+        a = ((D[n] / m + n / x) * psi - psi1) / ((D[n] / m + n / x) * xi - xi1)
+        b = ((m * D[n] + n / x) * psi - psi1) / ((m * D[n] + n / x) * xi - xi1)
+        q_ext += (2 * n + 1) * (a + b).real
+        q_sca += (2 * n + 1) * (abs(a) ** 2 + abs(b) ** 2)
+        back += (2 * n + 1) * (-1) ** n * (a - b)
+        psi0, psi1 = psi1, psi
+        chi0, chi1 = chi1, chi
+        xi1 = complex(psi1, -chi1)
+    s = 2.0 / (x * x)
+    return s * q_ext, s * q_sca, abs(back) ** 2 / (x * x)
+
+
+def cross_sections(eps_sphere, radius: float, wavelengths, n_host: float = 1.0):
+    """(C_ext, C_sca, C_abs) per wavelength (vacuum, same length unit as the radius) for a sphere of complex
+    permittivity eps_sphere (a scalar or one value per wavelength) in a host of index n_host."""
+    wl = np.atleast_1d(np.asarray(wavelengths, dtype=float))
+    eps = np.broadcast_to(np.asarray(eps_sphere, dtype=complex), wl.shape)
+    out = np.zeros((3, len(wl)))
+    for q, (lam, e) in enumerate(zip(wl, eps)):
+        k = 2 * np.pi * n_host / lam
+        qe, qs, _ = mie_q(np.sqrt(e) / n_host, k * radius)
+        g = np.pi * radius ** 2
+        out[:, q] = (qe * g, qs * g, (qe - qs) * g)
+    return out[0], out[1], out[2]

@@ -1,0 +1,94 @@
+"""The problem drivers end to end on the GPU (lightsmith_amd/fdtd/drivers.py) against closed forms: a dielectric
+sphere's cross sections against Mie theory (oracle/mie.py), an empty periodic cell and a dielectric half-space at
+normal and oblique incidence against Fresnel, a dipole's emitted power against the power through a box around
+it. The tolerances are discretisation tolerances (a staircase sphere of 12 cells radius; k dx up to 0.16), stated
+per test; the measured errors are printed."""
+
+import math
+
+import numpy as np
+import pytest
+
+from lightsmith_amd.fdtd import drivers
+from lightsmith_amd.fdtd.media import Medium
+
+pytestmark = pytest.mark.gpu
+
+
+def test_dielectric_sphere_cross_sections_match_mie(gpu):
+    from oracle.mie import cross_sections
+    r, dx, eps = 60.0, 5.0, 4.0
+    wl = np.array([400.0, 500.0, 600.0])
+    lay = drivers.scattering_layout((-r, -r, -r), (r, r, r), dx, float(wl.max()))
+    host, glass = Medium(1.0), Medium(eps)
+    inv, groups = drivers.staircase(lay.shape, lay.origin, dx, host, [drivers.Sphere((0.0, 0.0, 0.0), r, glass)])
+    res = drivers.solve_scattering(gpu, lay, inv, groups, [host, glass], wl, khat=(0, 0, 1.0), ehat=(1.0, 0, 0),
+                                   opts=drivers.DriverOptions(decay_by=1e-6))
+    ce, cs, _ = cross_sections(eps, r, wl)
+    e_ext = np.abs(res.c_ext / ce - 1)
+    e_sca = np.abs(res.c_sca / cs - 1)
+    print(f"sphere r={r} nm, dx={dx}: C_ext {res.c_ext} vs Mie {ce} (rel {e_ext}); C_sca rel {e_sca}; "
+          f"C_abs/C_ext {res.c_abs / res.c_ext}; steps {res.provenance['steps']}")
+    assert np.all(e_ext < 0.06) and np.all(e_sca < 0.06)
+    assert np.all(np.abs(res.c_abs) < 0.01 * res.c_ext)
+
+
+def _cell(eps_sub, n=(4, 4), d=10.0, pml=12, gap=24, pad=6, layer=10):
+    k_r = pml + gap
+    k_src = k_r + 2
+    k_top = k_src + pad
+    k_bot = k_top + layer
+    k_t = k_bot + 3
+    nz = k_t + gap + pml + 2
+    shape = (n[0] + 2, n[1] + 2, nz)
+    eps = np.ones(shape)
+    eps[:, :, k_bot:] = eps_sub
+    inv = np.broadcast_to(1 / eps, (3,) + shape).copy()
+    return drivers.PeriodicCell(n_cells=n, nz=nz, spacing=(d, d, d), inv_eps=inv, groups=[], pml=pml, k_src=k_src,
+                                k_r=k_r, k_t=k_t, k_top=k_top, eps_sup=1.0, eps_sub=eps_sub)
+
+
+def test_empty_cell_transmits_everything(gpu):
+    res = drivers.solve_periodic(gpu, _cell(1.0), [500.0, 600.0, 700.0])
+    R, T = res.r[:, 0, 0], res.t[:, 0, 0]
+    print(f"empty cell: R {R}, T {T}")
+    assert np.all(np.abs(R) < 1e-3) and np.all(np.abs(T - 1) < 1e-3)
+    assert np.all(np.abs(res.r).sum(axis=(1, 2)) < 2e-3)
+
+
+@pytest.mark.parametrize("pol", ["s", "p"])
+def test_half_space_normal_incidence_is_fresnel(gpu, pol):
+    n2 = 1.5
+    res = drivers.solve_periodic(gpu, _cell(n2 ** 2), [500.0, 600.0, 700.0], pol=pol)
+    R, T = res.r[:, 0, 0], res.t[:, 0, 0]
+    Rf = ((1 - n2) / (1 + n2)) ** 2
+    print(f"half-space {pol}: R {R} T {T} (Fresnel {Rf:.5f}, {1 - Rf:.5f})")
+    assert np.all(np.abs(R - Rf) < 4e-3) and np.all(np.abs(T - (1 - Rf)) < 4e-3)
+
+
+@pytest.mark.parametrize("pol", ["s", "p"])
+def test_half_space_oblique_incidence_is_fresnel(gpu, pol):
+    n2, theta, lam = 1.5, math.radians(30.0), 600.0
+    k = 2 * math.pi / lam * math.sin(theta)
+    res = drivers.solve_periodic(gpu, _cell(n2 ** 2), [lam], pol=pol, kpar=(k, 0.0))
+    R, T = float(res.r[0, 0, 0]), float(res.t[0, 0, 0])
+    ci = math.cos(theta)
+    ct = math.sqrt(1 - (math.sin(theta) / n2) ** 2)
+    if pol == "s":
+        Rf = ((ci - n2 * ct) / (ci + n2 * ct)) ** 2
+    else:
+        Rf = ((n2 * ci - ct) / (n2 * ci + ct)) ** 2
+    print(f"half-space oblique {pol}: R {R:.5f} T {T:.5f} (Fresnel {Rf:.5f})")
+    assert abs(R - Rf) < 8e-3 and abs(T - (1 - Rf)) < 8e-3  # the reference's tolerance (test_fdtd_periodic.py:91-103)
+
+
+def test_dipole_power_leaves_through_the_box(gpu):
+    dx = 2.0
+    wl = np.array([500.0, 600.0, 700.0])
+    lay = drivers.scattering_layout((-6, -6, -6), (6, 6, 6), dx, float(wl.max()))
+    host = Medium(1.0)
+    inv = np.ones((3,) + lay.shape)
+    res = drivers.solve_emission(gpu, lay, inv, [], [host], wl, pos=(0.3, -0.2, 0.1), pdir=(0.0, 0.0, 1.0))
+    ratio = res.p_rad / res.p_tot
+    print(f"dipole: p_rad / p_tot {ratio}, steps {res.provenance['steps']}")
+    assert np.all(res.p_tot > 0) and np.all(np.abs(ratio - 1) < 0.01)

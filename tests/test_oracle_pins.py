@@ -437,3 +437,25 @@ def test_vertex_tensor_in_one_medium_is_the_plain_update():
     eng.E_flat[tm.idx] = x
     g.step(eng)
     assert np.max(np.abs(eng.E_flat[tm.idx] - x / 1.77)) < 1e-14
+
+
+def test_mie_series_reproduces_the_worked_example_and_limits():
+    """oracle/mie.py: Bohren & Huffman's BHMIE example (m = 1.55, x = 5.213: Q_ext = Q_sca = 3.10543,
+    Q_back = 2.92534), the Rayleigh limit Q_sca = 8/3 x^4 |(m^2 - 1)/(m^2 + 2)|^2 and the optical theorem
+    (Q_ext = Q_sca without loss, Q_abs > 0 with it)."""
+    from oracle.mie import cross_sections, mie_q
+    qe, qs, qb = mie_q(1.55, 2 * np.pi * 0.525 / 0.6328)
+    assert abs(qe - 3.10543) < 2e-5 and abs(qs - 3.10543) < 2e-5 and abs(qb - 2.92534) < 2e-5
+    for m in (1.5, 2.0 + 0.3j):
+        x = 1e-3
+        qe, qs, _ = mie_q(m, x)
+        ray = 8 / 3 * x ** 4 * abs((m * m - 1) / (m * m + 2)) ** 2
+        assert abs(qs / ray - 1) < 1e-5
+    for x in (0.3, 2.0, 7.5):
+        qe, qs, _ = mie_q(1.8, x)
+        assert abs(qe - qs) < 1e-12 * max(1, qe)
+        qe, qs, _ = mie_q(1.8 + 0.2j, x)
+        assert qe - qs > 0
+    ce, cs, ca = cross_sections(4.0, 60.0, [500.0], n_host=1.0)
+    qe, qs, _ = mie_q(2.0, 2 * np.pi * 60.0 / 500.0)
+    assert abs(ce[0] - qe * np.pi * 3600) < 1e-9 * ce[0] and abs(ca[0]) < 1e-9 * ce[0]
