@@ -254,6 +254,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=3, help="timed steps of the NumPy CPU oracle (0: skip the CPU legs)")
     ap.add_argument("--cpu-c-steps", type=int, default=100, help="timed steps of the C (OpenMP) CPU oracle")
     ap.add_argument("--tiling", default=None, help="rows_per_lane,waves,x_cells,z_chunk")
+    ap.add_argument("--single-pass", choices=["auto", "on", "off"], default="auto",
+                    help="step with the single-pass kernel (on), the H and E kernels (off), or the engine default")
     ap.add_argument("--no-roofline-pass", action="store_true")
     ap.add_argument("--eager", action="store_true", help="launch each chunk's kernels directly instead of a HIP graph")
     ap.add_argument("--batched-steps", type=int, default=300,
@@ -301,7 +303,8 @@ def main() -> int:
     K = workloads.steps_for(args.steps, stride)
     W = workloads.steps_for(max(args.warmup, stride), stride)
     tiling = tuple(int(v) for v in args.tiling.split(",")) if args.tiling else None
-    eng = workloads.throughput_engine(scene, dev, max_steps=W + 2 * K + 4 * stride, tiling=tiling)
+    fused = {"auto": None, "on": True, "off": False}[args.single_pass]
+    eng = workloads.throughput_engine(scene, dev, max_steps=W + 2 * K + 4 * stride, tiling=tiling, fused=fused)
     torch.cuda.synchronize()
     log(f"set-up {time.perf_counter() - t_setup:.1f} s: {scene['shape']} cells, dt {scene['dt']:.4f} nm/c, "
         f"stride {stride}, {eng.n_ade} D-form nodes, {eng.n_psi} psi values, material {eng.material_mode}")
@@ -348,7 +351,8 @@ def main() -> int:
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": None if traffic is None else traffic.get("bytes_per_step"),
-                    "kernel": "k_update_h + k_update_e (one time step's bulk update)",
+                    "kernel": ("k_step_fused (one time step's bulk update in a single pass)" if h_ms == 0.0 else
+                               "k_update_h + k_update_e (one time step's bulk update)"),
                     "algorithmic_bytes_per_step": int(bytes_["total"]),
                     "kernel_us": {"h": round(h_ms * 1e3, 2), "e": round(e_ms * 1e3, 2)},
                     "step_achieved_GBs": round(bytes_["step_total"] / elapsed * K / 1e9, 1)}

@@ -173,7 +173,8 @@ int lsf_set_tiling(lsf_engine* eng, int rows_per_lane, int waves, int x_cells, i
 /* Run n_steps steps (Engine.step, engine.py:665-678) on `stream` (a hipStream_t; NULL = default). */
 int lsf_step(lsf_engine* eng, int n_steps, void* stream);
 /* As lsf_step, bracketing each H and E kernel with HIP events on `stream`; adds the summed kernel times (ms)
- * to ms[0] (H kernels) and ms[1] (E kernels). Synchronises the stream at the end. For measurement. */
+ * to ms[0] (H kernels) and ms[1] (E kernels; the single-pass kernel when the engine steps with it, ms[0] then
+ * gaining 0). Synchronises the stream at the end. For measurement. */
 int lsf_step_timed(lsf_engine* eng, int n_steps, void* stream, double* ms);
 /* One DFT/time-monitor sample (Engine._sample, engine.py:568-592). */
 int lsf_sample(lsf_engine* eng, void* stream);

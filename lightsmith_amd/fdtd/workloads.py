@@ -96,7 +96,8 @@ def steps_for(total: int, stride: int) -> int:
     return int(math.ceil(total / stride)) * stride
 
 
-def throughput_engine(scene: dict, device="cuda:0", max_steps: int = 10000, tiling=None, material: str = "auto"):
+def throughput_engine(scene: dict, device="cuda:0", max_steps: int = 10000, tiling=None, material: str = "auto",
+                      fused: bool | None = None):
     """The HIP engine of a throughput scene: pure D-form group, CPML (alpha from the lowest frequency, as the
     emission driver sets it, emission.py:176-177), the dipole and the DFT plane (energy proxy 1)."""
     from .engine import DFTMonitor, DispersiveGroup, Engine, PMLSpec
@@ -116,7 +117,7 @@ def throughput_engine(scene: dict, device="cuda:0", max_steps: int = 10000, tili
     try:
         eng = Engine(device, shape=shape, dx=scene["dx"], dt=scene["dt"], single=True, inv_eps=scene["inv_eps"],
                      groups=groups, pml=pml, sources=[src], monitors=[mon], dft_stride=scene["stride"],
-                     material=material)
+                     material=material, fused=fused)
     finally:
         Engine.tiling = old
     return eng

@@ -9,6 +9,7 @@ rule. Each test prints the measured errors.
 
 import numpy as np
 import pytest
+import torch
 
 from tests import cases
 
@@ -112,7 +113,6 @@ def test_bloch_periodic_complex(gpu):
             vals = rng.standard_normal(tuple(r.stop - r.start for r in R)) + 1j * rng.standard_normal(
                 tuple(r.stop - r.start for r in R))
             getattr(o, F)[c][R] = vals
-            import torch
             getattr(g, F)[c][R] = torch.as_tensor(vals, dtype=torch.complex64, device=g.E_flat.device)
     o.run(80)
     g.steps(80)
