@@ -563,6 +563,7 @@ class Engine:
                     block = torch.zeros(4 * size, dtype=self.fdt, device=self._dev)
                     self._keep.append(block)
                     self._psiH_blocks.append((block, size))
+                    self._state.append(len(self._psiH_blocks) - 1)  # resolved to the current set by parity
                 else:
                     block = self.zeros(2 * size)
                 comps = [c for c in range(3) if c != d]
@@ -886,10 +887,13 @@ class Engine:
         state, source lines, monitor accumulators, sub-engines'), each once: a snapshot of these restores the
         run exactly."""
         out, seen = [], set()
-        ts = [self.E_flat, self.H_flat, self._counter, self.energy, *self._state]
         par = self.parity
-        for block, size in self._psiH_blocks:
-            ts.append(block[par * 2 * size:(par + 1) * 2 * size])
+        ts = [self.E_flat, self.H_flat, self._counter, self.energy]
+        for t in self._state:  # an int names an H-side psi block of a two-set engine: its current half
+            if isinstance(t, int):
+                block, size = self._psiH_blocks[t]
+                t = block[par * 2 * size:(par + 1) * 2 * size]
+            ts.append(t)
         for m in self.monitors:
             ts += [getattr(m, "acc_e", None), getattr(m, "acc_h", None)]
         for m in self.time_monitors:
