@@ -143,6 +143,8 @@ struct lsf_engine {
   bool fused_set = false;              // second buffers given (lsf_set_fused)
   bool use_fused = false;              // the single-pass kernel steps this engine (decided at sync_args)
   lsf::FusedArgs* dev_fused = nullptr; // library scratch: [parity]
+  int* dev_tiles = nullptr;            // single-pass tile lists by CPML class, back to back
+  int fused_count[3] = {0, 0, 0};
   hipGraph_t graph2 = nullptr;         // the chunk graph starting at the other parity (odd strides)
   hipGraphExec_t exec2 = nullptr;
   int graph_parity = 0, graph_stride = 0;
@@ -228,11 +230,22 @@ long long psi_gap(const lsf_engine* e, int field, int ax, int parity_set) {
 bool fused_eligible(const lsf_engine* e) {
   if (!e->fused_set || e->cplx || any_periodic(e) || e->adjoint || e->ry != 1 || e->xw != 64 || has_h_phase_ops(e))
     return false;
+  // the kernel assumes the stepped boxes of a grid without periodic axes or mirror planes
+  for (int c = 0; c < 3; ++c)
+    for (int ax = 0; ax < 3; ++ax) {
+      const int n = e->g.n[ax];
+      if (e->g.rE[c][ax][0] != (ax == c ? 0 : 1) || e->g.rE[c][ax][1] != n - 1) return false;
+      if (e->g.rH[c][ax][0] != 0 || e->g.rH[c][ax][1] != (ax == c ? n : n - 1)) return false;
+    }
+  const long long lim = 1LL << 32;  // the kernel addresses fields and psi blocks with 32-bit byte offsets
+  if (3 * e->g.N * 4 >= lim) return false;
   for (int f = 0; f < 2; ++f)
     for (int ax = 0; ax < 3; ++ax)
       if (e->pml[f][ax].prof)
-        for (int p = 0; p < (f == 1 ? 2 : 1); ++p)
-          if (psi_gap(e, f, ax, p) < 0) return false;
+        for (int p = 0; p < (f == 1 ? 2 : 1); ++p) {
+          const long long gap = psi_gap(e, f, ax, p);
+          if (gap < 0 || 2 * gap * 4 >= lim) return false;
+        }
   return true;
 }
 
@@ -255,6 +268,7 @@ lsf::FusedArgs fused_args(const lsf_engine* e, int p) {
     a.psiHo[ax] = (float*)e->psiH[p ^ 1][ax][c1];
     for (int f = 0; f < 2; ++f) {
       a.psz[f][ax] = e->pml[f][ax].prof ? psi_gap(e, f, ax, 0) : 0;
+      for (int c = 0; c < 3; ++c) a.pso[f][ax][c] = c == ax ? 0u : (unsigned)(lsf::psi_slot(c, ax) * a.psz[f][ax] * 4);
       a.ns[f][ax] = e->pml[f][ax].ns;
       a.has_k[f][ax] = e->pml[f][ax].has_k;
     }
@@ -272,6 +286,52 @@ lsf::FusedArgs fused_args(const lsf_engine* e, int p) {
   a.tiles_y = (e->g.n[1] + e->waves - 1) / e->waves;
   a.zc = e->zc;
   return a;
+}
+
+// The single-pass tiles in three lists by the CPML axes they touch: a tile reads psi on the x span [x0 - 1, x0 + 64),
+// the rows [j0 - 1, j0 + waves) and the planes [z0 - 1, z1) (the halo cells it recomputes included). Lists: no CPML,
+// x CPML only, any other.
+int fused_tiles(lsf_engine* e, const lsf::FusedArgs& a) {
+  const int n[3] = {e->g.n[0], e->g.n[1], e->g.n[2]};
+  std::vector<char> slab[3];
+  for (int ax = 0; ax < 3; ++ax) {
+    slab[ax].assign(n[ax], 0);
+    for (int f = 0; f < 2; ++f) {
+      if (!e->pml[f][ax].prof) continue;
+      std::vector<float4> pr(n[ax]);
+      const int rc = hip_check(hipMemcpy(pr.data(), e->pml[f][ax].prof, n[ax] * sizeof(float4), hipMemcpyDeviceToHost),
+                               "CPML profile read");
+      if (rc) return rc;
+      for (int q = 0; q < n[ax]; ++q)
+        if ((int)pr[q].w >= 0) slab[ax][q] = 1;
+    }
+  }
+  auto touches = [&](int ax, int lo, int hi) {  // any slab node in [lo, hi) (clamped to the grid)
+    for (int q = std::max(lo, 0); q < std::min(hi, n[ax]); ++q)
+      if (slab[ax][q]) return true;
+    return false;
+  };
+  std::vector<int> lists[3];
+  const int tz_n = (n[2] + a.zc - 1) / a.zc;
+  for (int tz = 0; tz < tz_n; ++tz)
+    for (int ty = 0; ty < a.tiles_y; ++ty)
+      for (int tx = 0; tx < a.tiles_x; ++tx) {
+        const int z0 = tz * a.zc, z1 = std::min(z0 + a.zc, n[2]), j0 = ty * e->waves, x0 = tx * 64;
+        const int m = (touches(0, x0 - 1, x0 + 64) ? 1 : 0) | (touches(1, j0 - 1, j0 + e->waves) ? 2 : 0) |
+                      (touches(2, z0 - 1, z1) ? 4 : 0);
+        lists[m == 0 ? 0 : (m == 1 ? 1 : 2)].push_back((tz * a.tiles_y + ty) * a.tiles_x + tx);
+      }
+  std::vector<int> all;
+  for (int c = 0; c < 3; ++c) {
+    e->fused_count[c] = (int)lists[c].size();
+    all.insert(all.end(), lists[c].begin(), lists[c].end());
+  }
+  if (e->dev_tiles) (void)hipFree(e->dev_tiles);
+  e->dev_tiles = nullptr;
+  int rc = hip_check(hipMalloc(&e->dev_tiles, std::max<size_t>(1, all.size()) * sizeof(int)), "hipMalloc tile lists");
+  if (!rc) rc = hip_check(hipMemcpy(e->dev_tiles, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice),
+                          "tile list upload");
+  return rc;
 }
 
 // Upload the engine's kernel arguments (and its sub-engines') when its configuration changed. Not called
@@ -300,6 +360,8 @@ int sync_args(lsf_engine* e) {
     const lsf::FusedArgs f[2] = {fused_args(e, 0), fused_args(e, 1)};
     rc = hip_check(hipMemcpy(e->dev_fused, f, sizeof f, hipMemcpyHostToDevice), "fused argument upload");
     if (rc) return rc;
+    rc = fused_tiles(e, f[0]);
+    if (rc) return rc;
   }
   e->dirty = false;
   return LSF_OK;
@@ -308,7 +370,10 @@ int sync_args(lsf_engine* e) {
 void launch_fused_step(lsf_engine* e, hipStream_t s) {
   lsf::FusedLaunch L{};
   L.args = e->dev_fused + e->parity;
-  L.sc = lsf::Sched{nullptr, nullptr, e->nb, e->nbp};
+  L.tiles[0] = e->dev_tiles;
+  L.tiles[1] = e->dev_tiles + e->fused_count[0];
+  L.tiles[2] = e->dev_tiles + e->fused_count[0] + e->fused_count[1];
+  for (int c = 0; c < 3; ++c) L.count[c] = e->fused_count[c];
   L.waves = e->waves;
   L.mat = e->mid ? 1 : 0;
   L.np = pole_regs(e);
@@ -597,6 +662,7 @@ void lsf_destroy(lsf_engine* e) {
   if (e->mon_tab) (void)hipFree(e->mon_tab);
   if (e->dev_args) (void)hipFree(e->dev_args);
   if (e->dev_fused) (void)hipFree(e->dev_fused);
+  if (e->dev_tiles) (void)hipFree(e->dev_tiles);
   if (e->exec2) (void)hipGraphExecDestroy(e->exec2);
   if (e->graph2) (void)hipGraphDestroy(e->graph2);
   delete e;

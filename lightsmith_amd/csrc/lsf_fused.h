@@ -5,12 +5,16 @@
 // so a block never reads another block's new values and no store is ordered before a load. A block is 64 x cells
 // by WAVES rows (one row per wavefront) marching up a chunk of z planes. Per plane every wavefront computes
 // H^{n+1/2} of its row and publishes H_x, H_z to LDS (double-buffered by plane, one barrier per plane), then
-// updates E^{n+1} of its row from its own H (the node and, by shuffle, x - 1), the row below (LDS) and the
+// updates E^{n+1} of its row from its own H (the node and, by a lane shift, x - 1), the row below (LDS) and the
 // previous plane (registers). Values owned by neighbouring blocks are recomputed with their owners' arithmetic:
-// the row below the block (first wavefront), the column x - 1 of the tile (lane 0) and the plane below the chunk
-// (a first pass through the march), so the result is bit-identical to the two-kernel step. E-side psi and the pole
-// state are updated in place (one owner each). Per cell the pass moves E and H once in and once out (48 B) plus
-// the material, against 72 B for the two kernels.
+// the row below the block (first wavefront), the column x - 1 of the tile and the plane below the chunk (a first
+// pass through the march). E-side psi and the pole state are updated in place (one owner each). Per cell the pass
+// moves E and H once in and once out (48 B) plus the material, against 72 B for the two kernels.
+//
+// Instruction economy: every lane loads unconditionally from clamped, in-grid addresses given as 32-bit byte
+// offsets from a wave-uniform base (no branches around loads, no 64-bit address arithmetic); values outside a
+// component's stepped box are masked with selects; the x - 1 and x + 1 halo cells of the tile are wave-uniform and
+// come through scalar loads (E_in and H_in are read-only here); tests on the row and the plane are wave-uniform.
 #pragma once
 
 #include "lsf_step.h"
@@ -34,6 +38,7 @@ struct FusedArgs {
   const float* psiHi[3];
   float* psiHo[3];
   long long psz[2][3];
+  unsigned pso[2][3][3];  // byte offset of component c's slot in the (side, axis) psi block (c != axis)
   int ns[2][3];
   int has_k[2][3];
   const int* chunk_base;
@@ -47,16 +52,33 @@ struct FusedArgs {
 
 __host__ __device__ __forceinline__ constexpr int psi_slot(int c, int ax) { return c - (c > ax ? 1 : 0); }
 
-template <int MAT, int WAVES, int NP>
-__global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __restrict__ args, Sched sc) {
+namespace fused {
+
+// base + a 32-bit byte offset: the loads compile to global loads with a scalar base and a 32-bit vector offset
+__device__ __forceinline__ float ld(gptr<const float> base, unsigned off) {
+  return *(gptr<const float>)((gptr<const char>)base + off);
+}
+__device__ __forceinline__ void st(gptr<float> base, unsigned off, float v) { *(gptr<float>)((gptr<char>)base + off) = v; }
+// a wave-uniform load through the scalar cache (the data is read-only for the whole kernel)
+__device__ __forceinline__ float ldu(const float* base, unsigned off) { return *(cfloat_ptr)((const char*)base + off); }
+
+}  // namespace fused
+
+// PML: bit a set when the tiles given may hold CPML cells of axis a (their halos included); the psi paths of the
+// other axes drop out.
+template <int MAT, int WAVES, int NP, int PML>
+__global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __restrict__ args, Sched sc,
+                                                          const int* __restrict__ tiles) {
+  using namespace fused;
   __shared__ float tab[MAT == 1 ? 256 : 1];
   __shared__ float med_tab[NP > 0 ? MAX_MEDIA_FLOATS : 1];
   __shared__ float lhx[2][WAVES + 1][64];  // H_x, H_z of each row per plane parity; [0] is the row below the block
   __shared__ float lhz[2][WAVES + 1][64];
-  const int t = xcd_tile(blockIdx.x, sc.n_pad);
-  if (t >= sc.n_tiles) return;  // whole block
+  const int tb = xcd_tile(blockIdx.x, sc.n_pad);
+  if (tb >= sc.n_tiles) return;  // whole block
+  const int t = ld_uniform(tiles, tb);  // logical tile (x fastest, then y, then z chunk)
   const FusedArgs& a = args[0];
-  if (a.tick && t == 0 && threadIdx.x == 0) {
+  if (a.tick && t == 0 && threadIdx.x == 0) {  // tile 0 is in exactly one list
     long long* cnt = (long long*)as_global(a.counter);
     const long long n = cnt[0];
     cnt[1] = n;
@@ -68,189 +90,260 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
     for (int q = threadIdx.x; q < a.n_media_floats; q += 64 * WAVES) med_tab[q] = ((const float*)as_global(a.media))[q];
 
   const Grid& g = a.g;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tx = t % a.tiles_x, rest = t / a.tiles_x;
   const int ty = rest % a.tiles_y, tz = rest / a.tiles_y;
-  const int j = ty * WAVES + wv;  // this wavefront's row
-  const bool jin = j < g.n[1];     // rows past the grid still take part in the barriers
-  const int i = tx * 64 + lane;
-  const int z0 = tz * a.zc, z1 = min(z0 + a.zc, g.n[2]);
-  const unsigned N = (unsigned)g.N, nx = g.n[0], ny = g.n[1];
-  const bool inx = i < g.n[0];
-  const bool on = inx && jin;
-  const int ih = tx * 64 + 64;
-  const bool hx_ok = (lane == 63) && ih < g.n[0] && jin;
-  const int il = tx * 64 - 1;
-  const bool xl_ok = (lane == 0) && il >= 0 && jin;
-  const bool yhalo = wv == 0 && j >= 1;  // the first wavefront recomputes the row below the block
-  const int jm = j - 1;
+  const int x0 = tx * 64;
+  const int j = ty * WAVES + wv;  // this wavefront's row (rows past the grid still take part in the barriers)
+  const bool jin = j < ny;
+  const int jc = jin ? j : ny - 1;  // the row loads read
+  const int i = x0 + lane;
+  const int ic = i < nx ? i : nx - 1;
+  const int z0 = tz * a.zc, z1 = min(z0 + a.zc, nz);
+  const unsigned N4 = (unsigned)g.N * 4u, P4 = (unsigned)(nx * ny) * 4u, R4 = (unsigned)nx * 4u;
+  const unsigned ol = (unsigned)jc * R4 + (unsigned)ic * 4u;  // this lane's byte offset within a plane
+  const unsigned dup = jc + 1 < ny ? R4 : 0u;                  // to the row above (clamped)
+  const bool has_l = x0 >= 1 && jin;                            // the column x - 1 exists
+  const bool has_r = x0 + 64 < nx && jin;                       // the column x + 64 exists
+  const unsigned oxl = ((unsigned)jc * nx + (unsigned)(has_l ? x0 - 1 : 0)) * 4u;
+  const unsigned oxr = ((unsigned)jc * nx + (unsigned)(has_r ? x0 + 64 : 0)) * 4u;
+  const bool yhalo = wv == 0 && jin && j >= 1;  // the first wavefront recomputes the row below the block
+  const unsigned olm = yhalo ? ol - R4 : ol;
+  const unsigned oxrm = yhalo && has_r ? oxr - R4 : oxr;
   const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  const float* __restrict__ Ei = (const float*)as_global(a.E_in);
-  const float* __restrict__ Hi = (const float*)as_global(a.H_in);
-  float* __restrict__ Eo = (float*)as_global(a.E_out);
-  float* __restrict__ Ho = (float*)as_global(a.H_out);
 
-  const float4 pex = (a.prof[0][0] && inx) ? ((const float4*)as_global(a.prof[0][0]))[i] : no_pml();
-  const float4 phx = (a.prof[1][0] && inx) ? ((const float4*)as_global(a.prof[1][0]))[i] : no_pml();
-  const float4 phxl = (a.prof[1][0] && xl_ok) ? ((const float4*)as_global(a.prof[1][0]))[il] : no_pml();
-  const float4 pey = (a.prof[0][1] && jin) ? ld_uniform(a.prof[0][1], j) : no_pml();
-  const float4 phy = (a.prof[1][1] && jin) ? ld_uniform(a.prof[1][1], j) : no_pml();
-  const float4 phym = (a.prof[1][1] && yhalo) ? ld_uniform(a.prof[1][1], jm) : no_pml();
+  gptr<const float> __restrict__ Ex = as_global(a.E_in);
+  gptr<const float> __restrict__ Ey = as_global(a.E_in + g.N);
+  gptr<const float> __restrict__ Ez = as_global(a.E_in + 2 * g.N);
+  gptr<const float> __restrict__ Hx = as_global(a.H_in);
+  gptr<const float> __restrict__ Hy = as_global(a.H_in + g.N);
+  gptr<const float> __restrict__ Hz = as_global(a.H_in + 2 * g.N);
+  gptr<float> __restrict__ Eo = as_global(a.E_out);
+  gptr<float> __restrict__ Ho = as_global(a.H_out);
 
-  auto cell = [&](int x, int y, int k) -> unsigned { return cell32(nx, ny, x, y, k); };
-  auto psi_hi = [&](int c, int ax, int p, int x, int y, int k) -> float {
-    if (p < 0) return 0.f;
-    return ((const float*)as_global(a.psiHi[ax]))[psi_slot(c, ax) * a.psz[1][ax] +
-                                                  psi_index(ax, p, a.ns[1][ax], nx, ny, x, y, k)];
+  // stepped boxes without periodic axes or mirror planes (engine.py:47-66): E_c spans [0, n - 1) along c and
+  // [1, n - 1) across; H_c spans [0, n) along c and [0, n - 1) across. Per lane along x, per wavefront along y,
+  // per plane along z.
+  bool bxE[3], bxH[3], byE[3], byH[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    bxE[c] = c == 0 ? i < nx - 1 : (i >= 1 && i < nx - 1);
+    bxH[c] = c == 0 ? i < nx : i < nx - 1;
+    byE[c] = jin && (c == 1 ? j < ny - 1 : (j >= 1 && j < ny - 1));
+    byH[c] = jin && (c == 1 || j < ny - 1);
+  }
+  // the row below the block (j - 1 < ny - 1) and the column x - 1 (x0 - 1 < nx - 1) lie in the H boxes across
+  const bool lxH1 = has_l && byH[1];
+  const bool lxH2 = has_l && byH[2];
+
+  // CPML profiles {b, c, 1/kappa, slab index or -1}
+  const float4 pex = ((PML & 1) && a.prof[0][0]) ? ((const float4*)as_global(a.prof[0][0]))[ic] : no_pml();
+  const float4 phx = ((PML & 1) && a.prof[1][0]) ? ((const float4*)as_global(a.prof[1][0]))[ic] : no_pml();
+  const float4 phxl = ((PML & 1) && a.prof[1][0] && has_l) ? ld_uniform(a.prof[1][0], x0 - 1) : no_pml();
+  const float4 pey = ((PML & 2) && a.prof[0][1]) ? ld_uniform(a.prof[0][1], jc) : no_pml();
+  const float4 phy = ((PML & 2) && a.prof[1][1]) ? ld_uniform(a.prof[1][1], jc) : no_pml();
+  const float4 phym = ((PML & 2) && a.prof[1][1] && yhalo) ? ld_uniform(a.prof[1][1], j - 1) : no_pml();
+  const int pxE = (int)pex.w, pxH = (int)phx.w, pxl = (int)phxl.w, pyE = (int)pey.w, pyH = (int)phy.w,
+            pym = (int)phym.w;
+  // whether any lane of the tile lies in the x slabs (wave-uniform)
+  const bool xslab = (PML & 1) && __ballot(pxE >= 0 || pxH >= 0) != 0ull;
+
+  // byte offset of a psi value: slab axis 0 by lane, axes 1 and 2 by row / plane
+  auto psi_off = [&](int f, int c, int ax, int p, int k, int row, unsigned lane4) -> unsigned {
+    const unsigned base = a.pso[f][ax][c];
+    if (ax == 0) return base + (((unsigned)k * ny + (unsigned)row) * (unsigned)a.ns[f][0] + (unsigned)p) * 4u;
+    if (ax == 1) return base + ((unsigned)k * (unsigned)a.ns[f][1] + (unsigned)p) * R4 + lane4;
+    return base + ((unsigned)p * ny + (unsigned)row) * R4 + lane4;
   };
-  // h_new = h_old - cH (s_d1 da - s_d2 db) + CPML terms in ascending axis order (the H kernel's arithmetic)
-  auto h_new = [&](int c, float hold, float da, float db, float4 pr1, float4 pr2, float& ps1, float& ps2) -> float {
+  const unsigned ic4 = (unsigned)ic * 4u;
+
+  // h_new = h_old - cH (s_d1 da - s_d2 db) + CPML terms in ascending axis order (the H kernel's arithmetic);
+  // use1 / use2: the term of axis d1 / d2 applies (in its slab)
+  auto h_upd = [&](int c, float hold, float da, float db, float4 pr1, float4 pr2, bool use1, bool use2, float& ps1,
+                   float& ps2) -> float {
     const int d1 = (c + 1) % 3, d2 = (c + 2) % 3;
     float acc = -g.cH * (g.s[d1] * da - g.s[d2] * db);
     if (d1 < d2) {
-      if ((int)pr1.w >= 0) acc += pml_apply(pr1, a.has_k[1][d1], -g.cH * g.s[d1], da, ps1);
-      if ((int)pr2.w >= 0) acc += pml_apply(pr2, a.has_k[1][d2], g.cH * g.s[d2], db, ps2);
+      if (use1) acc += pml_apply(pr1, 1, -g.cH * g.s[d1], da, ps1);
+      if (use2) acc += pml_apply(pr2, 1, g.cH * g.s[d2], db, ps2);
     } else {
-      if ((int)pr2.w >= 0) acc += pml_apply(pr2, a.has_k[1][d2], g.cH * g.s[d2], db, ps2);
-      if ((int)pr1.w >= 0) acc += pml_apply(pr1, a.has_k[1][d1], -g.cH * g.s[d1], da, ps1);
+      if (use2) acc += pml_apply(pr2, 1, g.cH * g.s[d2], db, ps2);
+      if (use1) acc += pml_apply(pr1, 1, -g.cH * g.s[d1], da, ps1);
     }
     return hold + acc;
   };
-  auto ld = [&](const float* F, bool ok, int x, int y, int k) -> float { return ok ? F[cell(x, y, k)] : 0.f; };
 
-  const float* Exi = Ei;
-  const float* Eyi = Ei + N;
-  const float* Ezi = Ei + 2 * N;
   const int k0 = z0 > 0 ? z0 - 1 : z0;
-  // E^n of the row at plane k (carried) and x + 1 halos (lane 63)
-  float ex = ld(Exi, on && k0 < g.n[2], i, j, k0), ey = ld(Eyi, on && k0 < g.n[2], i, j, k0),
-        ez = ld(Ezi, on && k0 < g.n[2], i, j, k0);
-  float eyh = ld(Eyi, hx_ok && k0 < g.n[2], ih, j, k0), ezh = ld(Ezi, hx_ok && k0 < g.n[2], ih, j, k0);
+  // E^n of the row at plane k (carried), and the x + 64 halo column (wave-uniform, carried)
+  float ex, ey, ez, eyr, ezr;
+  {
+    const unsigned o = (unsigned)k0 * P4;
+    ex = ld(Ex, o + ol);
+    ey = ld(Ey, o + ol);
+    ez = ld(Ez, o + ol);
+    eyr = has_r ? ldu(a.E_in + g.N, o + oxr) : 0.f;
+    ezr = has_r ? ldu(a.E_in + 2 * g.N, o + oxr) : 0.f;
+  }
   float hxp = 0.f, hyp = 0.f;  // H_x, H_y of the row at the previous plane
-  // material / chunk bases, prefetched a plane ahead
+  // material and chunk bases of the plane being updated, prefetched one plane ahead
   uint32_t idc = 0u, idn = 0u;
   float cbc[MAT == 0 ? 3 : 1] = {}, cbn[MAT == 0 ? 3 : 1] = {};
   int abc[3] = {0, 0, 0}, abn[3] = {0, 0, 0};
   auto load_mat = [&](int k, uint32_t& id, float* cb, int* ab) {
-    const bool ok = on && k < g.n[2];
-    const unsigned c = cell(ok ? i : 0, ok ? j : 0, ok ? k : 0);
-    if (MAT == 1) id = ok ? ((const uint32_t*)as_global(a.mid))[c] : 0u;
+    const unsigned o = (unsigned)k * P4 + ol;
+    if (MAT == 1) id = *(gptr<const uint32_t>)((gptr<const char>)as_global(a.mid) + o);
     else
-      for (int q = 0; q < (MAT == 0 ? 3 : 1); ++q) cb[q] = ok ? ((const float*)as_global(a.cbd))[(unsigned)q * N + c] : 0.f;
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
-      ab[q] = (NP > 0 && jin && k < g.n[2])
-                  ? ld_uniform(a.chunk_base, (((unsigned)q * g.n[2] + (unsigned)k) * ny + (unsigned)j) * a.nchunk +
-                                                 (unsigned)tx * 2u)
-                  : 0;
+      for (int q = 0; q < 3; ++q) cb[q] = ld(as_global(a.cbd), (unsigned)q * N4 + o);
+    if (NP > 0)
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        ab[q] = ld_uniform(a.chunk_base,
+                           (((unsigned)q * nz + (unsigned)k) * ny + (unsigned)jc) * a.nchunk + (unsigned)tx * 2u);
   };
   load_mat(z0, idc, cbc, abc);
   __syncthreads();  // LDS tables ready
-  Pair<float>* P = (Pair<float>*)as_global((Pair<float>*)a.P);
+  gptr<float> P = as_global((float*)a.P);  // pole pairs {P^n, P^{n-1}}
 
   for (int k = k0; k < z1; ++k) {
     const bool own = k >= z0;
     const int b = k & 1;
-    const bool kin1 = k + 1 < g.n[2];
-    const float4 phz = a.prof[1][2] ? ld_uniform(a.prof[1][2], k) : no_pml();
-    const float4 pez = a.prof[0][2] ? ld_uniform(a.prof[0][2], k) : no_pml();
-    // ---- loads for H(k): E(k + 1) of the row, E(k) of the row above, old H, H-side psi
-    const float exn = ld(Exi, on && kin1, i, j, k + 1), eyn = ld(Eyi, on && kin1, i, j, k + 1),
-                ezn = ld(Ezi, on && kin1, i, j, k + 1);
-    const float eyhn = ld(Eyi, hx_ok && kin1, ih, j, k + 1), ezhn = ld(Ezi, hx_ok && kin1, ih, j, k + 1);
-    const bool up = on && j + 1 < g.n[1];
-    const float exu = ld(Exi, up, i, j + 1, k), ezu = ld(Ezi, up, i, j + 1, k);
-    const unsigned cc = cell(on ? i : 0, on ? j : 0, k);
-    const bool stx = on && in_box(g.rH[0], i, j, k), sty = on && in_box(g.rH[1], i, j, k),
-               stz = on && in_box(g.rH[2], i, j, k);
-    const float hox = stx ? Hi[cc] : 0.f, hoy = sty ? Hi[N + cc] : 0.f, hoz = stz ? Hi[2 * N + cc] : 0.f;
-    float qx0 = stx ? psi_hi(0, 1, (int)phy.w, i, j, k) : 0.f, qx1 = stx ? psi_hi(0, 2, (int)phz.w, i, j, k) : 0.f;
-    float qy0 = sty ? psi_hi(1, 2, (int)phz.w, i, j, k) : 0.f, qy1 = sty ? psi_hi(1, 0, (int)phx.w, i, j, k) : 0.f;
-    float qz0 = stz ? psi_hi(2, 0, (int)phx.w, i, j, k) : 0.f, qz1 = stz ? psi_hi(2, 1, (int)phy.w, i, j, k) : 0.f;
-    // lane 0: the column x - 1 (H_y, H_z)
-    const bool sly = xl_ok && in_box(g.rH[1], il, j, k), slz = xl_ok && in_box(g.rH[2], il, j, k);
-    const float exl = ld(Exi, xl_ok, il, j, k), exln = ld(Exi, xl_ok && kin1, il, j, k + 1);
-    const float exlu = ld(Exi, xl_ok && j + 1 < g.n[1], il, j + 1, k);
-    const float eyl = ld(Eyi, xl_ok, il, j, k), ezl = ld(Ezi, xl_ok, il, j, k);
-    const unsigned lc = cell(xl_ok ? il : 0, xl_ok ? j : 0, k);
-    const float holy = sly ? Hi[N + lc] : 0.f, holz = slz ? Hi[2 * N + lc] : 0.f;
-    float qly0 = sly ? psi_hi(1, 2, (int)phz.w, il, j, k) : 0.f, qly1 = sly ? psi_hi(1, 0, (int)phxl.w, il, j, k) : 0.f;
-    float qlz0 = slz ? psi_hi(2, 0, (int)phxl.w, il, j, k) : 0.f, qlz1 = slz ? psi_hi(2, 1, (int)phy.w, il, j, k) : 0.f;
-    // ---- loads for E(k): old E, E-side psi, pole state, next material
+    const unsigned ok = (unsigned)k * P4;
+    const unsigned o = ok + ol;
+    const unsigned on = (k + 1 < nz ? ok + P4 : ok) + ol;  // plane k + 1 (clamped)
+    const float4 phz = ((PML & 4) && a.prof[1][2]) ? ld_uniform(a.prof[1][2], k) : no_pml();
+    const float4 pez = ((PML & 4) && a.prof[0][2]) ? ld_uniform(a.prof[0][2], k) : no_pml();
+    const int pzH = (int)phz.w, pzE = (int)pez.w;
+    bool bzE[3], bzH[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      bzE[c] = c == 2 ? k < nz - 1 : (k >= 1 && k < nz - 1);
+      bzH[c] = c == 2 || k < nz - 1;
+    }
+    const bool stx = bxH[0] && byH[0] && bzH[0], sty = bxH[1] && byH[1] && bzH[1], stz = bxH[2] && byH[2] && bzH[2];
+
+    // ---- loads for H(k): E(k + 1) of the row, E(k) of the row above, old H
+    const float exn = ld(Ex, on), eyn = ld(Ey, on), ezn = ld(Ez, on);
+    const float exu = ld(Ex, o + dup), ezu = ld(Ez, o + dup);
+    const float hox = ld(Hx, o), hoy = ld(Hy, o), hoz = ld(Hz, o);
+    const unsigned onr = (k + 1 < nz ? ok + P4 : ok) + oxr;
+    const float eyrn = has_r ? ldu(a.E_in + g.N, onr) : 0.f, ezrn = has_r ? ldu(a.E_in + 2 * g.N, onr) : 0.f;
+    // H-side psi (slabs only): [component][term]
+    float qx0 = 0.f, qx1 = 0.f, qy0 = 0.f, qy1 = 0.f, qz0 = 0.f, qz1 = 0.f;
+    if (pyH >= 0) {  // y slab: H_x (d1 = y) and H_z (d2 = y)
+      qx0 = ld(as_global(a.psiHi[1]), psi_off(1, 0, 1, pyH, k, jc, ic4));
+      qz1 = ld(as_global(a.psiHi[1]), psi_off(1, 2, 1, pyH, k, jc, ic4));
+    }
+    if (pzH >= 0) {  // z slab: H_x (d2 = z) and H_y (d1 = z)
+      qx1 = ld(as_global(a.psiHi[2]), psi_off(1, 0, 2, pzH, k, jc, ic4));
+      qy0 = ld(as_global(a.psiHi[2]), psi_off(1, 1, 2, pzH, k, jc, ic4));
+    }
+    if (xslab) {  // x slab: H_y (d2 = x) and H_z (d1 = x), per lane
+      const int p = pxH >= 0 ? pxH : 0;
+      qy1 = ld(as_global(a.psiHi[0]), psi_off(1, 1, 0, p, k, jc, 0u));
+      qz0 = ld(as_global(a.psiHi[0]), psi_off(1, 2, 0, p, k, jc, 0u));
+    }
+    // ---- the column x - 1 (wave-uniform): H_y, H_z there
+    float hly = 0.f, hlz = 0.f;
+    if (has_l) {
+      const unsigned ol0 = ok + oxl;
+      const unsigned oln = (k + 1 < nz ? ok + P4 : ok) + oxl;
+      const float exl = ldu(a.E_in, ol0), exln = ldu(a.E_in, oln), exlu = ldu(a.E_in, ol0 + dup);
+      const float eyl = ldu(a.E_in + g.N, ol0), ezl = ldu(a.E_in + 2 * g.N, ol0);
+      const float holy = ldu(a.H_in + g.N, ol0), holz = ldu(a.H_in + 2 * g.N, ol0);
+      const float ezx0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ez), 0));
+      const float eyx0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ey), 0));
+      const bool sly = lxH1 && bzH[1], slz = lxH2 && bzH[2];
+      const unsigned xl4 = (unsigned)(x0 - 1) * 4u;
+      float ly0 = 0.f, ly1 = 0.f, lz0 = 0.f, lz1 = 0.f;
+      if (pzH >= 0) ly0 = ldu(a.psiHi[2], psi_off(1, 1, 2, pzH, k, jc, xl4));
+      if (pxl >= 0) {
+        ly1 = ldu(a.psiHi[0], psi_off(1, 1, 0, pxl, k, jc, 0u));
+        lz0 = ldu(a.psiHi[0], psi_off(1, 2, 0, pxl, k, jc, 0u));
+      }
+      if (pyH >= 0) lz1 = ldu(a.psiHi[1], psi_off(1, 2, 1, pyH, k, jc, xl4));
+      hly = sly ? h_upd(1, holy, exln - exl, ezx0 - ezl, phz, phxl, pzH >= 0, pxl >= 0, ly0, ly1) : holy;
+      hlz = slz ? h_upd(2, holz, eyx0 - eyl, exlu - exl, phxl, phy, pxl >= 0, pyH >= 0, lz0, lz1) : holz;
+    }
+
+    // ---- loads for E(k): E-side psi, pole state, next material
     bool ste[3];
-    float ev[3], pse[3][2];
-    int aq[3];
+    float pse[3][2] = {};
+    int aq[3] = {-1, -1, -1};
     Pair<float> pp[3][NP > 0 ? NP : 1];
     if (own) {
       if (k + 1 < z1) load_mat(k + 1, idn, cbn, abn);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        ste[c] = on && in_box(g.rE[c], i, j, k);
-        ev[c] = ste[c] ? Ei[(unsigned)c * N + cc] : 0.f;
+        ste[c] = bxE[c] && byE[c] && bzE[c];
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
           const int ax = (c + 1 + s) % 3;
-          const float4 pr = ax == 0 ? pex : (ax == 1 ? pey : pez);
-          const int p = (int)pr.w;
-          pse[c][s] = (ste[c] && p >= 0) ? ((const float*)as_global(a.psiE[ax]))[psi_slot(c, ax) * a.psz[0][ax] +
-                                                                                   psi_index(ax, p, a.ns[0][ax], nx, ny, i, j, k)]
-                                         : 0.f;
+          if (ax == 0 && xslab) pse[c][s] = ld(as_global((const float*)a.psiE[0]),
+                                               psi_off(0, c, 0, pxE >= 0 ? pxE : 0, k, jc, 0u));
+          if (ax == 1 && pyE >= 0) pse[c][s] = ld(as_global((const float*)a.psiE[1]), psi_off(0, c, 1, pyE, k, jc, ic4));
+          if (ax == 2 && pzE >= 0) pse[c][s] = ld(as_global((const float*)a.psiE[2]), psi_off(0, c, 2, pzE, k, jc, ic4));
         }
-        aq[c] = -1;
         if (NP > 0) {
           const float cv = MAT == 1 ? tab[(idc >> (8 * c)) & 255u] : cbc[MAT == 0 ? c : 0];
           const bool dsp = ste[c] && cv < 0.f;
           const unsigned long long m = __ballot(dsp);
-          if (m && dsp) aq[c] = abc[c] + __popcll(m & below);
+          if (m != 0ull && dsp) {  // pole pairs of this lane's node (read only where aq >= 0)
+            aq[c] = abc[c] + __popcll(m & below);
+            const unsigned po = (unsigned)aq[c] * (unsigned)a.max_poles * 8u;
 #pragma unroll
-          for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
-            pp[c][p] = Pair<float>{0.f, 0.f};
-            if (aq[c] >= 0 && p < a.max_poles) pp[c][p] = P[(unsigned)aq[c] * (unsigned)a.max_poles + (unsigned)p];
+            for (int p = 0; p < (NP > 0 ? NP : 1); ++p)
+              if (p < a.max_poles) {
+                pp[c][p].p = ld((gptr<const float>)P, po + 8u * p);
+                pp[c][p].q = ld((gptr<const float>)P, po + 8u * p + 4u);
+              }
           }
         }
       }
     }
 
     // ---- H^{n+1/2}(k) of the row
-    float eyp = shfl_dn(ey), ezp = shfl_dn(ez);
+    float eyp = shfl_dn(ey), ezp = shfl_dn(ez);  // x + 1
     if (lane == 63) {
-      eyp = eyh;
-      ezp = ezh;
+      eyp = eyr;
+      ezp = ezr;
     }
-    const float hx = stx ? h_new(0, hox, ezu - ez, eyn - ey, phy, phz, qx0, qx1) : hox;
-    const float hy = sty ? h_new(1, hoy, exn - ex, ezp - ez, phz, phx, qy0, qy1) : hoy;
-    const float hz = stz ? h_new(2, hoz, eyp - ey, exu - ex, phx, phy, qz0, qz1) : hoz;
-    const float hly = sly ? h_new(1, holy, exln - exl, ez - ezl, phz, phxl, qly0, qly1) : holy;
-    const float hlz = slz ? h_new(2, holz, ey - eyl, exlu - exl, phxl, phy, qlz0, qlz1) : holz;
-    lhx[b][wv + 1][lane] = hx;
-    lhz[b][wv + 1][lane] = hz;
-    if (wv == 0) {
-      // the first wavefront: H_x, H_z of the row below the block
-      float hmx = 0.f, hmz = 0.f;
-      if (yhalo) {
-        const bool ym = inx;
-        const bool smx = ym && in_box(g.rH[0], i, jm, k), smz = ym && in_box(g.rH[2], i, jm, k);
-        const float exm = ld(Exi, ym, i, jm, k), eym = ld(Eyi, ym, i, jm, k), ezm = ld(Ezi, ym, i, jm, k);
-        const float eymn = ld(Eyi, ym && kin1, i, jm, k + 1);
-        const float eymh = ld(Eyi, lane == 63 && ih < g.n[0], ih, jm, k);
-        const unsigned mc = cell(ym ? i : 0, jm, k);
-        const float homx = smx ? Hi[mc] : 0.f, homz = smz ? Hi[2 * N + mc] : 0.f;
-        float qmx0 = smx ? psi_hi(0, 1, (int)phym.w, i, jm, k) : 0.f;
-        float qmx1 = smx ? psi_hi(0, 2, (int)phz.w, i, jm, k) : 0.f;
-        float qmz0 = smz ? psi_hi(2, 0, (int)phx.w, i, jm, k) : 0.f;
-        float qmz1 = smz ? psi_hi(2, 1, (int)phym.w, i, jm, k) : 0.f;
-        float eymp = shfl_dn(eym);
-        if (lane == 63) eymp = eymh;
-        hmx = smx ? h_new(0, homx, ez - ezm, eymn - eym, phym, phz, qmx0, qmx1) : homx;
-        hmz = smz ? h_new(2, homz, eymp - eym, ex - exm, phx, phym, qmz0, qmz1) : homz;
-      }
-      lhx[b][0][lane] = hmx;
-      lhz[b][0][lane] = hmz;
-    }
-    __syncthreads();
+    const float hx = stx ? h_upd(0, hox, ezu - ez, eyn - ey, phy, phz, pyH >= 0, pzH >= 0, qx0, qx1) : hox;
+    const float hy = sty ? h_upd(1, hoy, exn - ex, ezp - ez, phz, phx, pzH >= 0, xslab && pxH >= 0, qy0, qy1) : hoy;
+    const float hz = stz ? h_upd(2, hoz, eyp - ey, exu - ex, phx, phy, xslab && pxH >= 0, pyH >= 0, qz0, qz1) : hoz;
 
     if (own) {
+      lhx[b][wv + 1][lane] = hx;
+      lhz[b][wv + 1][lane] = hz;
+      if (wv == 0) {
+        // the first wavefront: H_x, H_z of the row below the block
+        float hmx = 0.f, hmz = 0.f;
+        if (yhalo) {
+          const unsigned om = ok + olm;
+          const float exm = ld(Ex, om), eym = ld(Ey, om), ezm = ld(Ez, om);
+          const float eymn = ld(Ey, (k + 1 < nz ? ok + P4 : ok) + olm);
+          const float homx = ld(Hx, om), homz = ld(Hz, om);
+          const float eymr = has_r ? ldu(a.E_in + g.N, ok + oxrm) : 0.f;
+          const bool smx = bxH[0] && bzH[0], smz = bxH[2] && bzH[2];
+          float mx0 = 0.f, mx1 = 0.f, mz0 = 0.f, mz1 = 0.f;
+          if (pym >= 0) {
+            mx0 = ld(as_global(a.psiHi[1]), psi_off(1, 0, 1, pym, k, j - 1, ic4));
+            mz1 = ld(as_global(a.psiHi[1]), psi_off(1, 2, 1, pym, k, j - 1, ic4));
+          }
+          if (pzH >= 0) mx1 = ld(as_global(a.psiHi[2]), psi_off(1, 0, 2, pzH, k, j - 1, ic4));
+          if (xslab) mz0 = ld(as_global(a.psiHi[0]), psi_off(1, 2, 0, pxH >= 0 ? pxH : 0, k, j - 1, 0u));
+          float eymp = shfl_dn(eym);
+          if (lane == 63) eymp = eymr;
+          hmx = smx ? h_upd(0, homx, ez - ezm, eymn - eym, phym, phz, pym >= 0, pzH >= 0, mx0, mx1) : homx;
+          hmz = smz ? h_upd(2, homz, eymp - eym, ex - exm, phx, phym, xslab && pxH >= 0, pym >= 0, mz0, mz1) : homz;
+        }
+        lhx[b][0][lane] = hmx;
+        lhz[b][0][lane] = hmz;
+      }
+      __syncthreads();
+
       // ---- E^{n+1}(k) of the row
       const float hxm = lhx[b][wv][lane], hzmy = lhz[b][wv][lane];  // the row below
       float hym = shfl_up(hy), hzm = shfl_up(hz);                    // x - 1
@@ -258,6 +351,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
         hym = hly;
         hzm = hlz;
       }
+      const float eold[3] = {ex, ey, ez};
+      float ev[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         float da, db;
@@ -281,11 +376,12 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
           const int s = q == 0 ? first : 1 - first;
           const int ax = (c + 1 + s) % 3;
           const float4 pr = ax == 0 ? pex : (ax == 1 ? pey : pez);
-          if ((int)pr.w >= 0)
-            dE += pml_apply(pr, a.has_k[0][ax], s == 0 ? cb * g.s[d1] : -cb * g.s[d2], s == 0 ? da : db, pse[c][s]);
+          const bool use = ax == 0 ? (xslab && pxE >= 0) : (ax == 1 ? pyE >= 0 : pzE >= 0);
+          if (use)
+            dE += pml_apply(pr, 1, s == 0 ? cb * g.s[d1] : -cb * g.s[d2], s == 0 ? da : db, pse[c][s]);
         }
-        const float eold = ev[c];
-        float enew = eold + dE;
+        const float eo = eold[c];
+        float enew = eo + dE;
         if (NP > 0 && aq[c] >= 0) {
           const float* mp = med_tab + ((int)(-cbv) - 1) * a.media_stride;
           const int np = (int)mp[1];
@@ -293,54 +389,51 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
 #pragma unroll
           for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
             if (p < np) {
-              const float pn = mp[2 + 3 * p] * pp[c][p].p + mp[3 + 3 * p] * pp[c][p].q + mp[4 + 3 * p] * eold;
+              const float pn = mp[2 + 3 * p] * pp[c][p].p + mp[3 + 3 * p] * pp[c][p].q + mp[4 + 3 * p] * eo;
               dp = dp + (pn - pp[c][p].p);
               pp[c][p].q = pp[c][p].p;
               pp[c][p].p = pn;
             }
           }
-          enew = eold + mp[0] * (dE - dp);
+          enew = eo + mp[0] * (dE - dp);
         }
         ev[c] = enew;
       }
       // ---- stores
-      auto put_psi = [&](int c, int ax, float4 pr, float v) {
-        const int p = (int)pr.w;
-        if (p >= 0)
-          ((float*)as_global(a.psiHo[ax]))[psi_slot(c, ax) * a.psz[1][ax] + psi_index(ax, p, a.ns[1][ax], nx, ny, i, j, k)] = v;
-      };
       if (stx) {
-        Ho[cc] = hx;
-        put_psi(0, 1, phy, qx0);
-        put_psi(0, 2, phz, qx1);
+        st(Ho, o, hx);
+        if (pyH >= 0) st(as_global(a.psiHo[1]), psi_off(1, 0, 1, pyH, k, jc, ic4), qx0);
+        if (pzH >= 0) st(as_global(a.psiHo[2]), psi_off(1, 0, 2, pzH, k, jc, ic4), qx1);
       }
       if (sty) {
-        Ho[N + cc] = hy;
-        put_psi(1, 2, phz, qy0);
-        put_psi(1, 0, phx, qy1);
+        st(Ho, N4 + o, hy);
+        if (pzH >= 0) st(as_global(a.psiHo[2]), psi_off(1, 1, 2, pzH, k, jc, ic4), qy0);
+        if (xslab && pxH >= 0) st(as_global(a.psiHo[0]), psi_off(1, 1, 0, pxH, k, jc, 0u), qy1);
       }
       if (stz) {
-        Ho[2 * N + cc] = hz;
-        put_psi(2, 0, phx, qz0);
-        put_psi(2, 1, phy, qz1);
+        st(Ho, 2 * N4 + o, hz);
+        if (xslab && pxH >= 0) st(as_global(a.psiHo[0]), psi_off(1, 2, 0, pxH, k, jc, 0u), qz0);
+        if (pyH >= 0) st(as_global(a.psiHo[1]), psi_off(1, 2, 1, pyH, k, jc, ic4), qz1);
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         if (!ste[c]) continue;
-        Eo[(unsigned)c * N + cc] = ev[c];
+        st(Eo, (unsigned)c * N4 + o, ev[c]);
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
           const int ax = (c + 1 + s) % 3;
-          const float4 pr = ax == 0 ? pex : (ax == 1 ? pey : pez);
-          const int p = (int)pr.w;
-          if (p >= 0)
-            ((float*)as_global(a.psiE[ax]))[psi_slot(c, ax) * a.psz[0][ax] + psi_index(ax, p, a.ns[0][ax], nx, ny, i, j, k)] =
-                pse[c][s];
+          if (ax == 0 && xslab && pxE >= 0) st(as_global(a.psiE[0]), psi_off(0, c, 0, pxE, k, jc, 0u), pse[c][s]);
+          if (ax == 1 && pyE >= 0) st(as_global(a.psiE[1]), psi_off(0, c, 1, pyE, k, jc, ic4), pse[c][s]);
+          if (ax == 2 && pzE >= 0) st(as_global(a.psiE[2]), psi_off(0, c, 2, pzE, k, jc, ic4), pse[c][s]);
         }
         if (NP > 0 && aq[c] >= 0) {
 #pragma unroll
           for (int p = 0; p < (NP > 0 ? NP : 1); ++p)
-            if (p < a.max_poles) P[(unsigned)aq[c] * (unsigned)a.max_poles + (unsigned)p] = pp[c][p];
+            if (p < a.max_poles) {
+              const unsigned po = ((unsigned)aq[c] * (unsigned)a.max_poles + (unsigned)p) * 8u;
+              st((gptr<float>)P, po, pp[c][p].p);
+              st((gptr<float>)P, po + 4u, pp[c][p].q);
+            }
         }
       }
       idc = idn;
@@ -354,8 +447,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
     ex = exn;
     ey = eyn;
     ez = ezn;
-    eyh = eyhn;
-    ezh = ezhn;
+    eyr = eyrn;
+    ezr = ezrn;
   }
 }
 

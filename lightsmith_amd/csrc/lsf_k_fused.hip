@@ -1,5 +1,5 @@
 // Instantiations of the single-pass step kernel (real fields, no periodic axes, one row per wavefront, 64 x cells):
-// material mode x pole registers x wavefronts per block (4 or 8).
+// material mode x pole registers x wavefronts per block (4 or 8) x CPML class (none, x only, any).
 
 #include "lsf_fused.h"
 #include "lsf_launch.h"
@@ -9,7 +9,16 @@ namespace {
 
 template <int MAT, int W, int NP>
 void go(const FusedLaunch& L, hipStream_t s) {
-  hipLaunchKernelGGL((k_step_fused<MAT, W, NP>), dim3(L.sc.n_pad), dim3(64 * W), 0, s, L.args, L.sc);
+  const int pad[3] = {(L.count[0] + 7) & ~7, (L.count[1] + 7) & ~7, (L.count[2] + 7) & ~7};
+  if (L.count[2])
+    hipLaunchKernelGGL((k_step_fused<MAT, W, NP, 7>), dim3(pad[2]), dim3(64 * W), 0, s, L.args,
+                       Sched{nullptr, nullptr, L.count[2], pad[2]}, L.tiles[2]);
+  if (L.count[1])
+    hipLaunchKernelGGL((k_step_fused<MAT, W, NP, 1>), dim3(pad[1]), dim3(64 * W), 0, s, L.args,
+                       Sched{nullptr, nullptr, L.count[1], pad[1]}, L.tiles[1]);
+  if (L.count[0])
+    hipLaunchKernelGGL((k_step_fused<MAT, W, NP, 0>), dim3(pad[0]), dim3(64 * W), 0, s, L.args,
+                       Sched{nullptr, nullptr, L.count[0], pad[0]}, L.tiles[0]);
 }
 
 template <int MAT, int NP>

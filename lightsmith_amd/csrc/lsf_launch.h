@@ -41,9 +41,12 @@ LSF_E_LAUNCHERS(LSF_DECLARE_E)
 
 // the single-pass step (real fields, no periodic axes, one row per wavefront, 64 x cells): lsf_k_fused.hip
 struct FusedArgs;
+// The tiles are split three ways by the CPML axes they touch (their halos included): none, x only, any other;
+// each class runs its own kernel (the psi paths of absent axes compiled out) over its tile list.
 struct FusedLaunch {
   const FusedArgs* args;  // device, one entry
-  Sched sc;
+  const int* tiles[3];    // device tile lists: no CPML, x CPML only, other
+  int count[3];
   int waves;
   int mat, np;
 };

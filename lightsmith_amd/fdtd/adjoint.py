@@ -198,9 +198,8 @@ class AdjointProblem:
         cb = self.cb_design.to(torch.float64)
         # cb = C / eps  ->  dcb/deps = -cb^2 / C ; deps/drho = eps_hi - eps_lo
         dcb_drho = -(cb * cb) / self.fwd.cdx * (d.eps_hi - d.eps_lo)
-        grad = torch.zeros(d.n_voxels, dtype=torch.float64, device=self._dev)
-        grad.index_add_(0, self._vox, self.g * dcb_drho)
-        return grad
+        # nodes are component-major over the voxels (DesignRegion.nodes): a fixed-order sum, no atomics
+        return (self.g * dcb_drho).view(3, d.n_voxels).sum(0)
 
     def close(self):
         if self._run is not None:

@@ -306,44 +306,70 @@ def test_vertex_tensor_sphere(gpu, bloch):
     assert d < DFT_TOL
 
 
-def _same_state(a, b, label):
+def _state_diff(a, b) -> tuple[float, float]:
+    """(largest relative L2 difference over the state tensors, largest absolute difference)."""
     ta, tb = a.state_tensors(), b.state_tensors()
     assert len(ta) == len(tb)
-    worst = 0.0
+    rel = absd = 0.0
     for x, y in zip(ta, tb):
-        if not torch.equal(x, y):
-            d = (x.double() - y.double()).abs().max().item() if x.dtype.is_floating_point or x.is_complex() else -1
-            worst = max(worst, d)
-    print(f"{label}: single pass vs two kernels, max |diff| {worst:.3e}")
-    assert worst == 0.0
+        assert x.shape == y.shape and x.dtype == y.dtype
+        if torch.equal(x, y):
+            continue
+        d = (x - y).abs().double()
+        absd = max(absd, d.max().item())
+        rel = max(rel, (d.norm() / max(y.abs().double().norm().item(), 1e-300)).item())
+    return rel, absd
 
 
-@pytest.mark.parametrize("material", ["indexed", "dense"])
-def test_single_pass_matches_two_kernels(gpu, material):
-    """The single-pass step (lsf_set_fused) against the two-kernel step on the dispersive CPML box (kappa and alpha,
-    D-form poles, dipole, DFT), eagerly and through graphs with an odd stride: bit-identical state."""
+@pytest.mark.parametrize("material,start", [("indexed", "random"), ("dense", "random"), ("indexed", "rest"),
+                                            ("dense", "rest")])
+def test_single_pass_matches_the_oracle_and_the_two_kernel_step(gpu, material, start):
+    """The single-pass step (lsf_set_fused) on a CPML box (kappa and alpha, dipole, DFT, odd stride) of 136 x 28 x 40
+    cells, whose tiles fall in all three CPML classes (none, x only, other): random fields in a dielectric box, or
+    D-form Lorentz nodes driven from rest by the dipole (the D-form update holds D = eps_inf E + sum P, which the
+    reference's zero D and E_prev match only for fields at rest). Against the FP64 oracle to the parity tolerance;
+    against the two-kernel step to rounding (after the run within 2e-5 relative L2 on every state tensor, the pole
+    state being the most sensitive); eager stepping and graph replay (two graphs for the odd stride) bit-identical."""
     steps = 140
+    kw = dict(shape=(136, 28, 40), n_steps_hint=steps, material=material, kappa_max=2.0, alpha=0.02, stride=7,
+              frac=0.0 if start == "random" else 0.1)
     ns = cases.product_ns()
-    a = cases.dispersive_box(ns, gpu, shape=(70, 36, 33), n_steps_hint=steps, material=material, kappa_max=2.0,
-                             alpha=0.02, fused=True, stride=7)
-    b = cases.dispersive_box(ns, gpu, shape=(70, 36, 33), n_steps_hint=steps, material=material, kappa_max=2.0,
-                             alpha=0.02, fused=True, stride=7)
-    assert a._two_sets
-    cases.random_fields(a, 5)
-    cases.random_fields(b, 5)
-    ref = cases.dispersive_box(ns, gpu, shape=(70, 36, 33), n_steps_hint=steps, material=material, kappa_max=2.0,
-                               alpha=0.02, fused=False, stride=7)
-    assert not ref._two_sets
-    cases.random_fields(ref, 5)
-    for _ in range(steps // a.stride):
+    a = cases.dispersive_box(ns, gpu, fused=True, **kw)
+    b = cases.dispersive_box(ns, gpu, fused=True, **kw)
+    ref = cases.dispersive_box(ns, gpu, fused=False, **kw)
+    o = cases.dispersive_box(cases.oracle_ns(), **{k: v for k, v in kw.items() if k != "material"})
+    assert a._two_sets and not ref._two_sets
+    if start == "random":
+        for e in (a, b, ref, o):
+            cases.random_fields(e, 5)
+    a.steps(1)
+    ref.steps(1)
+    torch.cuda.synchronize()
+    rel1, abs1 = _state_diff(a, ref)
+    a.steps(a.stride - 1)
+    ref.steps(ref.stride - 1)
+    o.chunk()
+    a.sample()
+    ref.sample()
+    for _ in range(steps // a.stride - 1):
         a.chunk()
         ref.chunk()
+        o.chunk()
     torch.cuda.synchronize()
     assert a.parity == (steps % 2)
-    _same_state(a, ref, f"eager, {material}")
     run, used = b.make_runner(True)
+    assert used
     for _ in range(steps // b.stride):
         run()
     b.release()
     torch.cuda.synchronize()
-    _same_state(b, ref, f"graphs, {material}")
+    same = all(torch.equal(x, y) for x, y in zip(a.state_tensors(), b.state_tensors()))
+    rel, absd = _state_diff(a, ref)
+    print(f"single pass ({material}, {start}) vs two kernels: after 1 step rel {rel1:.1e} max {abs1:.1e}; after "
+          f"{steps} steps rel {rel:.1e} max {absd:.1e}; graphs bit-identical to eager: {same}")
+    _compare(o, a, f"single pass ({material}, {start}) vs FP64 oracle, {steps} steps")
+    eo, ho = o.monitors[0].results()
+    ea, ha = a.monitors[0].results()
+    assert cases.rel_l2(ea, eo) < DFT_TOL and cases.rel_l2(ha, ho) < DFT_TOL
+    assert same
+    assert rel1 < 1e-7 and rel < 2e-5
