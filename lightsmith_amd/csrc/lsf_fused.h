@@ -3,13 +3,14 @@
 //
 // The fields are double-buffered: the kernel reads E_in, H_in and writes E_out, H_out (the H-side psi likewise),
 // so a block never reads another block's new values and no store is ordered before a load. A block is 64 x cells
-// by WAVES rows (one row per wavefront) marching up a chunk of z planes. Per plane every wavefront computes
-// H^{n+1/2} of its row and publishes H_x, H_z to LDS (double-buffered by plane, one barrier per plane), then
-// updates E^{n+1} of its row from its own H (the node and, by a lane shift, x - 1), the row below (LDS) and the
-// previous plane (registers). Values owned by neighbouring blocks are recomputed with their owners' arithmetic:
-// the row below the block (first wavefront), the column x - 1 of the tile and the plane below the chunk (a first
-// pass through the march). E-side psi and the pole state are updated in place (one owner each). Per cell the pass
-// moves E and H once in and once out (48 B) plus the material, against 72 B for the two kernels.
+// by WAVES rows (one row per wavefront) marching up a chunk of z planes; it owns the upper WAVES - 1 rows and
+// recomputes the row below them (blocks overlap by one row). Per plane every wavefront computes H^{n+1/2} of its
+// row and publishes H_x, H_z to LDS (double-buffered by plane, one barrier per plane); the owning wavefronts then
+// update E^{n+1} of their rows from their own H (the node and, by a lane shift, x - 1), the row below (LDS) and the
+// previous plane (registers). The column x - 1 of the tile and the plane below the chunk (a first pass through the
+// march) are recomputed too, with their owners' arithmetic. E-side psi and the pole state are updated in place
+// (one owner each). Per cell the pass moves E and H once in and once out (48 B) plus the material, against 72 B for
+// the two kernels.
 //
 // Instruction economy: every lane loads unconditionally from clamped, in-grid addresses given as 32-bit byte
 // offsets from a wave-uniform base (no branches around loads, no 64-bit address arithmetic); values outside a
@@ -59,8 +60,22 @@ __device__ __forceinline__ float ld(gptr<const float> base, unsigned off) {
   return *(gptr<const float>)((gptr<const char>)base + off);
 }
 __device__ __forceinline__ void st(gptr<float> base, unsigned off, float v) { *(gptr<float>)((gptr<char>)base + off) = v; }
-// a wave-uniform load through the scalar cache (the data is read-only for the whole kernel)
-__device__ __forceinline__ float ldu(const float* base, unsigned off) { return *(cfloat_ptr)((const char*)base + off); }
+__device__ __forceinline__ int ldi(gptr<const int> base, unsigned off) {
+  return *(gptr<const int>)((gptr<const char>)base + off);
+}
+// A copy of a wave-uniform constant in a vector register. The kernel keeps its arithmetic constants there: the
+// scalar register file then holds the bases and loop state without spilling, and no constant is reloaded from the
+// argument block inside the march (each reload would wait on the scalar and LDS counter).
+__device__ __forceinline__ float vreg(float x) {
+  float r;
+  asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+__device__ __forceinline__ int vregi(int x) {
+  int r;
+  asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
 
 }  // namespace fused
 
@@ -72,8 +87,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
   using namespace fused;
   __shared__ float tab[MAT == 1 ? 256 : 1];
   __shared__ float med_tab[NP > 0 ? MAX_MEDIA_FLOATS : 1];
-  __shared__ float lhx[2][WAVES + 1][64];  // H_x, H_z of each row per plane parity; [0] is the row below the block
-  __shared__ float lhz[2][WAVES + 1][64];
+  __shared__ float lhx[2][WAVES][64];  // H_x, H_z of each row per plane parity
+  __shared__ float lhz[2][WAVES][64];
   const int tb = xcd_tile(blockIdx.x, sc.n_pad);
   if (tb >= sc.n_tiles) return;  // whole block
   const int t = ld_uniform(tiles, tb);  // logical tile (x fastest, then y, then z chunk)
@@ -91,14 +106,21 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
 
   const Grid& g = a.g;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const float cH = vreg(g.cH), cdx = vreg(g.cdx);
+  const float sv[3] = {vreg(g.s[0]), vreg(g.s[1]), vreg(g.s[2])};
+  const int mpol = vregi(a.max_poles), mstr = vregi(a.media_stride);  // pole pairs per node; floats per medium
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tx = t % a.tiles_x, rest = t / a.tiles_x;
   const int ty = rest % a.tiles_y, tz = rest / a.tiles_y;
   const int x0 = tx * 64;
-  const int j = ty * WAVES + wv;  // this wavefront's row (rows past the grid still take part in the barriers)
-  const bool jin = j < ny;
-  const int jc = jin ? j : ny - 1;  // the row loads read
+  // Wavefront w computes H on row j0 - 1 + w (j0 = ty (WAVES - 1)); wavefronts 1.. also update E there and own the
+  // row. Wavefront 0 recomputes the row below the block (owned by the block below), H only, so that every
+  // wavefront does the same H arithmetic between barriers. Rows outside the grid still take part in the barriers.
+  const int j = ty * (WAVES - 1) + wv - 1;
+  const bool jin = j >= 0 && j < ny;
+  const bool owner = wv >= 1;
+  const int jc = j < 0 ? 0 : (j < ny ? j : ny - 1);  // the row loads read
   const int i = x0 + lane;
   const int ic = i < nx ? i : nx - 1;
   const int z0 = tz * a.zc, z1 = min(z0 + a.zc, nz);
@@ -109,9 +131,6 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
   const bool has_r = x0 + 64 < nx && jin;                       // the column x + 64 exists
   const unsigned oxl = ((unsigned)jc * nx + (unsigned)(has_l ? x0 - 1 : 0)) * 4u;
   const unsigned oxr = ((unsigned)jc * nx + (unsigned)(has_r ? x0 + 64 : 0)) * 4u;
-  const bool yhalo = wv == 0 && jin && j >= 1;  // the first wavefront recomputes the row below the block
-  const unsigned olm = yhalo ? ol - R4 : ol;
-  const unsigned oxrm = yhalo && has_r ? oxr - R4 : oxr;
   const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
 
   gptr<const float> __restrict__ Ex = as_global(a.E_in);
@@ -144,9 +163,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
   const float4 phxl = ((PML & 1) && a.prof[1][0] && has_l) ? ld_uniform(a.prof[1][0], x0 - 1) : no_pml();
   const float4 pey = ((PML & 2) && a.prof[0][1]) ? ld_uniform(a.prof[0][1], jc) : no_pml();
   const float4 phy = ((PML & 2) && a.prof[1][1]) ? ld_uniform(a.prof[1][1], jc) : no_pml();
-  const float4 phym = ((PML & 2) && a.prof[1][1] && yhalo) ? ld_uniform(a.prof[1][1], j - 1) : no_pml();
-  const int pxE = (int)pex.w, pxH = (int)phx.w, pxl = (int)phxl.w, pyE = (int)pey.w, pyH = (int)phy.w,
-            pym = (int)phym.w;
+  const int pxE = (int)pex.w, pxH = (int)phx.w, pxl = (int)phxl.w, pyE = (int)pey.w, pyH = (int)phy.w;
   // whether any lane of the tile lies in the x slabs (wave-uniform)
   const bool xslab = (PML & 1) && __ballot(pxE >= 0 || pxH >= 0) != 0ull;
 
@@ -164,27 +181,41 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
   auto h_upd = [&](int c, float hold, float da, float db, float4 pr1, float4 pr2, bool use1, bool use2, float& ps1,
                    float& ps2) -> float {
     const int d1 = (c + 1) % 3, d2 = (c + 2) % 3;
-    float acc = -g.cH * (g.s[d1] * da - g.s[d2] * db);
+    float acc = -cH * (sv[d1] * da - sv[d2] * db);
     if (d1 < d2) {
-      if (use1) acc += pml_apply(pr1, 1, -g.cH * g.s[d1], da, ps1);
-      if (use2) acc += pml_apply(pr2, 1, g.cH * g.s[d2], db, ps2);
+      if (use1) acc += pml_apply(pr1, 1, -cH * sv[d1], da, ps1);
+      if (use2) acc += pml_apply(pr2, 1, cH * sv[d2], db, ps2);
     } else {
-      if (use2) acc += pml_apply(pr2, 1, g.cH * g.s[d2], db, ps2);
-      if (use1) acc += pml_apply(pr1, 1, -g.cH * g.s[d1], da, ps1);
+      if (use2) acc += pml_apply(pr2, 1, cH * sv[d2], db, ps2);
+      if (use1) acc += pml_apply(pr1, 1, -cH * sv[d1], da, ps1);
     }
     return hold + acc;
   };
 
   const int k0 = z0 > 0 ? z0 - 1 : z0;
-  // E^n of the row at plane k (carried), and the x + 64 halo column (wave-uniform, carried)
-  float ex, ey, ez, eyr, ezr;
+  // The march is software-pipelined: the field values a plane needs (E of the row at k and k + 1, E of the row
+  // above, old H, the x + 64 halo column) are loaded one iteration ahead, so their latency overlaps the previous
+  // plane's arithmetic, barrier and stores.
+  float ex, ey, ez, eyr, ezr;       // E(k) of the row; E_y, E_z of the column x + 64 (wave-uniform)
+  float exn, eyn, ezn, eyrn, ezrn;  // the same at plane k + 1 (clamped)
+  float exu, ezu, hox, hoy, hoz;    // E_x, E_z of the row above at k; old H at k
   {
-    const unsigned o = (unsigned)k0 * P4;
+    const unsigned o = (unsigned)k0 * P4, o1 = (unsigned)min(k0 + 1, nz - 1) * P4;
     ex = ld(Ex, o + ol);
     ey = ld(Ey, o + ol);
     ez = ld(Ez, o + ol);
-    eyr = has_r ? ldu(a.E_in + g.N, o + oxr) : 0.f;
-    ezr = has_r ? ldu(a.E_in + 2 * g.N, o + oxr) : 0.f;
+    exn = ld(Ex, o1 + ol);
+    eyn = ld(Ey, o1 + ol);
+    ezn = ld(Ez, o1 + ol);
+    exu = ld(Ex, o + ol + dup);
+    ezu = ld(Ez, o + ol + dup);
+    hox = ld(Hx, o + ol);
+    hoy = ld(Hy, o + ol);
+    hoz = ld(Hz, o + ol);
+    eyr = ld(Ey, o + oxr);  // all lanes read the same (in-grid) address; used when has_r
+    ezr = ld(Ez, o + oxr);
+    eyrn = ld(Ey, o1 + oxr);
+    ezrn = ld(Ez, o1 + oxr);
   }
   float hxp = 0.f, hyp = 0.f;  // H_x, H_y of the row at the previous plane
   // material and chunk bases of the plane being updated, prefetched one plane ahead
@@ -200,8 +231,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
     if (NP > 0)
 #pragma unroll
       for (int q = 0; q < 3; ++q)
-        ab[q] = ld_uniform(a.chunk_base,
-                           (((unsigned)q * nz + (unsigned)k) * ny + (unsigned)jc) * a.nchunk + (unsigned)tx * 2u);
+        ab[q] = ldi(as_global(a.chunk_base),
+                    ((((unsigned)q * nz + (unsigned)k) * ny + (unsigned)jc) * a.nchunk + (unsigned)tx * 2u) * 4u);
   };
   load_mat(z0, idc, cbc, abc);
   __syncthreads();  // LDS tables ready
@@ -212,7 +243,6 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
     const int b = k & 1;
     const unsigned ok = (unsigned)k * P4;
     const unsigned o = ok + ol;
-    const unsigned on = (k + 1 < nz ? ok + P4 : ok) + ol;  // plane k + 1 (clamped)
     const float4 phz = ((PML & 4) && a.prof[1][2]) ? ld_uniform(a.prof[1][2], k) : no_pml();
     const float4 pez = ((PML & 4) && a.prof[0][2]) ? ld_uniform(a.prof[0][2], k) : no_pml();
     const int pzH = (int)phz.w, pzE = (int)pez.w;
@@ -224,12 +254,6 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
     }
     const bool stx = bxH[0] && byH[0] && bzH[0], sty = bxH[1] && byH[1] && bzH[1], stz = bxH[2] && byH[2] && bzH[2];
 
-    // ---- loads for H(k): E(k + 1) of the row, E(k) of the row above, old H
-    const float exn = ld(Ex, on), eyn = ld(Ey, on), ezn = ld(Ez, on);
-    const float exu = ld(Ex, o + dup), ezu = ld(Ez, o + dup);
-    const float hox = ld(Hx, o), hoy = ld(Hy, o), hoz = ld(Hz, o);
-    const unsigned onr = (k + 1 < nz ? ok + P4 : ok) + oxr;
-    const float eyrn = has_r ? ldu(a.E_in + g.N, onr) : 0.f, ezrn = has_r ? ldu(a.E_in + 2 * g.N, onr) : 0.f;
     // H-side psi (slabs only): [component][term]
     float qx0 = 0.f, qx1 = 0.f, qy0 = 0.f, qy1 = 0.f, qz0 = 0.f, qz1 = 0.f;
     if (pyH >= 0) {  // y slab: H_x (d1 = y) and H_z (d2 = y)
@@ -250,20 +274,20 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
     if (has_l) {
       const unsigned ol0 = ok + oxl;
       const unsigned oln = (k + 1 < nz ? ok + P4 : ok) + oxl;
-      const float exl = ldu(a.E_in, ol0), exln = ldu(a.E_in, oln), exlu = ldu(a.E_in, ol0 + dup);
-      const float eyl = ldu(a.E_in + g.N, ol0), ezl = ldu(a.E_in + 2 * g.N, ol0);
-      const float holy = ldu(a.H_in + g.N, ol0), holz = ldu(a.H_in + 2 * g.N, ol0);
+      const float exl = ld(Ex, ol0), exln = ld(Ex, oln), exlu = ld(Ex, ol0 + dup);
+      const float eyl = ld(Ey, ol0), ezl = ld(Ez, ol0);
+      const float holy = ld(Hy, ol0), holz = ld(Hz, ol0);
       const float ezx0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ez), 0));
       const float eyx0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ey), 0));
       const bool sly = lxH1 && bzH[1], slz = lxH2 && bzH[2];
       const unsigned xl4 = (unsigned)(x0 - 1) * 4u;
       float ly0 = 0.f, ly1 = 0.f, lz0 = 0.f, lz1 = 0.f;
-      if (pzH >= 0) ly0 = ldu(a.psiHi[2], psi_off(1, 1, 2, pzH, k, jc, xl4));
+      if (pzH >= 0) ly0 = ld(as_global(a.psiHi[2]), psi_off(1, 1, 2, pzH, k, jc, xl4));
       if (pxl >= 0) {
-        ly1 = ldu(a.psiHi[0], psi_off(1, 1, 0, pxl, k, jc, 0u));
-        lz0 = ldu(a.psiHi[0], psi_off(1, 2, 0, pxl, k, jc, 0u));
+        ly1 = ld(as_global(a.psiHi[0]), psi_off(1, 1, 0, pxl, k, jc, 0u));
+        lz0 = ld(as_global(a.psiHi[0]), psi_off(1, 2, 0, pxl, k, jc, 0u));
       }
-      if (pyH >= 0) lz1 = ldu(a.psiHi[1], psi_off(1, 2, 1, pyH, k, jc, xl4));
+      if (pyH >= 0) lz1 = ld(as_global(a.psiHi[1]), psi_off(1, 2, 1, pyH, k, jc, xl4));
       hly = sly ? h_upd(1, holy, exln - exl, ezx0 - ezl, phz, phxl, pzH >= 0, pxl >= 0, ly0, ly1) : holy;
       hlz = slz ? h_upd(2, holz, eyx0 - eyl, exlu - exl, phxl, phy, pxl >= 0, pyH >= 0, lz0, lz1) : holz;
     }
@@ -272,8 +296,9 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
     bool ste[3];
     float pse[3][2] = {};
     int aq[3] = {-1, -1, -1};
+    bool anyd[3] = {false, false, false};  // some lane of the row holds a D-form node of the component
     Pair<float> pp[3][NP > 0 ? NP : 1];
-    if (own) {
+    if (own && owner) {
       if (k + 1 < z1) load_mat(k + 1, idn, cbn, abn);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -290,20 +315,29 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
           const float cv = MAT == 1 ? tab[(idc >> (8 * c)) & 255u] : cbc[MAT == 0 ? c : 0];
           const bool dsp = ste[c] && cv < 0.f;
           const unsigned long long m = __ballot(dsp);
-          if (m != 0ull && dsp) {  // pole pairs of this lane's node (read only where aq >= 0)
-            aq[c] = abc[c] + __popcll(m & below);
-            const unsigned po = (unsigned)aq[c] * (unsigned)a.max_poles * 8u;
+          anyd[c] = m != 0ull;
+          if (anyd[c]) {  // every lane loads: its node's pole pairs, or node 0's (in range) where it has none
+            const int slot = abc[c] + __popcll(m & below);
+            aq[c] = dsp ? slot : -1;
+            const unsigned po = (unsigned)(dsp ? slot : 0) * (unsigned)mpol * 8u;
 #pragma unroll
-            for (int p = 0; p < (NP > 0 ? NP : 1); ++p)
-              if (p < a.max_poles) {
-                pp[c][p].p = ld((gptr<const float>)P, po + 8u * p);
-                pp[c][p].q = ld((gptr<const float>)P, po + 8u * p + 4u);
-              }
+            for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
+              const unsigned pq = (unsigned)min(p, mpol - 1) * 8u;
+              pp[c][p].p = ld((gptr<const float>)P, po + pq);
+              pp[c][p].q = ld((gptr<const float>)P, po + pq + 4u);
+            }
           }
         }
       }
     }
 
+    // ---- the next plane's field values (used in the next iteration), issued after this
+    // plane's loads so that waiting for those leaves these in flight
+    const unsigned p1 = (unsigned)min(k + 1, nz - 1) * P4, p2 = (unsigned)min(k + 2, nz - 1) * P4;
+    const float exnn = ld(Ex, p2 + ol), eynn = ld(Ey, p2 + ol), eznn = ld(Ez, p2 + ol);
+    const float exun = ld(Ex, p1 + ol + dup), ezun = ld(Ez, p1 + ol + dup);
+    const float hoxn = ld(Hx, p1 + ol), hoyn = ld(Hy, p1 + ol), hozn = ld(Hz, p1 + ol);
+    const float eyrnn = ld(Ey, p2 + oxr), ezrnn = ld(Ez, p2 + oxr);
     // ---- H^{n+1/2}(k) of the row
     float eyp = shfl_dn(ey), ezp = shfl_dn(ez);  // x + 1
     if (lane == 63) {
@@ -315,37 +349,13 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
     const float hz = stz ? h_upd(2, hoz, eyp - ey, exu - ex, phx, phy, xslab && pxH >= 0, pyH >= 0, qz0, qz1) : hoz;
 
     if (own) {
-      lhx[b][wv + 1][lane] = hx;
-      lhz[b][wv + 1][lane] = hz;
-      if (wv == 0) {
-        // the first wavefront: H_x, H_z of the row below the block
-        float hmx = 0.f, hmz = 0.f;
-        if (yhalo) {
-          const unsigned om = ok + olm;
-          const float exm = ld(Ex, om), eym = ld(Ey, om), ezm = ld(Ez, om);
-          const float eymn = ld(Ey, (k + 1 < nz ? ok + P4 : ok) + olm);
-          const float homx = ld(Hx, om), homz = ld(Hz, om);
-          const float eymr = has_r ? ldu(a.E_in + g.N, ok + oxrm) : 0.f;
-          const bool smx = bxH[0] && bzH[0], smz = bxH[2] && bzH[2];
-          float mx0 = 0.f, mx1 = 0.f, mz0 = 0.f, mz1 = 0.f;
-          if (pym >= 0) {
-            mx0 = ld(as_global(a.psiHi[1]), psi_off(1, 0, 1, pym, k, j - 1, ic4));
-            mz1 = ld(as_global(a.psiHi[1]), psi_off(1, 2, 1, pym, k, j - 1, ic4));
-          }
-          if (pzH >= 0) mx1 = ld(as_global(a.psiHi[2]), psi_off(1, 0, 2, pzH, k, j - 1, ic4));
-          if (xslab) mz0 = ld(as_global(a.psiHi[0]), psi_off(1, 2, 0, pxH >= 0 ? pxH : 0, k, j - 1, 0u));
-          float eymp = shfl_dn(eym);
-          if (lane == 63) eymp = eymr;
-          hmx = smx ? h_upd(0, homx, ez - ezm, eymn - eym, phym, phz, pym >= 0, pzH >= 0, mx0, mx1) : homx;
-          hmz = smz ? h_upd(2, homz, eymp - eym, ex - exm, phx, phym, xslab && pxH >= 0, pym >= 0, mz0, mz1) : homz;
-        }
-        lhx[b][0][lane] = hmx;
-        lhz[b][0][lane] = hmz;
-      }
+      lhx[b][wv][lane] = hx;
+      lhz[b][wv][lane] = hz;
       __syncthreads();
-
+    }
+    if (own && owner) {
       // ---- E^{n+1}(k) of the row
-      const float hxm = lhx[b][wv][lane], hzmy = lhz[b][wv][lane];  // the row below
+      const float hxm = lhx[b][wv - 1][lane], hzmy = lhz[b][wv - 1][lane];  // the row below
       float hym = shfl_up(hy), hzm = shfl_up(hz);                    // x - 1
       if (lane == 0) {
         hym = hly;
@@ -353,6 +363,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
       }
       const float eold[3] = {ex, ey, ez};
       float ev[3];
+      int npc[3] = {0, 0, 0};  // poles of each lane's medium (0 without a D-form node)
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         float da, db;
@@ -368,8 +379,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
         }
         const int d1 = (c + 1) % 3, d2 = (c + 2) % 3;
         const float cbv = MAT == 1 ? tab[(idc >> (8 * c)) & 255u] : cbc[MAT == 0 ? c : 0];
-        const float cb = cbv < 0.f ? g.cdx : cbv;
-        float dE = cb * (g.s[d1] * da - g.s[d2] * db);
+        const float cb = cbv < 0.f ? cdx : cbv;
+        float dE = cb * (sv[d1] * da - sv[d2] * db);
         const int first = d1 < d2 ? 0 : 1;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -378,24 +389,24 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
           const float4 pr = ax == 0 ? pex : (ax == 1 ? pey : pez);
           const bool use = ax == 0 ? (xslab && pxE >= 0) : (ax == 1 ? pyE >= 0 : pzE >= 0);
           if (use)
-            dE += pml_apply(pr, 1, s == 0 ? cb * g.s[d1] : -cb * g.s[d2], s == 0 ? da : db, pse[c][s]);
+            dE += pml_apply(pr, 1, s == 0 ? cb * sv[d1] : -cb * sv[d2], s == 0 ? da : db, pse[c][s]);
         }
         const float eo = eold[c];
         float enew = eo + dE;
-        if (NP > 0 && aq[c] >= 0) {
-          const float* mp = med_tab + ((int)(-cbv) - 1) * a.media_stride;
-          const int np = (int)mp[1];
+        if (NP > 0 && anyd[c]) {  // branch-free over the row: lanes without a D-form node keep enew
+          const bool d = aq[c] >= 0;
+          const float* mp = med_tab + (d ? (int)(-cbv) - 1 : 0) * mstr;
+          npc[c] = d ? (int)mp[1] : 0;
           float dp = 0.f;
 #pragma unroll
           for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
-            if (p < np) {
-              const float pn = mp[2 + 3 * p] * pp[c][p].p + mp[3 + 3 * p] * pp[c][p].q + mp[4 + 3 * p] * eo;
-              dp = dp + (pn - pp[c][p].p);
-              pp[c][p].q = pp[c][p].p;
-              pp[c][p].p = pn;
-            }
+            const bool use = p < npc[c];
+            const float pn = mp[2 + 3 * p] * pp[c][p].p + mp[3 + 3 * p] * pp[c][p].q + mp[4 + 3 * p] * eo;
+            dp = use ? dp + (pn - pp[c][p].p) : dp;
+            pp[c][p].q = use ? pp[c][p].p : pp[c][p].q;
+            pp[c][p].p = use ? pn : pp[c][p].p;
           }
-          enew = eo + mp[0] * (dE - dp);
+          enew = d ? eo + mp[0] * (dE - dp) : enew;
         }
         ev[c] = enew;
       }
@@ -426,11 +437,11 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
           if (ax == 1 && pyE >= 0) st(as_global(a.psiE[1]), psi_off(0, c, 1, pyE, k, jc, ic4), pse[c][s]);
           if (ax == 2 && pzE >= 0) st(as_global(a.psiE[2]), psi_off(0, c, 2, pzE, k, jc, ic4), pse[c][s]);
         }
-        if (NP > 0 && aq[c] >= 0) {
+        if (NP > 0 && aq[c] >= 0) {  // the medium's poles (the others are unchanged)
 #pragma unroll
           for (int p = 0; p < (NP > 0 ? NP : 1); ++p)
-            if (p < a.max_poles) {
-              const unsigned po = ((unsigned)aq[c] * (unsigned)a.max_poles + (unsigned)p) * 8u;
+            if (p < npc[c]) {
+              const unsigned po = ((unsigned)aq[c] * (unsigned)mpol + (unsigned)p) * 8u;
               st((gptr<float>)P, po, pp[c][p].p);
               st((gptr<float>)P, po + 4u, pp[c][p].q);
             }
@@ -447,8 +458,18 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
     ex = exn;
     ey = eyn;
     ez = ezn;
+    exn = exnn;
+    eyn = eynn;
+    ezn = eznn;
+    exu = exun;
+    ezu = ezun;
+    hox = hoxn;
+    hoy = hoyn;
+    hoz = hozn;
     eyr = eyrn;
     ezr = ezrn;
+    eyrn = eyrnn;
+    ezrn = ezrnn;
   }
 }
 
