@@ -101,6 +101,20 @@ __device__ __forceinline__ gptr<T> as_global(T* p) {
   return (gptr<T>)p;
 }
 
+// A copy of a wave-uniform constant in a vector register. The bulk kernels keep their arithmetic constants there:
+// the scalar register file then holds the bases and loop state without spilling, and no constant is reloaded from
+// the argument block inside the z march (each reload would wait on the counter it shares with the LDS).
+__device__ __forceinline__ float vreg(float x) {
+  float r;
+  asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+__device__ __forceinline__ int vregi(int x) {
+  int r;
+  asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 // Blocks are dealt round-robin to the 8 XCDs (each with its own L2). Give each XCD a contiguous range of
 // logical tiles so that tiles sharing halo planes share an L2. The grid is padded to a multiple of 8.
 __device__ __forceinline__ int xcd_tile(int b, int nblocks_padded) {

@@ -63,20 +63,6 @@ __device__ __forceinline__ void st(gptr<float> base, unsigned off, float v) { *(
 __device__ __forceinline__ int ldi(gptr<const int> base, unsigned off) {
   return *(gptr<const int>)((gptr<const char>)base + off);
 }
-// A copy of a wave-uniform constant in a vector register. The kernel keeps its arithmetic constants there: the
-// scalar register file then holds the bases and loop state without spilling, and no constant is reloaded from the
-// argument block inside the march (each reload would wait on the scalar and LDS counter).
-__device__ __forceinline__ float vreg(float x) {
-  float r;
-  asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));
-  return r;
-}
-__device__ __forceinline__ int vregi(int x) {
-  int r;
-  asm("v_mov_b32 %0, %1" : "=v"(r) : "v"(x));
-  return r;
-}
-
 }  // namespace fused
 
 // PML: bit a set when the tiles given may hold CPML cells of axis a (their halos included); the psi paths of the
