@@ -316,7 +316,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_h(const StepArgs* __restr
 // advances its poles from E^n, P^{n+1} = c1 P^n + c2 P^{n-1} + c3 E^n, and takes
 // E^{n+1} = E^n + (dE - sum_p (P_p^{n+1} - P_p^n)) / eps_inf, which is (D^{n+1} - sum P^{n+1}) / eps_inf with
 // D^{n+1} = D^n + dE and D^n = eps_inf E^n + sum P^n (the reference's D-form, without storing D).
-// The H plane and the material of plane k+1 are loaded during iteration k.
+// Iteration k loads the H plane of plane k + 1; without D-form poles also the cell state (old E, psi) of plane
+// k + 1 and the material of plane k + 2, so that the latency of every load overlaps a plane's arithmetic.
 // ---------------------------------------------------------------------------------------------
 template <class T, int MAT, int RY, int WAVES, int XW, bool PER, int NP>
 __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* tab, const float* med_tab) {
@@ -356,10 +357,10 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
   T cHx[RY + 1], cHy[RY], cHz[RY + 1], xHy[RY], xHz[RY];
   T nHx[RY + 1], nHy[RY], nHz[RY + 1], nxHy[RY], nxHz[RY];
   T pHx[RY], pHy[RY];
-  // material of the current / next plane: packed 8-bit ids (MAT 1) or the three coefficients (MAT 0)
-  uint32_t id_c[RY], id_n[RY];
-  float cbd_c[RY][MAT == 0 ? 3 : 1], cbd_n[RY][MAT == 0 ? 3 : 1];
-  int ab_c[RY][3], ab_n[RY][3];  // first D-form slot of each row's x chunk per component
+  // material of planes k, k + 1, k + 2: packed 8-bit ids (MAT 1) or the three coefficients (MAT 0)
+  uint32_t id_c[RY], id_n[RY], id_nn[RY];
+  float cbd_c[RY][MAT == 0 ? 3 : 1], cbd_n[RY][MAT == 0 ? 3 : 1], cbd_nn[RY][MAT == 0 ? 3 : 1];
+  int ab_c[RY][3], ab_n[RY][3], ab_nn[RY][3];  // first D-form slot of each row's x chunk per component
   const unsigned cq = (unsigned)tx * (XW / 32);
   auto load_bases = [&](int k, int (*ab)[3]) {
 #pragma unroll
@@ -376,13 +377,26 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
         ab[r][c] = (XW == 32 && yl) ? vb : va;
       }
   };
-
-  auto load_plane = [&](int k, T* xH, T* yH, T* zH, T* hy, T* hz, uint32_t* ids, float (*cb)[MAT == 0 ? 3 : 1]) {
+  auto load_mat = [&](int k, uint32_t* ids, float (*cb)[MAT == 0 ? 3 : 1]) {
+    const uint32_t* __restrict__ mid = (const uint32_t*)midg;
+    const float* __restrict__ cbd = (const float*)cbdg;
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int j = j0 + r;
+      const bool ok = inx && j < g.n[1] && k < g.n[2];
+      const unsigned cell = cell32(nx, ny, i, ok ? j : 0, ok ? k : 0);
+      if (MAT == 1) {
+        ids[r] = ok ? mid[cell] : 0u;
+      } else {
+#pragma unroll
+        for (int c = 0; c < (MAT == 0 ? 3 : 1); ++c) cb[r][c] = ok ? cbd[(unsigned)c * N + cell] : 0.f;
+      }
+    }
+  };
+  auto load_h = [&](int k, T* xH, T* yH, T* zH, T* hy, T* hz) {
     const T* __restrict__ Hx = (const T*)Hg;
     const T* __restrict__ Hy = Hx + N;
     const T* __restrict__ Hz = Hx + 2 * N;
-    const uint32_t* __restrict__ mid = (const uint32_t*)midg;
-    const float* __restrict__ cbd = (const float*)cbdg;
 #pragma unroll
     for (int r = -1; r < RY; ++r) {
       const int j = j0 + r;
@@ -394,19 +408,61 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
         const bool lok = xl_ok && j < g.n[1] && k < g.n[2];
         hy[r] = lok ? load_lo<PER>(Hy, g, il, j, k) : zero<T>();
         hz[r] = lok ? load_lo<PER>(Hz, g, il, j, k) : zero<T>();
-        const unsigned cell = cell32(nx, ny, i, ok ? j : 0, ok ? k : 0);
-        if (MAT == 1) {
-          ids[r] = ok ? mid[cell] : 0u;
-        } else {
-#pragma unroll
-          for (int c = 0; c < (MAT == 0 ? 3 : 1); ++c) cb[r][c] = ok ? cbd[(unsigned)c * N + cell] : 0.f;
-        }
       }
     }
   };
-  auto coef = [&](int r, int c) -> float {
-    if (MAT == 1) return tab[(id_c[r] >> (8 * c)) & 255u];
-    return cbd_c[r][MAT == 0 ? c : 0];
+  auto coef_of = [&](const uint32_t* ids, const float (*cb)[MAT == 0 ? 3 : 1], int r, int c) -> float {
+    if (MAT == 1) return tab[(ids[r] >> (8 * c)) & 255u];
+    return cb[r][MAT == 0 ? c : 0];
+  };
+  auto coef = [&](int r, int c) -> float { return coef_of(id_c, cbd_c, r, c); };
+
+  const gptr<Pair<T>> Pg = as_global((Pair<T>*)a.P);
+  // The cell state of a plane: stepped flags, old E, E-side psi, D-form slots and pole pairs. It is loaded one
+  // plane ahead (from the material loaded two planes ahead), so its latency overlaps a plane's arithmetic.
+  bool st[RY][3], st_n[RY][3];
+  T ev[RY][3], ev_n[RY][3];
+  T ps[RY][3][2], ps_n[RY][3][2];
+  int aq[RY][3], aq_n[RY][3];
+  Pair<T> pp[RY][3][NP > 0 ? NP : 1], pp_n[RY][3][NP > 0 ? NP : 1];
+  auto load_cells = [&](int k, const uint32_t* ids, const float (*cb)[MAT == 0 ? 3 : 1], const int (*ab)[3],
+                        bool (*stq)[3], T (*evq)[3], T (*psq)[3][2], int (*aqq)[3],
+                        Pair<T> (*ppq)[3][NP > 0 ? NP : 1]) {
+    const T* __restrict__ E = (const T*)Eg;
+    const Pair<T>* __restrict__ P = (const Pair<T>*)Pg;
+    const float4 pz = (a.pml[2].prof && k < g.n[2]) ? ld_uniform(a.pml[2].prof, k) : no_pml();
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const int j = j0 + r;
+      const unsigned cell = cell32(nx, ny, i, j, k);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        stq[r][c] = inx && j < g.n[1] && k < g.n[2] && in_box(g.rE[c], i, j, k);
+        evq[r][c] = stq[r][c] ? E[(unsigned)c * N + cell] : zero<T>();
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const int ax = (c + 1 + s) % 3;
+          const float4 pr = ax == 0 ? px : (ax == 1 ? py[r] : pz);
+          const int p = (int)pr.w;
+          psq[r][c][s] = zero<T>();
+          if (stq[r][c] && p >= 0)
+            psq[r][c][s] = ((const T*)psi[ax][c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)];
+        }
+        aqq[r][c] = -1;
+        if (NP > 0) {
+          // D-form node: its slot in the compact pole arrays (chunk base + rank among its row's lanes)
+          const bool dsp = stq[r][c] && coef_of(ids, cb, r, c) < 0.f;
+          const unsigned long long m = __ballot(dsp);
+          if (m && dsp) aqq[r][c] = ab[r][c] + __popcll(m & below);
+#pragma unroll
+          for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
+            ppq[r][c][p] = Pair<T>{zero<T>(), zero<T>()};
+            if (aqq[r][c] >= 0 && p < a.max_poles)
+              ppq[r][c][p] = P[(unsigned)aqq[r][c] * (unsigned)a.max_poles + (unsigned)p];
+          }
+        }
+      }
+    }
   };
 
 #pragma unroll
@@ -417,55 +473,39 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
     pHx[r] = ok ? load_lo<PER>(Hx, g, i, j, k) : zero<T>();
     pHy[r] = ok ? load_lo<PER>(Hx + N, g, i, j, k) : zero<T>();
   }
-  load_plane(z0, cHx, cHy, cHz, xHy, xHz, id_c, cbd_c);
+  // Without D-form nodes the cell state is loaded one plane ahead; with them the pole pairs of two planes would
+  // cost more registers (occupancy) than the earlier loads gain, so the cell state of plane k is loaded in
+  // iteration k, behind the prefetched H plane and material.
+  constexpr bool AHEAD = NP == 0;
+  load_h(z0, cHx, cHy, cHz, xHy, xHz);
+  load_mat(z0, id_c, cbd_c);
   load_bases(z0, ab_c);
+  if (AHEAD) {
+    load_mat(z0 + 1, id_n, cbd_n);
+    load_bases(z0 + 1, ab_n);
+    load_cells(z0, id_c, cbd_c, ab_c, st, ev, ps, aq, pp);
+  }
 
-  const gptr<Pair<T>> Pg = as_global((Pair<T>*)a.P);
   for (int k = z0; k < z1; ++k) {
     T* __restrict__ E = (T*)Eg;
     Pair<T>* __restrict__ P = (Pair<T>*)Pg;
     const float4 pz = a.pml[2].prof ? ld_uniform(a.pml[2].prof, k) : no_pml();
-    // ---- loads: next H plane and material (prefetch), old E, psi, pole state
+    // ---- loads: H of plane k + 1; the cells of k + 1 and the material of k + 2 (AHEAD), or the material of k + 1
+    // and the cells of k
     if (k + 1 < z1) {
-      load_plane(k + 1, nHx, nHy, nHz, nxHy, nxHz, id_n, cbd_n);
-      load_bases(k + 1, ab_n);
-    }
-    bool st[RY][3];
-    T ev[RY][3];
-    T ps[RY][3][2];
-    int aq[RY][3];
-    Pair<T> pp[RY][3][NP > 0 ? NP : 1];
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      const int j = j0 + r;
-      const unsigned cell = cell32(nx, ny, i, j, k);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        st[r][c] = inx && j < g.n[1] && in_box(g.rE[c], i, j, k);
-        ev[r][c] = st[r][c] ? E[(unsigned)c * N + cell] : zero<T>();
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          const int ax = (c + 1 + s) % 3;
-          const float4 pr = ax == 0 ? px : (ax == 1 ? py[r] : pz);
-          const int p = (int)pr.w;
-          ps[r][c][s] = zero<T>();
-          if (st[r][c] && p >= 0)
-            ps[r][c][s] = ((const T*)psi[ax][c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)];
+      load_h(k + 1, nHx, nHy, nHz, nxHy, nxHz);
+      if (AHEAD) {
+        load_cells(k + 1, id_n, cbd_n, ab_n, st_n, ev_n, ps_n, aq_n, pp_n);
+        if (k + 2 < z1) {
+          load_mat(k + 2, id_nn, cbd_nn);
+          load_bases(k + 2, ab_nn);
         }
-        aq[r][c] = -1;
-        if (NP > 0) {
-          // D-form node: its slot in the compact pole arrays (chunk base + rank among its row's lanes)
-          const bool dsp = st[r][c] && coef(r, c) < 0.f;
-          const unsigned long long m = __ballot(dsp);
-          if (m && dsp) aq[r][c] = ab_c[r][c] + __popcll(m & below);
-#pragma unroll
-          for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
-            pp[r][c][p] = Pair<T>{zero<T>(), zero<T>()};
-            if (aq[r][c] >= 0 && p < a.max_poles) pp[r][c][p] = P[(unsigned)aq[r][c] * (unsigned)a.max_poles + (unsigned)p];
-          }
-        }
+      } else {
+        load_mat(k + 1, id_n, cbd_n);
+        load_bases(k + 1, ab_n);
       }
     }
+    if (!AHEAD) load_cells(k, id_c, cbd_c, ab_c, st, ev, ps, aq, pp);
     // ---- compute
     T hzm[RY], hym[RY];
 #pragma unroll
@@ -550,15 +590,31 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
         }
       }
     }
+    // ---- the next plane becomes the current one
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       pHx[r] = cHx[r + 1];
       pHy[r] = cHy[r];
       id_c[r] = id_n[r];
+      if (AHEAD) id_n[r] = id_nn[r];
 #pragma unroll
-      for (int c = 0; c < (MAT == 0 ? 3 : 1); ++c) cbd_c[r][c] = cbd_n[r][c];
+      for (int c = 0; c < (MAT == 0 ? 3 : 1); ++c) {
+        cbd_c[r][c] = cbd_n[r][c];
+        if (AHEAD) cbd_n[r][c] = cbd_nn[r][c];
+      }
 #pragma unroll
-      for (int c = 0; c < 3; ++c) ab_c[r][c] = ab_n[r][c];
+      for (int c = 0; c < 3; ++c) {
+        ab_c[r][c] = ab_n[r][c];
+        if (!AHEAD) continue;
+        ab_n[r][c] = ab_nn[r][c];
+        st[r][c] = st_n[r][c];
+        ev[r][c] = ev_n[r][c];
+        ps[r][c][0] = ps_n[r][c][0];
+        ps[r][c][1] = ps_n[r][c][1];
+        aq[r][c] = aq_n[r][c];
+#pragma unroll
+        for (int p = 0; p < (NP > 0 ? NP : 1); ++p) pp[r][c][p] = pp_n[r][c][p];
+      }
     }
 #pragma unroll
     for (int r = 0; r <= RY; ++r) {
