@@ -259,7 +259,7 @@ class Engine:
 
     # launch tiling (rows per lane, wavefronts per block, x cells per wavefront row, z planes per block);
     # see lsf_set_tiling
-    tiling = (1, 8, 64, 16)
+    tiling = (1, 4, 64, 8)
     # default of the `fused` argument: step eligible engines (real fields, no periodic axes, one row per lane,
     # 64-cell rows, no H-phase operations) with the single-pass kernel over a second buffer set
     single_pass = False
