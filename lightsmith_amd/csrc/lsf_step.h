@@ -116,6 +116,35 @@ __device__ __forceinline__ bool in_box(const int (&r)[3][2], int i, int j, int k
   return i >= r[0][0] && i < r[0][1] && j >= r[1][0] && j < r[1][1] && k >= r[2][0] && k < r[2][1];
 }
 
+// The stepped boxes follow from the grid and its periodic axes (engine.py:47-66, no mirror planes; the host builds
+// Grid::rE/rH the same way): a periodic axis steps [1, n - 1); otherwise E_c spans [0, n - 1) along c and
+// [1, n - 1) across, H_c [0, n) along c and [0, n - 1) across. Computing them here keeps the 36 bounds out of the
+// scalar registers. PER false: no axis is periodic. Indices are >= 0.
+template <bool PER>
+__device__ __forceinline__ bool e_box(const Grid& g, int c, int i, int j, int k) {
+  const int v[3] = {i, j, k};
+  bool ok = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int n = g.n[a];
+    const bool per = PER && g.per[a];
+    ok = ok && (per || a != c ? (v[a] >= 1 && v[a] < n - 1) : v[a] < n - 1);
+  }
+  return ok;
+}
+template <bool PER>
+__device__ __forceinline__ bool h_box(const Grid& g, int c, int i, int j, int k) {
+  const int v[3] = {i, j, k};
+  bool ok = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int n = g.n[a];
+    const bool per = PER && g.per[a];
+    ok = ok && (per ? (v[a] >= 1 && v[a] < n - 1) : (a == c ? v[a] < n : v[a] < n - 1));
+  }
+  return ok;
+}
+
 // The y profile of a lane's row r: rows jw + r (lanes of the first x row) or jw + RY + r (the second x row
 // when XW = 32). Both are wavefront-uniform loads.
 template <int XW, int RY>
@@ -210,7 +239,7 @@ __device__ __forceinline__ void h_tile(const StepArgs& a, int t) {
       const unsigned cell = cell32(nx, ny, i, j, k);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        st[r][c] = inx && j < g.n[1] && in_box(g.rH[c], i, j, k);
+        st[r][c] = inx && j < g.n[1] && h_box<PER>(g, c, i, j, k);
         hv[r][c] = st[r][c] ? H[(unsigned)c * N + cell] : zero<T>();
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
@@ -437,7 +466,7 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
       const unsigned cell = cell32(nx, ny, i, j, k);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        stq[r][c] = inx && j < g.n[1] && k < g.n[2] && in_box(g.rE[c], i, j, k);
+        stq[r][c] = inx && j < g.n[1] && k < g.n[2] && e_box<PER>(g, c, i, j, k);
         evq[r][c] = stq[r][c] ? E[(unsigned)c * N + cell] : zero<T>();
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
