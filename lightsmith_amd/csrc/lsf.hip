@@ -712,7 +712,8 @@ int lsf_set_ade(lsf_engine* e, const int32_t* chunk_base, void* P, int64_t n, in
   if (max_poles < 0 || max_poles > lsf::MAX_POLES) return fail(LSF_EINVAL, "pure D-form media may have at most 4 poles");
   if (n > 0 && max_poles < 1) return fail(LSF_EINVAL, "D-form nodes need at least one pole");
   if (n >= (1ll << 30)) return fail(LSF_EINVAL, "too many D-form nodes for 32-bit indexing");
-  if (n > 0 && n_media * (2 + 3 * max_poles) > lsf::MAX_MEDIA_FLOATS)
+  if (n > 0 && (n_media * (2 + 3 * max_poles) > lsf::MAX_MEDIA_FLOATS ||
+                n_media * ((max_poles <= 2 ? 2 : max_poles) + 1) > lsf::MAX_MEDIA_VEC4))
     return fail(LSF_EINVAL, "too many D-form media for the E kernel's table (at most 16 media of 4 poles)");
   e->chunk_base = (const int*)chunk_base;
   e->P = P;

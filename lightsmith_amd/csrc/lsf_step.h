@@ -55,8 +55,12 @@ struct Sched {
   int n_pad;           // n_tiles rounded up to a multiple of 8 (the period of the XCD-aware order)
 };
 
-// D-form media table held in LDS by the E kernel: n_media * (2 + 3 * max_poles) floats
+// D-form media table in device memory: n_media * (2 + 3 * max_poles) floats
+// ({1/eps_inf, poles, c1 c2 c3 of each pole}, zeros past a medium's poles)
 constexpr int MAX_MEDIA_FLOATS = 16 * (2 + 3 * 4);
+// The bulk E kernel's LDS copy: NP + 1 float4 per medium, {1/eps_inf, poles, 0, 0} then {c1, c2, c3, 0} per
+// pole, zeros past max_poles. Every table that MAX_MEDIA_FLOATS admits fits (44 media of 1 pole, NP = 2: 132).
+constexpr int MAX_MEDIA_VEC4 = 132;
 
 template <class T>
 struct Pair {
@@ -349,7 +353,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_h(const StepArgs* __restr
 // k + 1 and the material of plane k + 2, so that the latency of every load overlaps a plane's arithmetic.
 // ---------------------------------------------------------------------------------------------
 template <class T, int MAT, int RY, int WAVES, int XW, bool PER, int NP>
-__device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* tab, const float* med_tab) {
+__device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* tab, const float4* med_tab) {
   const Grid& g = a.g;
   const int lane = threadIdx.x & 63;
   const int xl = lane & (XW - 1);
@@ -396,12 +400,15 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
     for (int r = 0; r < RY; ++r)
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
+        // vector loads (the index is moved to a VGPR): they join the plane's other loads instead of
+        // waiting one by one in the scalar cache
         int va = 0, vb = 0;
         if (NP > 0 && k < g.n[2]) {
+          const int* __restrict__ cbase = (const int*)as_global(a.chunk_base);
           const unsigned rowc = ((unsigned)c * g.n[2] + (unsigned)k) * ny;
-          if (jw + r < g.n[1]) va = ld_uniform(a.chunk_base, (rowc + (unsigned)(jw + r)) * a.nchunk + cq);
+          if (jw + r < g.n[1]) va = cbase[vregi((int)((rowc + (unsigned)(jw + r)) * a.nchunk + cq))];
           if (XW == 32 && jw + RY + r < g.n[1])
-            vb = ld_uniform(a.chunk_base, (rowc + (unsigned)(jw + RY + r)) * a.nchunk + cq);
+            vb = cbase[vregi((int)((rowc + (unsigned)(jw + RY + r)) * a.nchunk + cq))];
         }
         ab[r][c] = (XW == 32 && yl) ? vb : va;
       }
@@ -482,7 +489,7 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
           // D-form node: its slot in the compact pole arrays (chunk base + rank among its row's lanes)
           const bool dsp = stq[r][c] && coef_of(ids, cb, r, c) < 0.f;
           const unsigned long long m = __ballot(dsp);
-          if (m && dsp) aqq[r][c] = ab[r][c] + __popcll(m & below);
+          if (dsp) aqq[r][c] = ab[r][c] + __popcll(m & below);
 #pragma unroll
           for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
             ppq[r][c][p] = Pair<T>{zero<T>(), zero<T>()};
@@ -577,21 +584,22 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
         }
         const T eold = ev[r][c];
         T enew = add(eold, dE);
-        if (NP > 0 && aq[r][c] >= 0) {
-          const float* mp = med_tab + ((int)(-cbv) - 1) * a.media_stride;
-          const int np = (int)mp[1];
+        if (NP > 0) {
+          // branch-free over the wavefront: every lane advances NP poles with its medium's coefficients (medium 0
+          // for lanes without a D-form node, whose registers are then discarded); a pole past the medium's count
+          // has zero coefficients and stays zero, adding exactly 0 to dp
+          const bool d = aq[r][c] >= 0;
+          const float4* mp = med_tab + (d ? (int)(-cbv) - 1 : 0) * (NP + 1);
           T dp = zero<T>();  // sum of the poles' increments
 #pragma unroll
           for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
-            if (p < np) {
-              const T pn = add(add(scl(mp[2 + 3 * p], pp[r][c][p].p), scl(mp[3 + 3 * p], pp[r][c][p].q)),
-                               scl(mp[4 + 3 * p], eold));
-              dp = add(dp, sub(pn, pp[r][c][p].p));
-              pp[r][c][p].q = pp[r][c][p].p;
-              pp[r][c][p].p = pn;
-            }
+            const float4 q = mp[1 + p];
+            const T pn = add(add(scl(q.x, pp[r][c][p].p), scl(q.y, pp[r][c][p].q)), scl(q.z, eold));
+            dp = add(dp, sub(pn, pp[r][c][p].p));
+            pp[r][c][p].q = pp[r][c][p].p;
+            pp[r][c][p].p = pn;
           }
-          enew = add(eold, scl(mp[0], sub(dE, dp)));
+          if (d) enew = add(eold, scl(mp[0].x, sub(dE, dp)));
         }
         ev[r][c] = enew;
       }
@@ -663,7 +671,7 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
 template <class T, int MAT, int RY, int WAVES, int XW, bool PER, int NP, bool LOOP>
 __global__ void __launch_bounds__(64 * WAVES) k_update_e(const StepArgs* __restrict__ args, Sched sc) {
   __shared__ float tab[MAT == 1 ? 256 : 1];
-  __shared__ float med_tab[NP > 0 ? MAX_MEDIA_FLOATS : 1];
+  __shared__ float4 med_tab[NP > 0 ? MAX_MEDIA_VEC4 : 1];
   int cur = -1;
   for (int b = blockIdx.x; b < sc.n_pad; b += gridDim.x) {
     const int t = xcd_tile(b, sc.n_pad);
@@ -684,9 +692,20 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(const StepArgs* __restr
       if (cur >= 0) __syncthreads();  // the previous member's readers are done
       if (MAT == 1)
         for (int q = threadIdx.x; q < 256; q += 64 * WAVES) tab[q] = ((const float*)as_global(a.mtab))[q];
-      if (NP > 0)
-        for (int q = threadIdx.x; q < a.n_media_floats; q += 64 * WAVES)
-          med_tab[q] = ((const float*)as_global(a.media))[q];
+      if (NP > 0) {
+        const float* src = (const float*)as_global(a.media);
+        const int per = NP + 1, n = a.n_media_floats / a.media_stride;
+        for (int q = threadIdx.x; q < n * per; q += 64 * WAVES) {
+          const float* s = src + (q / per) * a.media_stride;
+          const int p = q % per - 1;
+          float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (p < 0)
+            v = make_float4(s[0], s[1], 0.f, 0.f);
+          else if (p < a.max_poles)
+            v = make_float4(s[2 + 3 * p], s[3 + 3 * p], s[4 + 3 * p], 0.f);
+          med_tab[q] = v;
+        }
+      }
       __syncthreads();
       cur = m;
     }
