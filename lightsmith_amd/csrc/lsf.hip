@@ -195,11 +195,9 @@ StepArgs make_args(const lsf_engine* e, int field, const Tiling& t) {
 
 bool any_periodic(const lsf_engine* e) { return e->g.per[0] || e->g.per[1] || e->g.per[2]; }
 
-// pole registers of the E kernel instantiation for this engine
-int pole_regs(const lsf_engine* e) {
-  if (e->n_ade <= 0) return 0;
-  return e->max_poles <= 2 ? 2 : (e->max_poles == 3 ? 3 : 4);
-}
+// pole registers of the E kernel instantiation for this engine: its max_poles (the slot stride of the pole
+// arrays); the single-pass kernel rounds 1 up to 2
+int pole_regs(const lsf_engine* e) { return e->n_ade > 0 ? e->max_poles : 0; }
 
 void set_parity(lsf_engine* e, int p) {
   e->parity = p;
@@ -713,7 +711,7 @@ int lsf_set_ade(lsf_engine* e, const int32_t* chunk_base, void* P, int64_t n, in
   if (n > 0 && max_poles < 1) return fail(LSF_EINVAL, "D-form nodes need at least one pole");
   if (n >= (1ll << 30)) return fail(LSF_EINVAL, "too many D-form nodes for 32-bit indexing");
   if (n > 0 && (n_media * (2 + 3 * max_poles) > lsf::MAX_MEDIA_FLOATS ||
-                n_media * ((max_poles <= 2 ? 2 : max_poles) + 1) > lsf::MAX_MEDIA_VEC4))
+                n_media * (max_poles + 1) > lsf::MAX_MEDIA_VEC4))
     return fail(LSF_EINVAL, "too many D-form media for the E kernel's table (at most 16 media of 4 poles)");
   e->chunk_base = (const int*)chunk_base;
   e->P = P;

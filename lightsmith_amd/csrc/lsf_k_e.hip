@@ -39,6 +39,7 @@ void by_shape(const KLaunch& L, hipStream_t s) {
 void LSF_E_ENTRY(const KLaunch& L, hipStream_t s) {
   switch (L.np) {
     case 0: return by_shape<0>(L, s);
+    case 1: return by_shape<1>(L, s);
     case 2: return by_shape<2>(L, s);
     case 3: return by_shape<3>(L, s);
     default: return by_shape<4>(L, s);

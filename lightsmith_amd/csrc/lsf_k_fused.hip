@@ -31,6 +31,7 @@ template <int MAT>
 void by_poles(const FusedLaunch& L, hipStream_t s) {
   switch (L.np) {
     case 0: return by_waves<MAT, 0>(L, s);
+    case 1:
     case 2: return by_waves<MAT, 2>(L, s);
     case 3: return by_waves<MAT, 3>(L, s);
     default: return by_waves<MAT, 4>(L, s);

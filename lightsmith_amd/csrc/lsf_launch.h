@@ -17,7 +17,7 @@ struct KLaunch {
   int cplx;              // complex64 fields
   int per;               // some axis periodic
   int mat;               // E: 1 = packed material ids, 0 = dense coefficients
-  int np;                // E: pole registers (0, 2, 3 or 4)
+  int np;                // E: pole registers (0 or the engine's max_poles, 1 to 4)
 };
 
 // The tile shapes (rows per lane, wavefronts per block, x cells per row) with compiled kernels.

@@ -59,8 +59,8 @@ struct Sched {
 // ({1/eps_inf, poles, c1 c2 c3 of each pole}, zeros past a medium's poles)
 constexpr int MAX_MEDIA_FLOATS = 16 * (2 + 3 * 4);
 // The bulk E kernel's LDS copy: NP + 1 float4 per medium, {1/eps_inf, poles, 0, 0} then {c1, c2, c3, 0} per
-// pole, zeros past max_poles. Every table that MAX_MEDIA_FLOATS admits fits (44 media of 1 pole, NP = 2: 132).
-constexpr int MAX_MEDIA_VEC4 = 132;
+// pole. Every table that MAX_MEDIA_FLOATS admits fits (44 media of 1 pole: 88 float4).
+constexpr int MAX_MEDIA_VEC4 = 88;
 
 template <class T>
 struct Pair {
@@ -394,23 +394,20 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
   uint32_t id_c[RY], id_n[RY], id_nn[RY];
   float cbd_c[RY][MAT == 0 ? 3 : 1], cbd_n[RY][MAT == 0 ? 3 : 1], cbd_nn[RY][MAT == 0 ? 3 : 1];
   int ab_c[RY][3], ab_n[RY][3], ab_nn[RY][3];  // first D-form slot of each row's x chunk per component
-  const unsigned cq = (unsigned)tx * (XW / 32);
+  // first D-form slot of each row's 32-cell x chunk per component: chunk_base[((c n2 + k) ny + j) nchunk + x / 32],
+  // one vector load per row and component (the index sits in a VGPR), so that these loads join the plane's other
+  // loads instead of waiting one by one in the scalar cache
+  const int* __restrict__ cbase = (const int*)as_global(a.chunk_base);
+  const unsigned cb_k = ny * (unsigned)a.nchunk;  // per plane
+  const unsigned cb_j = (unsigned)vregi((int)((unsigned)j0 * (unsigned)a.nchunk + (unsigned)tx * (XW / 32)));
   auto load_bases = [&](int k, int (*ab)[3]) {
 #pragma unroll
     for (int r = 0; r < RY; ++r)
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        // vector loads (the index is moved to a VGPR): they join the plane's other loads instead of
-        // waiting one by one in the scalar cache
-        int va = 0, vb = 0;
-        if (NP > 0 && k < g.n[2]) {
-          const int* __restrict__ cbase = (const int*)as_global(a.chunk_base);
-          const unsigned rowc = ((unsigned)c * g.n[2] + (unsigned)k) * ny;
-          if (jw + r < g.n[1]) va = cbase[vregi((int)((rowc + (unsigned)(jw + r)) * a.nchunk + cq))];
-          if (XW == 32 && jw + RY + r < g.n[1])
-            vb = cbase[vregi((int)((rowc + (unsigned)(jw + RY + r)) * a.nchunk + cq))];
-        }
-        ab[r][c] = (XW == 32 && yl) ? vb : va;
+        ab[r][c] = 0;
+        if (NP > 0 && k < g.n[2] && j0 + r < g.n[1])
+          ab[r][c] = cbase[cb_j + (unsigned)r * (unsigned)a.nchunk + ((unsigned)c * g.n[2] + (unsigned)k) * cb_k];
       }
   };
   auto load_mat = [&](int k, uint32_t* ids, float (*cb)[MAT == 0 ? 3 : 1]) {
@@ -490,11 +487,11 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
           const bool dsp = stq[r][c] && coef_of(ids, cb, r, c) < 0.f;
           const unsigned long long m = __ballot(dsp);
           if (dsp) aqq[r][c] = ab[r][c] + __popcll(m & below);
+          // a node's NP pole pairs are consecutive (NP is the engine's max_poles)
 #pragma unroll
           for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
             ppq[r][c][p] = Pair<T>{zero<T>(), zero<T>()};
-            if (aqq[r][c] >= 0 && p < a.max_poles)
-              ppq[r][c][p] = P[(unsigned)aqq[r][c] * (unsigned)a.max_poles + (unsigned)p];
+            if (aqq[r][c] >= 0) ppq[r][c][p] = P[(unsigned)aqq[r][c] * (unsigned)NP + (unsigned)p];
           }
         }
       }
@@ -622,8 +619,7 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
         }
         if (NP > 0 && aq[r][c] >= 0) {
 #pragma unroll
-          for (int p = 0; p < (NP > 0 ? NP : 1); ++p)
-            if (p < a.max_poles) P[(unsigned)aq[r][c] * (unsigned)a.max_poles + (unsigned)p] = pp[r][c][p];
+          for (int p = 0; p < (NP > 0 ? NP : 1); ++p) P[(unsigned)aq[r][c] * (unsigned)NP + (unsigned)p] = pp[r][c][p];
         }
       }
     }
@@ -698,12 +694,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(const StepArgs* __restr
         for (int q = threadIdx.x; q < n * per; q += 64 * WAVES) {
           const float* s = src + (q / per) * a.media_stride;
           const int p = q % per - 1;
-          float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (p < 0)
-            v = make_float4(s[0], s[1], 0.f, 0.f);
-          else if (p < a.max_poles)
-            v = make_float4(s[2 + 3 * p], s[3 + 3 * p], s[4 + 3 * p], 0.f);
-          med_tab[q] = v;
+          med_tab[q] = p < 0 ? make_float4(s[0], s[1], 0.f, 0.f)
+                             : make_float4(s[2 + 3 * p], s[3 + 3 * p], s[4 + 3 * p], 0.f);
         }
       }
       __syncthreads();
