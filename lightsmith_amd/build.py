@@ -9,6 +9,8 @@ Translation units (compiled by hipcc for gfx950 in parallel, then linked into li
                    and periodicity
 Packed-FP32 instructions are switched off (-packed-fp32-ops): the kernels are bandwidth-bound and the
 scalar forms are the ones this project trusts.
+Floating-point contraction is off (-ffp-contract=off): the kernels fuse multiply-adds explicitly (fmad in
+csrc/lsf_common.h), so every instantiation of a kernel rounds alike.
 """
 
 from __future__ import annotations
@@ -46,7 +48,7 @@ def hipcc() -> str:
 
 def flags() -> list[str]:
     return [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=off",
             "-Wno-unused-command-line-argument",
             "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 

@@ -20,6 +20,13 @@ __device__ __forceinline__ float sub(float a, float b) { return a - b; }
 __device__ __forceinline__ c32 sub(c32 a, c32 b) { return mk(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float scl(float s, float a) { return s * a; }
 __device__ __forceinline__ c32 scl(float s, c32 a) { return mk(s * a.x, s * a.y); }
+// s a + b with one rounding. The library compiles with floating-point contraction off and fuses only here, so
+// that a kernel rounds the same way in every instantiation (tile shape, CPML class, one tile per block or
+// persistent): a batch member then matches its run alone bit for bit.
+__device__ __forceinline__ float fmad(float s, float a, float b) { return __builtin_fmaf(s, a, b); }
+__device__ __forceinline__ c32 fmad(float s, c32 a, c32 b) {
+  return mk(__builtin_fmaf(s, a.x, b.x), __builtin_fmaf(s, a.y, b.y));
+}
 // a * b with a, b of the field type (complex product for c32)
 __device__ __forceinline__ float mul(float a, float b) { return a * b; }
 __device__ __forceinline__ c32 mul(c32 a, c32 b) { return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }

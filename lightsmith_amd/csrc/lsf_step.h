@@ -112,8 +112,8 @@ __device__ __forceinline__ T load_lo(const T* __restrict__ F, const Grid& g, int
 // (engine.py:636-643, 656-663) with cp = coef_s c and kp = coef_s (1/kappa - 1).
 template <class T>
 __device__ __forceinline__ T pml_apply(const float4 pr, int has_k, float coef_s, T der, T& psi) {
-  psi = add(scl(pr.x, psi), scl(coef_s * pr.y, der));
-  return has_k ? add(scl(coef_s * (pr.z - 1.f), der), psi) : psi;
+  psi = fmad(pr.x, psi, scl(coef_s * pr.y, der));
+  return has_k ? fmad(coef_s * (pr.z - 1.f), der, psi) : psi;
 }
 
 __device__ __forceinline__ bool in_box(const int (&r)[3][2], int i, int j, int k) {
@@ -284,7 +284,7 @@ __device__ __forceinline__ void h_tile(const StepArgs& a, int t) {
           db = sub(cEx[r + 1], cEx[r]);
         }
         const int d1 = (c + 1) % 3, d2 = (c + 2) % 3;
-        T acc = scl(-g.cH, sub(scl(g.s[d1], da), scl(g.s[d2], db)));
+        T acc = scl(-g.cH, fmad(g.s[d1], da, scl(-g.s[d2], db)));
         // CPML terms in ascending axis order (the reference's term order)
         const int first = d1 < d2 ? 0 : 1;
 #pragma unroll
@@ -568,7 +568,7 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
         const int d1 = (c + 1) % 3, d2 = (c + 2) % 3;
         const float cbv = coef(r, c);
         const float cb = cbv < 0.f ? g.cdx : cbv;
-        T dE = scl(cb, sub(scl(g.s[d1], da), scl(g.s[d2], db)));
+        T dE = scl(cb, fmad(g.s[d1], da, scl(-g.s[d2], db)));
         const int first = d1 < d2 ? 0 : 1;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -591,12 +591,12 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
 #pragma unroll
           for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
             const float4 q = mp[1 + p];
-            const T pn = add(add(scl(q.x, pp[r][c][p].p), scl(q.y, pp[r][c][p].q)), scl(q.z, eold));
+            const T pn = fmad(q.x, pp[r][c][p].p, fmad(q.y, pp[r][c][p].q, scl(q.z, eold)));
             dp = add(dp, sub(pn, pp[r][c][p].p));
             pp[r][c][p].q = pp[r][c][p].p;
             pp[r][c][p].p = pn;
           }
-          if (d) enew = add(eold, scl(mp[0].x, sub(dE, dp)));
+          if (d) enew = fmad(mp[0].x, sub(dE, dp), eold);
         }
         ev[r][c] = enew;
       }
