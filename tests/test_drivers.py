@@ -60,3 +60,48 @@ def test_periodic_time_step_single_cell_axis_uses_the_bloch_eigenvalue():
     k = 0.02
     dt = drivers.periodic_time_step(cell, 0.9, media, (k, 0.0))
     assert math.isclose(dt, 0.9 / math.sqrt(1 / 100 + (math.sin(k * 5.0) / 10.0) ** 2))
+
+
+def thin_film(n1, n2, n3, t, lam, theta, pol):
+    """(R, T) of a film (index n2, thickness t) between two half-spaces, s or p, incidence angle theta in medium 1:
+    the closed-form three-layer Airy sum (e^{-i omega t} convention, Im n2 >= 0)."""
+    s1 = n1 * math.sin(theta)
+    n = (n1, n2, n3)
+    c = [np.sqrt(1 - (s1 / m) ** 2 + 0j) for m in n]
+
+    def r(a, b):
+        if pol == "s":
+            return (n[a] * c[a] - n[b] * c[b]) / (n[a] * c[a] + n[b] * c[b])
+        return (n[b] * c[a] - n[a] * c[b]) / (n[b] * c[a] + n[a] * c[b])
+
+    def tr(a, b):
+        if pol == "s":
+            return 2 * n[a] * c[a] / (n[a] * c[a] + n[b] * c[b])
+        return 2 * n[a] * c[a] / (n[b] * c[a] + n[a] * c[b])
+
+    e2 = np.exp(4j * np.pi * n2 * c[1] * t / lam)  # round trip through the film
+    den = 1 + r(0, 1) * r(1, 2) * e2
+    rr = (r(0, 1) + r(1, 2) * e2) / den
+    tt = tr(0, 1) * tr(1, 2) * np.sqrt(e2) / den
+    fac = (n3 * c[2]).real / (n1 * c[0]).real if pol == "s" else (np.conj(n3) * c[2]).real / (np.conj(n1) * c[0]).real
+    return float(abs(rr) ** 2), float(fac * abs(tt) ** 2)
+
+
+def test_tmm_closed_form_conserves_energy_and_reduces_to_fresnel():
+    """The test's own closed form: lossless films conserve energy; a film of the substrate's index is a bare
+    interface (Fresnel); a quarter-wave layer of index sqrt(n1 n3) reflects nothing, a half-wave layer is absent
+    (these two pin the film's phase)."""
+    for pol in "sp":
+        for th in (0.0, 0.5, 1.0):
+            R, T = thin_film(1.0, 2.0, 1.5, 30.0, 600.0, th, pol)
+            assert abs(R + T - 1) < 1e-12
+            R, _ = thin_film(1.0, 1.5, 1.5, 30.0, 600.0, th, pol)
+            ci, ct = math.cos(th), math.sqrt(1 - (math.sin(th) / 1.5) ** 2)
+            Rf = ((ci - 1.5 * ct) / (ci + 1.5 * ct)) ** 2 if pol == "s" else ((1.5 * ci - ct) / (1.5 * ci + ct)) ** 2
+            assert abs(R - Rf) < 1e-12
+    n2 = math.sqrt(1.5)
+    R, T = thin_film(1.0, n2, 1.5, 600.0 / (4 * n2), 600.0, 0.0, "s")
+    assert R < 1e-20 and abs(T - 1) < 1e-12
+    R, _ = thin_film(1.0, 2.0, 1.5, 600.0 / (2 * 2.0), 600.0, 0.0, "p")
+    assert abs(R - 0.04) < 1e-12  # ((1 - 1.5) / (1 + 1.5))^2
+
