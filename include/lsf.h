@@ -78,7 +78,9 @@ int lsf_set_pml(lsf_engine* eng, int field, int axis, const void* dev_profile, i
  * zeroed by the caller.
  * D is not stored: D^n = eps_inf E^n + sum P^n holds after every step, so the update advances E by
  * (dE - sum (P^{n+1} - P^n)) / eps_inf. dev_media[n_media][2 + 3*max_poles] float =
- * {1/eps_inf, n_poles, (c1, c2, c3) per pole} (pole_coefficients, engine.py:108-111); n_media <= 16. */
+ * {1/eps_inf, n_poles, (c1, c2, c3) per pole} (pole_coefficients, engine.py:108-111), zeros for the poles past a
+ * medium's n_poles (the E kernel advances max_poles poles for every node; zero coefficients keep those poles at
+ * zero); n_media * (2 + 3*max_poles) <= 224 (16 media of 4 poles, 44 of 1). */
 int lsf_set_ade(lsf_engine* eng, const int32_t* dev_chunk_base, void* dev_P, int64_t n_nodes, int max_poles,
                 const float* dev_media, int n_media);
 

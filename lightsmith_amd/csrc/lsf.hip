@@ -713,6 +713,21 @@ int lsf_set_ade(lsf_engine* e, const int32_t* chunk_base, void* P, int64_t n, in
   if (n > 0 && (n_media * (2 + 3 * max_poles) > lsf::MAX_MEDIA_FLOATS ||
                 n_media * (max_poles + 1) > lsf::MAX_MEDIA_VEC4))
     return fail(LSF_EINVAL, "too many D-form media for the E kernel's table (at most 16 media of 4 poles)");
+  if (n > 0) {  // the E kernel advances max_poles poles for every node: a medium's unused poles must be all zero
+    const int stride = 2 + 3 * max_poles;
+    std::vector<float> tab((size_t)n_media * stride);
+    const int rc = hip_check(hipMemcpy(tab.data(), media, tab.size() * sizeof(float), hipMemcpyDeviceToHost),
+                             "D-form media table read");
+    if (rc) return rc;
+    for (int m = 0; m < n_media; ++m) {
+      const float* t = tab.data() + (size_t)m * stride;
+      const int np = (int)t[1];
+      if ((float)np != t[1] || np < 0 || np > max_poles)
+        return fail(LSF_EINVAL, "D-form media table: a medium's pole count must be an integer in [0, max_poles]");
+      for (int q = 2 + 3 * np; q < stride; ++q)
+        if (t[q] != 0.f) return fail(LSF_EINVAL, "D-form media table: the coefficients past a medium's poles must be 0");
+    }
+  }
   e->chunk_base = (const int*)chunk_base;
   e->P = P;
   e->n_ade = n;

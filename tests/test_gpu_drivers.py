@@ -96,12 +96,13 @@ def test_dipole_power_leaves_through_the_box(gpu):
     assert np.all(res.p_tot > 0) and np.all(np.abs(ratio - 1) < 0.01)
 
 
-def _film_cell(d, nf, film, eps_sub=2.25, n=(4, 4), pml_nm=80.0, gap_nm=240.0, pad_nm=60.0, layer_nm=60.0):
-    """A periodic cell with a dispersive film of nf cells (thickness nf d) on a substrate: the film's E nodes are
-    D-form nodes of `film`; interfaces sit half a cell below the film's first and last E_x/E_y planes, the E_z
+def _film_cell(d, nf, film, eps_sub=2.25, n=(4, 4), pml_nm=80.0, gap_nm=240.0, pad_nm=60.0, above_nm=30.0):
+    """A periodic cell with a film of nf cells (thickness nf d) on a substrate: the film's E nodes take 1/eps_inf of
+    `film`, or are D-form nodes of it when it has poles; interfaces sit half a cell below the film's first and last E_x/E_y planes, the E_z
     nodes (half a cell up) take the medium at their own position."""
     from lightsmith_amd.fdtd.engine import DispersiveGroup
-    pml, gap, pad, layer = (int(round(v / d)) for v in (pml_nm, gap_nm, pad_nm, layer_nm))
+    pml, gap, pad = (int(round(v / d)) for v in (pml_nm, gap_nm, pad_nm))
+    layer = nf + int(round(above_nm / d))  # the incident plane k_top sits above_nm above the film
     k_r = pml + gap
     k_src = k_r + 2
     k_top = k_src + pad
@@ -115,10 +116,14 @@ def _film_cell(d, nf, film, eps_sub=2.25, n=(4, 4), pml_nm=80.0, gap_nm=240.0, p
     inv[2, :, :, k_bot - 1:] = 1 / eps_sub
     film_sel[0:2, :, :, k_bot - nf:k_bot] = True
     film_sel[2, :, :, k_bot - nf - 1:k_bot - 1] = True
-    inv[film_sel] = 1.0
-    idx = np.nonzero(film_sel.reshape(-1))[0].astype(np.int64)
-    grp = DispersiveGroup(idx=idx, media=[film], frac=np.ones((1, len(idx))), w=None)
-    return drivers.PeriodicCell(n_cells=n, nz=nz, spacing=(d, d, d), inv_eps=inv, groups=[grp], pml=pml, k_src=k_src,
+    groups = []
+    if film.poles:
+        inv[film_sel] = 1.0
+        idx = np.nonzero(film_sel.reshape(-1))[0].astype(np.int64)
+        groups.append(DispersiveGroup(idx=idx, media=[film], frac=np.ones((1, len(idx))), w=None))
+    else:
+        inv[film_sel] = 1.0 / film.eps_inf
+    return drivers.PeriodicCell(n_cells=n, nz=nz, spacing=(d, d, d), inv_eps=inv, groups=groups, pml=pml, k_src=k_src,
                                 k_r=k_r, k_t=k_t, k_top=k_top, eps_sup=1.0, eps_sub=eps_sub)
 
 
@@ -153,3 +158,30 @@ def test_drude_film_is_the_thin_film_formula(gpu):
             print(f"Drude film, {deg:.0f} deg {pol}: R {R:.5f} (TMM {Rt:.5f}), T {T:.5f} (TMM {Tt:.5f})")
             worst = max(worst, abs(R - Rt), abs(T - Tt))
     assert worst < 0.003
+
+
+def test_dielectric_slab_is_the_thin_film_formula(gpu):
+    """A 300 nm n = 2 slab on glass, the reference's layer-stack case (test_fdtd_periodic.py:55-63: 10 nm cells,
+    400-800 nm, tolerance 0.01 at normal incidence; :91-103: 450 and 750 nm at 30 and 60 deg, s and p, 0.008),
+    against the three-layer closed form."""
+    d, nf = 10.0, 30
+    cell = _film_cell(d, nf, Medium(4.0))
+    wl = np.linspace(400.0, 800.0, 9)
+    res = drivers.solve_periodic(gpu, cell, wl, pol="s")
+    worst = 0.0
+    for q, lam in enumerate(wl):
+        Rt, Tt = thin_film(1.0, 2.0, 1.5, nf * d, lam, 0.0, "s")
+        R, T = float(res.r[q, 0, 0]), float(res.t[q, 0, 0])
+        worst = max(worst, abs(R - Rt), abs(T - Tt))
+    print(f"n = 2 slab, normal incidence: max |dR|, |dT| {worst:.4f}")
+    assert worst < 0.01
+    worst = 0.0
+    for deg in (30.0, 60.0):
+        th = math.radians(deg)
+        for lam in (450.0, 750.0):
+            for pol in "sp":
+                r = drivers.solve_periodic(gpu, cell, [lam], pol=pol, kpar=(2 * math.pi / lam * math.sin(th), 0.0))
+                Rt, Tt = thin_film(1.0, 2.0, 1.5, nf * d, lam, th, pol)
+                worst = max(worst, abs(float(r.r[0, 0, 0]) - Rt), abs(float(r.t[0, 0, 0]) - Tt))
+                print(f"n = 2 slab, {deg:.0f} deg {lam:.0f} nm {pol}: R {float(r.r[0, 0, 0]):.5f} (TMM {Rt:.5f})")
+    assert worst < 0.008

@@ -373,3 +373,26 @@ def test_single_pass_matches_the_oracle_and_the_two_kernel_step(gpu, material, s
     assert cases.rel_l2(ea, eo) < DFT_TOL and cases.rel_l2(ha, ho) < DFT_TOL
     assert same
     assert rel1 < 1e-7 and rel < 2e-5
+
+
+def test_ade_media_table_is_validated(gpu):
+    """lsf_set_ade refuses a D-form media table whose pole count is not an integer in [0, max_poles] or whose
+    coefficients past a medium's poles are not zero (the E kernel advances max_poles poles for every node), and
+    accepts the engine's own table again afterwards."""
+    from lightsmith_amd import _lib
+    eng = cases.dispersive_box(cases.product_ns(), gpu, shape=(20, 18, 16), with_dipole=False, monitors=False)
+    lib, mp = eng.lib, eng.ade_max_poles
+    assert mp == 3 and eng.n_ade > 0
+
+    def set_ade(tab):
+        return lib.lsf_set_ade(eng._h, _lib.ptr(eng._ade_chunk), _lib.ptr(eng._ade_P), eng.n_ade, mp, _lib.ptr(tab),
+                               len(eng.ade_media))
+
+    bad = eng._ade_tab.clone()
+    bad[0, 1] = 2.0  # two poles declared, the third pole's coefficients still set
+    assert set_ade(bad) == _lib.LSF_EINVAL and b"past a medium's poles" in lib.lsf_last_error()
+    bad = eng._ade_tab.clone()
+    bad[0, 1] = 2.5
+    assert set_ade(bad) == _lib.LSF_EINVAL and b"integer" in lib.lsf_last_error()
+    assert set_ade(eng._ade_tab) == _lib.LSF_OK
+    eng.close()
