@@ -397,7 +397,7 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
   // first D-form slot of each row's 32-cell x chunk per component: chunk_base[((c n2 + k) ny + j) nchunk + x / 32],
   // one vector load per row and component (the index sits in a VGPR), so that these loads join the plane's other
   // loads instead of waiting one by one in the scalar cache
-  const int* __restrict__ cbase = (const int*)as_global(a.chunk_base);
+  const gptr<const int> cbase = as_global(a.chunk_base);
   const unsigned cb_k = ny * (unsigned)a.nchunk;  // per plane
   const unsigned cb_j = (unsigned)vregi((int)((unsigned)j0 * (unsigned)a.nchunk + (unsigned)tx * (XW / 32)));
   auto load_bases = [&](int k, int (*ab)[3]) {
@@ -407,7 +407,7 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
       for (int c = 0; c < 3; ++c) {
         ab[r][c] = 0;
         if (NP > 0 && k < g.n[2] && j0 + r < g.n[1])
-          ab[r][c] = cbase[cb_j + (unsigned)r * (unsigned)a.nchunk + ((unsigned)c * g.n[2] + (unsigned)k) * cb_k];
+          ab[r][c] = ((const int*)cbase)[cb_j + (unsigned)r * (unsigned)a.nchunk + ((unsigned)c * g.n[2] + (unsigned)k) * cb_k];
       }
   };
   auto load_mat = [&](int k, uint32_t* ids, float (*cb)[MAT == 0 ? 3 : 1]) {
