@@ -97,11 +97,12 @@ def random_fields(eng, seed: int, periodic=(False, False, False)):
 
 # --- configurations -------------------------------------------------------------------------------
 
-def lorentz3(rng):
-    """A 3-pole Lorentz medium with seeded poles (omega_p, omega_0 in [1, 10] eV, gamma in [0.05, 0.5] eV)."""
+def lorentz3(rng, n_poles=3):
+    """An n-pole (3 by default) Lorentz medium with seeded poles (omega_p, omega_0 in [1, 10] eV, gamma in
+    [0.05, 0.5] eV)."""
     from oracle.media import HBARC_EV_NM
     poles = []
-    for _ in range(3):
+    for _ in range(n_poles):
         wp, w0, g = rng.uniform(1, 10), rng.uniform(1, 10), rng.uniform(0.05, 0.5)
         poles.append(((wp / HBARC_EV_NM) ** 2, w0 / HBARC_EV_NM, g / HBARC_EV_NM))
     return poles
@@ -113,9 +114,10 @@ def medium(ns, eps_inf, poles):
 
 def dispersive_box(ns, device=None, shape=(40, 36, 32), dx=2.0, seed=0, pml_cells=6, n_steps_hint=400,
                    with_dipole=True, frac=0.1, kappa_max=1.0, alpha=0.0, monitors=True, material="auto", fused=None,
-                   stride=10):
+                   stride=10, pole_counts=None):
     """CPML box with random dielectric values, a fraction of D-form 3-pole Lorentz nodes, a z dipole and a
-    DFT plane: a small version of the 320^3 throughput configuration."""
+    DFT plane: a small version of the 320^3 throughput configuration. pole_counts (e.g. (1, 3)): one Lorentz
+    medium of each count instead, each D-form cell given one of them at random (one pure group per medium)."""
     rng = np.random.default_rng(seed)
     eps = rng.choice([1.0, 2.25, 4.0, 12.0], size=shape)
     inv_eps = np.broadcast_to(1.0 / eps, (3,) + shape).copy()
@@ -129,6 +131,17 @@ def dispersive_box(ns, device=None, shape=(40, 36, 32), dx=2.0, seed=0, pml_cell
     idx = np.concatenate([ns.flat_index(c, ii, jj, kk, shape) for c in range(3)])
     inv_eps.reshape(3, -1)[:, np.ravel_multi_index((ii, jj, kk), shape)] = 1.0
     groups = [ns.DispersiveGroup(idx=idx, media=[med], frac=np.ones((1, len(idx))), w=None)] if len(idx) else []
+    if pole_counts:
+        prng = np.random.default_rng(seed + 200)
+        which = prng.integers(0, len(pole_counts), size=len(ii))
+        groups = []
+        for q, npl in enumerate(pole_counts):
+            sel = which == q
+            if not sel.any():
+                continue
+            m = medium(ns, 1.0, lorentz3(np.random.default_rng(seed + 300 + q), npl))
+            gi = np.sort(np.concatenate([ns.flat_index(c, ii[sel], jj[sel], kk[sel], shape) for c in range(3)]))
+            groups.append(ns.DispersiveGroup(idx=gi, media=[m], frac=np.ones((1, len(gi))), w=None))
     dt = 0.95 * dx / np.sqrt(3)
     sources = []
     if with_dipole:

@@ -396,3 +396,18 @@ def test_ade_media_table_is_validated(gpu):
     assert set_ade(bad) == _lib.LSF_EINVAL and b"integer" in lib.lsf_last_error()
     assert set_ade(eng._ade_tab) == _lib.LSF_OK
     eng.close()
+
+
+@pytest.mark.parametrize("pole_counts", [(1,), (2,), (4,), (1, 3), (4, 2, 1)])
+def test_pole_counts_and_mixed_media(gpu, pole_counts):
+    """The E kernel is instantiated per exact pole count (the engine's max_poles); media with fewer poles than that
+    share it with zero coefficients in their unused poles. Lorentz media of 1, 2 and 4 poles alone, and mixtures in
+    one engine (one pure group per medium), against the FP64 oracle at the parity tolerance."""
+    steps = 200
+    o = cases.dispersive_box(cases.oracle_ns(), n_steps_hint=steps, pole_counts=pole_counts)
+    g = cases.dispersive_box(cases.product_ns(), gpu, n_steps_hint=steps, pole_counts=pole_counts)
+    assert g.ade_max_poles == max(pole_counts) and len(g.ade_media) == len(pole_counts)
+    for _ in range(steps // o.stride):
+        o.chunk()
+        g.chunk()
+    _compare(o, g, f"pole counts {pole_counts}, {steps} steps")
