@@ -26,13 +26,16 @@ def _sweep(gpu, runs, stride, steps):
 
 def _mixed(gpu, steps):
     """Oblique-sweep members (real TFSF at normal incidence and complex Bloch runs) plus dispersive CPML boxes
-    (D-form poles, dipole; indexed and dense materials), all on one DFT stride."""
+    (D-form poles, dipole; indexed and dense materials; 3-pole media, and 2- and 1-pole media in one box), all on one
+    DFT stride."""
     runs = workloads.oblique_sweep_scene(**SMALL)
     stride = 10
     engs = _sweep(gpu, runs, stride, steps)
     ns = cases.product_ns()
     engs.append(cases.dispersive_box(ns, gpu, shape=(40, 36, 32), n_steps_hint=steps, material="indexed"))
     engs.append(cases.dispersive_box(ns, gpu, shape=(29, 45, 21), seed=3, n_steps_hint=steps, material="dense"))
+    engs.append(cases.dispersive_box(ns, gpu, shape=(24, 30, 20), seed=5, n_steps_hint=steps, material="indexed",
+                                     pole_counts=(2, 1)))  # another pole count: its own E launch
     for e in engs:
         assert e.stride == stride
     return engs
@@ -115,7 +118,7 @@ def test_batched_member_matches_the_oracle(gpu):
         o.chunk()
         b.chunk()
     torch.cuda.synchronize()
-    g = members[-2]
+    g = members[-3]  # the (40, 36, 32) indexed box
     Eo, Ho = cases.fields(o)
     Eg, Hg = cases.fields(g)
     err = max(cases.rel_l2(Eg, Eo), cases.rel_l2(Hg, Ho))
@@ -138,7 +141,7 @@ def test_batch_refuses_what_it_cannot_run(gpu):
     odd = workloads.oblique_engine(workloads.oblique_sweep_scene(**SMALL)[1], gpu, max_steps=100, stride=7)
     with pytest.raises(ValueError):
         Batch([engs[2], odd])  # different DFT strides
-    np.testing.assert_equal(len(engs), 8)
+    np.testing.assert_equal(len(engs), 9)
 
 
 def test_batch_run_control_matches_sequential_runs(gpu):
