@@ -119,7 +119,7 @@ __device__ __forceinline__ void ah_tile(const StepArgs& a, int t) {
       const unsigned cell = cell32(nx, ny, i, j, k);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        st[r][c] = inx && j < g.n[1] && in_box(g.rH[c], i, j, k);
+        st[r][c] = inx && j < g.n[1] && h_box<false>(g, c, i, j, k);
         hv[r][c] = st[r][c] ? H[(unsigned)c * N + cell] : 0.f;
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
@@ -328,7 +328,7 @@ __device__ __forceinline__ void ae_tile(const StepArgs& a, int t, const float* t
       const unsigned cell = cell32(nx, ny, i, j, k);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        st[r][c] = inx && j < g.n[1] && in_box(g.rE[c], i, j, k);
+        st[r][c] = inx && j < g.n[1] && e_box<false>(g, c, i, j, k);
         ev[r][c] = st[r][c] ? E[(unsigned)c * N + cell] : 0.f;
 #pragma unroll
         for (int s = 0; s < 2; ++s) {

@@ -116,10 +116,6 @@ __device__ __forceinline__ T pml_apply(const float4 pr, int has_k, float coef_s,
   return has_k ? fmad(coef_s * (pr.z - 1.f), der, psi) : psi;
 }
 
-__device__ __forceinline__ bool in_box(const int (&r)[3][2], int i, int j, int k) {
-  return i >= r[0][0] && i < r[0][1] && j >= r[1][0] && j < r[1][1] && k >= r[2][0] && k < r[2][1];
-}
-
 // The stepped boxes follow from the grid and its periodic axes (engine.py:47-66, no mirror planes; the host builds
 // Grid::rE/rH the same way): a periodic axis steps [1, n - 1); otherwise E_c spans [0, n - 1) along c and
 // [1, n - 1) across, H_c [0, n) along c and [0, n - 1) across. Computing them here keeps the 36 bounds out of the
