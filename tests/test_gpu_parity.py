@@ -411,3 +411,45 @@ def test_pole_counts_and_mixed_media(gpu, pole_counts):
         o.chunk()
         g.chunk()
     _compare(o, g, f"pole counts {pole_counts}, {steps} steps")
+
+
+@pytest.mark.parametrize("pole_counts", [(3,), (2, 3)])
+def test_pole_pairs_complex_bloch(gpu, pole_counts):
+    """Complex64 fields, Bloch-periodic in x and y, with Lorentz D-form nodes of 2 and 3 poles (a complex node's
+    pole pairs move in one 16-byte access each): random E and H on the stepped cells, zero E at the D-form nodes
+    (their pole state starts at rest), against the FP64 oracle."""
+    shape, dx, steps = (22, 18, 30), 2.0, 120
+    periodic, kpar = (True, True, False), (0.06, -0.04, 0.0)
+    rng = np.random.default_rng(11)
+    eps = rng.choice([1.0, 2.25, 4.0], size=shape)
+    inv_eps = np.broadcast_to(1.0 / eps, (3,) + shape).copy()
+    disp = np.zeros(shape, bool)
+    disp[2:-2, 2:-2, 8:-8] = rng.random((shape[0] - 4, shape[1] - 4, shape[2] - 16)) < 0.15
+    ii, jj, kk = np.nonzero(disp)
+    inv_eps.reshape(3, -1)[:, np.ravel_multi_index((ii, jj, kk), shape)] = 1.0
+    which = rng.integers(0, len(pole_counts), size=len(ii))
+    kw = dict(shape=shape, dx=dx, dt=0.9 * dx / np.sqrt(3), inv_eps=inv_eps, periodic=periodic, bloch=kpar)
+    engines = []
+    for ns, dev in ((cases.oracle_ns(), None), (cases.product_ns(), gpu)):
+        groups = []
+        for q, npl in enumerate(pole_counts):
+            sel = which == q
+            m = cases.medium(ns, 1.0, cases.lorentz3(np.random.default_rng(40 + q), npl))
+            gi = np.sort(np.concatenate([ns.flat_index(c, ii[sel], jj[sel], kk[sel], shape) for c in range(3)]))
+            groups.append(ns.DispersiveGroup(idx=gi, media=[m], frac=np.ones((1, len(gi))), w=None))
+        pml = ns.PMLSpec(cells=((0, 0), (0, 0), (6, 6)))
+        engines.append(cases.make(ns, dev, pml=pml, groups=groups, **kw))
+    o, g = engines
+    assert g.ade_max_poles == max(pole_counts)
+    frng = np.random.default_rng(5)
+    for c in range(3):
+        for F, R in (("E", o.RE[c]), ("H", o.RH[c])):
+            size = tuple(r.stop - r.start for r in R)
+            vals = frng.standard_normal(size) + 1j * frng.standard_normal(size)
+            if F == "E":
+                vals[disp[R]] = 0.0
+            getattr(o, F)[c][R] = vals
+            getattr(g, F)[c][R] = torch.as_tensor(vals, dtype=torch.complex64, device=g.E_flat.device)
+    o.run(steps)
+    g.steps(steps)
+    _compare(o, g, f"complex Bloch, pole counts {pole_counts}, {steps} steps", mask=cases.real_mask(shape, periodic))
