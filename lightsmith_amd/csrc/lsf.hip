@@ -130,7 +130,7 @@ struct lsf_engine {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   lsf::StepArgs* dev_args = nullptr;  // library scratch: [0] E kernel, [1] H kernel arguments
-  int nb = 0, nbp = 0;                 // tiles of this engine's own launches (and padded to a multiple of 8)
+  int nb[2] = {0, 0}, nbp[2] = {0, 0};  // tiles of the E [0] and H [1] launches (and padded to a multiple of 8)
   bool dirty = true;                   // dev_args out of date
   bool launched = false;               // kernels may be reading dev_args
   int held = 0;                        // batches holding this engine (its configuration is then frozen)
@@ -160,6 +160,10 @@ struct Tiling {
 
 Tiling own_tiling(const lsf_engine* e) { return Tiling{e->ry, e->waves, e->xw, e->zc}; }
 
+// pole registers of the E kernel instantiation for this engine: its max_poles (the slot stride of the pole
+// arrays); the single-pass kernel rounds 1 up to 2
+int pole_regs(const lsf_engine* e) { return e->n_ade > 0 ? e->max_poles : 0; }
+
 // Kernel arguments of engine e for field 0 (E kernel) or 1 (H kernel) under tiling t.
 StepArgs make_args(const lsf_engine* e, int field, const Tiling& t) {
   StepArgs a{};
@@ -186,8 +190,10 @@ StepArgs make_args(const lsf_engine* e, int field, const Tiling& t) {
   const int rows = t.ry * t.waves * (64 / t.xw);
   a.tiles_x = (e->g.n[0] + t.xw - 1) / t.xw;
   a.tiles_y = (e->g.n[1] + rows - 1) / rows;
-  a.zc = t.zc;
-  const int tiles_z = (e->g.n[2] + t.zc - 1) / t.zc;
+  // the E kernel with pole state marches twice the z chunk (320^3, 10 % 3-pole nodes: E kernel 451 us at 8 planes
+  // per tile, 442 us at 16; H kernel 269 us at 8, 276 us at 16)
+  a.zc = field == 0 && !e->adjoint && pole_regs(e) > 0 ? 2 * t.zc : t.zc;
+  const int tiles_z = (e->g.n[2] + a.zc - 1) / a.zc;
   a.nb = a.tiles_x * a.tiles_y * tiles_z;
   a.nbp = (a.nb + 7) / 8 * 8;
   return a;
@@ -195,9 +201,6 @@ StepArgs make_args(const lsf_engine* e, int field, const Tiling& t) {
 
 bool any_periodic(const lsf_engine* e) { return e->g.per[0] || e->g.per[1] || e->g.per[2]; }
 
-// pole registers of the E kernel instantiation for this engine: its max_poles (the slot stride of the pole
-// arrays); the single-pass kernel rounds 1 up to 2
-int pole_regs(const lsf_engine* e) { return e->n_ade > 0 ? e->max_poles : 0; }
 
 void set_parity(lsf_engine* e, int p) {
   e->parity = p;
@@ -349,8 +352,10 @@ int sync_args(lsf_engine* e) {
   const StepArgs h[2] = {make_args(e, 0, t), make_args(e, 1, t)};
   rc = hip_check(hipMemcpy(e->dev_args, h, sizeof h, hipMemcpyHostToDevice), "kernel argument upload");
   if (rc) return rc;
-  e->nb = h[0].nb;
-  e->nbp = h[0].nbp;
+  for (int f = 0; f < 2; ++f) {
+    e->nb[f] = h[f].nb;
+    e->nbp[f] = h[f].nbp;
+  }
   e->use_fused = fused_eligible(e);
   if (e->use_fused) {
     if (!e->dev_fused) rc = hip_check(hipMalloc(&e->dev_fused, 2 * sizeof(lsf::FusedArgs)), "hipMalloc fused arguments");
@@ -383,8 +388,8 @@ void launch_fused_step(lsf_engine* e, hipStream_t s) {
 lsf::KLaunch own_launch(lsf_engine* e, int field) {
   lsf::KLaunch L{};
   L.args = e->dev_args + field;
-  L.sc = lsf::Sched{nullptr, nullptr, e->nb, e->nbp};
-  L.grid = e->nbp;
+  L.sc = lsf::Sched{nullptr, nullptr, e->nb[field], e->nbp[field]};
+  L.grid = e->nbp[field];
   L.ry = e->ry;
   L.waves = e->waves;
   L.xw = e->xw;
