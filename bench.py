@@ -381,6 +381,8 @@ def main() -> int:
                       "cells": scene["n_cells"], "dx_nm": scene["dx"], "dt_nm_per_c": scene["dt"],
                       "dft_stride": stride, "dispersive_nodes": int(eng.n_ade), "psi_values": int(eng.n_psi),
                       "material": eng.material_mode, "tiling": list(eng.tiling if tiling is None else tiling),
+                      "tiling_note": "rows per lane, wavefronts per block, x cells per row, z planes per tile; the E "
+                                     "kernel of an engine with pole state marches twice the z planes",
                       "hip_graphs": graphs, "parallelism": "single GPU"},
            "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "adjoint": adjoint}
     print(json.dumps(out), flush=True)
