@@ -169,7 +169,8 @@ int lsf_add_tensor_group(lsf_engine* eng, int64_t n_nodes, const int64_t* dev_no
 int lsf_set_energy(lsf_engine* eng, float* dev_energy);
 
 /* Launch tuning: rows per lane (1 or 2), wavefronts per block (4 or 8), x cells per wavefront row (64: one
- * row per wavefront; 32: two rows), z planes per block (>= 1). No reference counterpart (a launch choice). */
+ * row per wavefront; 32: two rows), z planes per block (>= 1; the E kernel of an engine with pole state marches
+ * twice as many). No reference counterpart (a launch choice). */
 int lsf_set_tiling(lsf_engine* eng, int rows_per_lane, int waves, int x_cells, int z_chunk);
 
 /* Run n_steps steps (Engine.step, engine.py:665-678) on `stream` (a hipStream_t; NULL = default). */

@@ -257,8 +257,8 @@ def _device_of(device) -> torch.device:
 class Engine:
     """FDTD engine whose stepping runs in the HIP library (engine.py:230-315 constructor contract)."""
 
-    # launch tiling (rows per lane, wavefronts per block, x cells per wavefront row, z planes per block);
-    # see lsf_set_tiling
+    # launch tiling (rows per lane, wavefronts per block, x cells per wavefront row, z planes per block; the E
+    # kernel of an engine with pole state marches twice the z planes); see lsf_set_tiling
     tiling = (1, 4, 64, 8)
     # default of the `fused` argument: step eligible engines (real fields, no periodic axes, one row per lane,
     # 64-cell rows, no H-phase operations) with the single-pass kernel over a second buffer set
