@@ -349,9 +349,9 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_h(const StepArgs* __restr
 // k + 1 and the material of plane k + 2, so that the latency of every load overlaps a plane's arithmetic.
 // ---------------------------------------------------------------------------------------------
 // A D-form node's NP pole pairs {P^n, P^{n-1}} are adjacent in memory: move them with 16-byte (and one 8-byte)
-// accesses instead of one access per pair. Slots are only 4-byte aligned (a float node's pairs take 8 NP bytes);
-// gfx950 global accesses take any dword alignment. Fewer, wider memory instructions for the same bytes cut the
-// E kernel's time by ~8 % at 10 % D-form cells.
+// accesses instead of one access per pair. A float node's slot takes 8 NP bytes, so for odd NP it is 8-byte but
+// not always 16-byte aligned; the vector types declare 4-byte alignment, which gfx950 global accesses accept.
+// Fewer, wider memory instructions for the same bytes cut the E kernel's time by ~8 % at 10 % D-form cells.
 typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
 typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
 template <int NP>
