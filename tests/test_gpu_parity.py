@@ -453,3 +453,24 @@ def test_pole_pairs_complex_bloch(gpu, pole_counts):
     o.run(steps)
     g.steps(steps)
     _compare(o, g, f"complex Bloch, pole counts {pole_counts}, {steps} steps", mask=cases.real_mask(shape, periodic))
+
+
+@pytest.mark.parametrize("tiling", [(1, 4, 64, 5), (2, 4, 64, 16), (1, 8, 32, 3)])
+def test_tilings_agree_with_poles(gpu, tiling):
+    """The E kernel of an engine with pole state marches twice the tiling's z planes (10, 32, 6 here, partial last
+    tiles included) while the H kernel marches the tiling's own: 3-pole D-form nodes in a CPML box against the FP64
+    oracle under tilings other than the default."""
+    steps = 120
+    o = cases.dispersive_box(cases.oracle_ns(), n_steps_hint=steps)
+    ns = cases.product_ns()
+    old = ns.Engine.tiling
+    ns.Engine.tiling = tiling
+    try:
+        g = cases.dispersive_box(ns, gpu, n_steps_hint=steps)
+    finally:
+        ns.Engine.tiling = old
+    assert g.ade_max_poles == 3
+    for _ in range(steps // o.stride):
+        o.chunk()
+        g.chunk()
+    _compare(o, g, f"tiling {tiling} with 3-pole nodes, {steps} steps")
