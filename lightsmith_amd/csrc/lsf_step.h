@@ -413,6 +413,9 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
   for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
     for (int c = 0; c < 3; ++c) psi[ax][c] = as_global((T*)a.pml[ax].psi[c]);
+  // CPML slab widths in vector registers: read as kernel arguments inside the march they are reloaded from memory
+  // (the scalar registers are full), and each reload stalls the plane's load issue until it returns
+  const int nsv[3] = {vregi(a.pml[0].ns), vregi(a.pml[1].ns), vregi(a.pml[2].ns)};
   const bool inx = i < g.n[0];
   const int il = tx * XW - 1;
   const bool xl_ok = (xl == 0) && il >= 0;
@@ -516,7 +519,7 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
           const int p = (int)pr.w;
           psq[r][c][s] = zero<T>();
           if (stq[r][c] && p >= 0)
-            psq[r][c][s] = ((const T*)psi[ax][c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)];
+            psq[r][c][s] = ((const T*)psi[ax][c])[psi_index(ax, p, nsv[ax], nx, ny, i, j, k)];
         }
         aqq[r][c] = -1;
         if (NP > 0) {
@@ -650,7 +653,7 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
           const int ax = (c + 1 + s) % 3;
           const float4 pr = ax == 0 ? px : (ax == 1 ? py[r] : pz);
           const int p = (int)pr.w;
-          if (p >= 0) ((T*)psi[ax][c])[psi_index(ax, p, a.pml[ax].ns, nx, ny, i, j, k)] = ps[r][c][s];
+          if (p >= 0) ((T*)psi[ax][c])[psi_index(ax, p, nsv[ax], nx, ny, i, j, k)] = ps[r][c][s];
         }
         if (NP > 0 && aq[r][c] >= 0) store_slot(P + (unsigned)aq[r][c] * (unsigned)NP, pp[r][c]);
       }
