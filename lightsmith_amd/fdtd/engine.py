@@ -137,6 +137,11 @@ class TimeMonitor:
     pos: object = field(default=None, repr=False)
 
     def results(self, n: int) -> np.ndarray:
+        """(channels, samples) for the first n samples. The kernel drops samples past n_max and keeps counting,
+        so an overflow raises here, as the reference's indexed store does at the sample (engine.py:590-592)."""
+        taken = int(self.pos.item())
+        if taken > int(self.n_max):
+            raise IndexError(f"time monitor overflow: {taken} samples taken, buffer holds n_max={self.n_max}")
         return self.buf.cpu().numpy()[:n].T
 
 

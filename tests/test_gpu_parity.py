@@ -242,6 +242,11 @@ def test_mode_injection_and_time_monitor(gpu):
     o, g = out
     _compare(o, g, "mode injection")
     assert cases.rel_l2(g.time_monitors[0].results(30), o.time_monitors[0].results(30)) < FIELD_TOL
+    # past n_max samples the reference's indexed store raises; the engine reports the overflow at readback
+    for _ in range(40 - steps // 5 + 1):
+        g.chunk()
+    with pytest.raises(IndexError, match="time monitor overflow"):
+        g.time_monitors[0].results(40)
 
 
 def test_mixed_groups_and_offdiagonal(gpu):
