@@ -207,14 +207,17 @@ int lsf_add_tape(lsf_engine* eng, int64_t n_nodes, const int64_t* dev_nodes, flo
 int lsf_add_adjoint_gradient(lsf_engine* eng, int64_t n_nodes, const int64_t* dev_nodes, const float* dev_cb,
                              const float* dev_tape, int64_t n_slots, const int64_t* dev_base, int64_t n_steps,
                              double* dev_g);
-/* Adjoint engine: the source of a DFT monitor objective. dev_W[nf][np] complex64 = conj(dL/dacc) (torch's
- * gradient of the monitor's E or H accumulator). E side: after the adjoint E half-step reaching E^n, for
- * n > 0, n % stride == 0: F[t1[p]] += scale[p] * sum_f Re(W[f,p] exp(i omega_f n dt)) (scale = cb at the node).
- * H side: after the adjoint H half-step reaching H^{n-1/2}: both t1[p] and t2[p] get the term with
- * t = n dt - dt/2 (scale = -cH / 2). The source of E^{n_steps} itself is the caller's initial condition. */
+/* Adjoint engine: the source of a DFT monitor objective. dev_W[nf][n_pts] complex64 = conj(dL/dacc) (torch's
+ * gradient of the monitor's E or H accumulator). The targets are given as CSR rows, one per distinct node:
+ * row r adds to node dev_target[r] the sum over its entries q (dev_row_ptr[r] .. dev_row_ptr[r+1]) of
+ * dev_scale[q] * sum_f Re(W[f, dev_point[q]] exp(i omega_f t)), summed in that fixed order (deterministic where
+ * monitor points share a node). E side: after the adjoint E half-step reaching E^n, for n > 0, n % stride == 0,
+ * t = n dt (scale = cb at the node). H side: after the adjoint H half-step reaching H^{n-1/2}, t = n dt - dt/2;
+ * each H point enters the rows of both its averaged nodes (scale = -cH / 2). The source of E^{n_steps} itself is
+ * the caller's initial condition. */
 int lsf_add_adjoint_source(lsf_engine* eng, int h_side, int nf, int64_t n_pts, const double* dev_omega, double dt,
-                           const void* dev_W, const int64_t* dev_t1, const int64_t* dev_t2, const float* dev_scale,
-                           int64_t n_steps, int stride);
+                           const void* dev_W, int64_t n_rows, const int32_t* dev_row_ptr, const int64_t* dev_target,
+                           const int64_t* dev_point, const float* dev_scale, int64_t n_steps, int stride);
 
 /* ---- Batches: many independent engines stepped together -----------------------------------------
  * No reference counterpart: the reference runs the angles/wavelengths/polarisations of an oblique sweep

@@ -67,8 +67,9 @@ struct Op {
   int S = 0, maxp = 0, has_h = 0, has_a = 0;
   const float *f = nullptr, *wmix = nullptr, *h0 = nullptr, *inv_a = nullptr, *sm = nullptr;
   void* state = nullptr;
-  // adjoint: tape rows, forward step count, DFT stride, gradient accumulator, second H target, H side
-  long long n_slots = 0, n_steps = 0;
+  // adjoint: tape rows, forward step count, DFT stride, gradient accumulator, second H target (adjoint source:
+  // the contributing point of each CSR entry), H side, monitor points
+  long long n_slots = 0, n_steps = 0, m_pts = 0;
   int dft_stride = 1, h_side = 0;
   double* gacc = nullptr;
   const long long* t2 = nullptr;
@@ -483,8 +484,9 @@ void run_op(lsf_engine* e, const Op& o, hipStream_t s) {
       break;
     }
     case OP_ADJ_SRC: {
-      lsf::AdjSrcJob J{o.h_side, o.width, o.n, o.omega, o.dt, (const c32*)o.w, o.target, o.t2, o.scale,
-                       (float*)(o.h_side ? e->H : e->E), (const long long*)e->counter, o.n_steps, o.dft_stride};
+      lsf::AdjSrcJob J{o.h_side, o.width, o.m_pts, o.omega, o.dt, (const c32*)o.w, o.n, o.row_ptr, o.target,
+                       o.t2, o.scale, (float*)(o.h_side ? e->H : e->E), (const long long*)e->counter, o.n_steps,
+                       o.dft_stride};
       if (o.n > 0) hipLaunchKernelGGL(lsf::k_adj_src, dim3(nblk(o.n, 256)), dim3(256), 0, s, J);
       break;
     }
@@ -936,12 +938,13 @@ int lsf_add_adjoint_gradient(lsf_engine* e, int64_t n, const int64_t* nodes, con
   return add_op(e, o);
 }
 
-int lsf_add_adjoint_source(lsf_engine* e, int h_side, int nf, int64_t np, const double* omega, double dt,
-                           const void* W, const int64_t* t1, const int64_t* t2, const float* scale, int64_t n_steps,
-                           int stride) {
+int lsf_add_adjoint_source(lsf_engine* e, int h_side, int nf, int64_t n_pts, const double* omega, double dt,
+                           const void* W, int64_t n_rows, const int32_t* row_ptr, const int64_t* target,
+                           const int64_t* point, const float* scale, int64_t n_steps, int stride) {
   if (valid(e)) return LSF_EINVAL;
   if (!e->adjoint) return fail(LSF_ESTATE, "adjoint sources belong to an adjoint engine (lsf_set_adjoint)");
-  if (nf < 1 || np < 1 || !omega || !W || !t1 || !scale || n_steps < 1 || stride < 1 || (h_side && !t2))
+  if (nf < 1 || n_pts < 1 || n_rows < 1 || !omega || !W || !row_ptr || !target || !point || !scale || n_steps < 1 ||
+      stride < 1)
     return fail(LSF_EINVAL, "bad adjoint source");
   Op o;
   o.kind = OP_ADJ_SRC;
@@ -949,12 +952,14 @@ int lsf_add_adjoint_source(lsf_engine* e, int h_side, int nf, int64_t np, const 
   o.field = h_side ? 1 : 0;
   o.h_side = h_side ? 1 : 0;
   o.width = nf;
-  o.n = np;
+  o.n = n_rows;
+  o.m_pts = n_pts;
   o.omega = omega;
   o.dt = dt;
   o.w = W;
-  o.target = (const long long*)t1;
-  o.t2 = (const long long*)t2;
+  o.row_ptr = (const int*)row_ptr;
+  o.target = (const long long*)target;
+  o.t2 = (const long long*)point;
   o.scale = scale;
   o.n_steps = n_steps;
   o.dft_stride = stride;

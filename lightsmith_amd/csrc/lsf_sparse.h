@@ -545,21 +545,25 @@ __global__ void k_adj_grad(GradJob J) {
   J.g[q] += (double)(J.E[J.node[q]] / J.cb[q]) * (double)J.tape[slot * J.n + q];
 }
 
-// Adjoint DFT source: F[target p] += scale_p * sum_f Re(W[f, p] exp(i omega_f t)) at sampling steps, where
-// W = conj(dL/dacc) in torch's gradient convention for the monitor accumulator acc[f, p]. E side (h_side = 0):
-// after an adjoint E half-step the state is E^n, n = n_steps - 1 - counter[1]; sampled when n > 0 and n % stride
-// == 0, t = n dt. H side (h_side = 1): after an adjoint H half-step the state is H^{n-1/2} with
+// Adjoint DFT source: F[node] += sum over the points p that touch the node of scale * sum_f Re(W[f, p] e^{i omega_f t})
+// at sampling steps, where W = conj(dL/dacc) in torch's gradient convention for the monitor accumulator acc[f, p].
+// E side (h_side = 0): after an adjoint E half-step the state is E^n, n = n_steps - 1 - counter[1]; sampled when
+// n > 0 and n % stride == 0, t = n dt. H side (h_side = 1): after an adjoint H half-step the state is H^{n-1/2} with
 // n = n_steps - counter[0], sampled at n with t = n dt - dt/2; each H point is the average of two nodes.
+// Points that share a node (edges and corners of a flux box) meet in one CSR row built on the host; a thread sums
+// its row in a fixed order, so the result does not depend on scheduling (no atomics).
 struct AdjSrcJob {
   int h_side;
   int nf;
-  long long np;
+  long long np;          // monitor points (columns of W)
   const double* omega;
   double dt;
   const c32* W;          // [nf][np]
-  const long long* t1;   // targets (E node, or first H node)
-  const long long* t2;   // second H node (H side)
-  const float* scale;    // [np] cb at the E node, or -cH / 2 for H
+  long long n_rows;      // distinct target nodes
+  const int* row_ptr;    // [n_rows + 1] into point / scale
+  const long long* target;  // [n_rows] native flat node
+  const long long* point;   // contributing monitor point per entry
+  const float* scale;    // per entry: cb at the E node, or -cH / 2 for an H node
   float* F;
   const long long* counter;
   long long n_steps;
@@ -567,20 +571,23 @@ struct AdjSrcJob {
 };
 
 __global__ void k_adj_src(AdjSrcJob J) {
-  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= J.np) return;
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= J.n_rows) return;
   const long long n = J.h_side ? J.n_steps - J.counter[0] : J.n_steps - 1 - J.counter[1];
   if (n <= 0 || n % J.stride != 0) return;
   const double t = (double)n * J.dt - (J.h_side ? 0.5 * J.dt : 0.0);
-  double v = 0.0;
-  for (int f = 0; f < J.nf; ++f) {
-    const double ph = J.omega[f] * t;
-    const c32 w = J.W[(long long)f * J.np + p];
-    v += (double)w.x * cos(ph) - (double)w.y * sin(ph);
+  double acc = 0.0;
+  for (int q = J.row_ptr[r]; q < J.row_ptr[r + 1]; ++q) {
+    const long long p = J.point[q];
+    double v = 0.0;
+    for (int f = 0; f < J.nf; ++f) {
+      const double ph = J.omega[f] * t;
+      const c32 w = J.W[(long long)f * J.np + p];
+      v += (double)w.x * cos(ph) - (double)w.y * sin(ph);
+    }
+    acc += v * (double)J.scale[q];
   }
-  const float add = (float)v * J.scale[p];
-  atomicAdd(J.F + J.t1[p], add);  // points may share a node (edges of a flux box)
-  if (J.h_side) atomicAdd(J.F + J.t2[p], add);
+  J.F[J.target[r]] += (float)acc;
 }
 
 }  // namespace lsf

@@ -58,6 +58,34 @@ class DesignRegion:
         return nodes, vox
 
 
+class _Rows:
+    """Monitor points grouped by the grid node they touch, as CSR rows in a fixed order (node, then point): the
+    adjoint source of points that share a node (edges and corners of a flux box) is summed per row by one thread,
+    so gradients do not depend on scheduling."""
+
+    def __init__(self, targets, points, dev):
+        targets, points = np.asarray(targets, dtype=np.int64), np.asarray(points, dtype=np.int64)
+        order = np.lexsort((points, targets))
+        t, p = targets[order], points[order]
+        uniq, start, count = np.unique(t, return_index=True, return_counts=True)
+        self.n_rows = len(uniq)
+        self.target = torch.as_tensor(uniq, device=dev)
+        self.entry_target = torch.as_tensor(t, device=dev)
+        self.point = torch.as_tensor(p, device=dev)
+        self.row_ptr = torch.as_tensor(np.append(start, len(t)).astype(np.int32), device=dev)
+        # padded (row, slot) -> entry map for a fixed-order row sum in torch (slot past the row: a zero entry)
+        width = int(count.max())
+        slot = np.arange(width)[None, :]
+        pad = np.where(slot < count[:, None], start[:, None] + slot, len(t))
+        self._pad = torch.as_tensor(pad, device=dev)
+
+    def row_sum(self, per_point: torch.Tensor, per_entry_scale: torch.Tensor) -> torch.Tensor:
+        """sum over each row's entries of per_entry_scale * per_point[point], in the rows' fixed order."""
+        vals = per_point[self.point] * per_entry_scale
+        vals = torch.cat([vals, vals.new_zeros(1)])
+        return vals[self._pad].sum(1)
+
+
 class AdjointProblem:
     """A real-field FDTD run (CPML, sources, one DFT monitor) whose design-region permittivity follows rho."""
 
@@ -105,24 +133,33 @@ class AdjointProblem:
               "lsf_add_adjoint_gradient")
         m = monitor
         nf = len(np.asarray(m.omega))
-        self._e_nodes = m._e
-        self._cb_e = self.fwd._mat[self._e_nodes].clone()
-        self.W_E = torch.zeros((nf, len(m._e)), dtype=torch.complex64, device=dev)
-        check(lib.lsf_add_adjoint_source(self.adj._h, 0, nf, len(m._e), ptr(m._omega), float(dt), ptr(self.W_E),
-                                         ptr(m._e), None, ptr(self._cb_e), self.n_steps, self.stride),
-              "lsf_add_adjoint_source")
+        e_nat = self.fwd.native_index(m.e_idx)
+        self._e_rows = _Rows(e_nat, np.arange(len(e_nat)), dev)
+        self._e_nodes = self._e_rows.target
+        self._cb_e = self.fwd._mat[self._e_rows.entry_target].clone()
+        self.W_E = torch.zeros((nf, len(e_nat)), dtype=torch.complex64, device=dev)
+        r = self._e_rows
+        check(lib.lsf_add_adjoint_source(self.adj._h, 0, nf, len(e_nat), ptr(m._omega), float(dt), ptr(self.W_E),
+                                         r.n_rows, ptr(r.row_ptr), ptr(r.target), ptr(r.point), ptr(self._cb_e),
+                                         self.n_steps, self.stride), "lsf_add_adjoint_source")
         self.W_H = None
         if m.acc_h is not None:
-            self.W_H = torch.zeros((nf, len(m._h1)), dtype=torch.complex64, device=dev)
-            self._h_scale = torch.full((len(m._h1),), -0.5 * self.fwd.cH, dtype=torch.float32, device=dev)
-            check(lib.lsf_add_adjoint_source(self.adj._h, 1, nf, len(m._h1), ptr(m._omega), float(dt), ptr(self.W_H),
-                                             ptr(m._h1), ptr(m._h2), ptr(self._h_scale), self.n_steps, self.stride),
-                  "lsf_add_adjoint_source")
+            h1, h2 = self.fwd.native_index(m.h_idx1), self.fwd.native_index(m.h_idx2)
+            n_h = len(h1)
+            self._h_rows = _Rows(np.concatenate([h1, h2]), np.tile(np.arange(n_h), 2), dev)
+            self.W_H = torch.zeros((nf, n_h), dtype=torch.complex64, device=dev)
+            self._h_scale = torch.full((len(self._h_rows.entry_target),), -0.5 * self.fwd.cH, dtype=torch.float32,
+                                       device=dev)
+            r = self._h_rows
+            check(lib.lsf_add_adjoint_source(self.adj._h, 1, nf, n_h, ptr(m._omega), float(dt), ptr(self.W_H),
+                                             r.n_rows, ptr(r.row_ptr), ptr(r.target), ptr(r.point),
+                                             ptr(self._h_scale), self.n_steps, self.stride), "lsf_add_adjoint_source")
         self._omega = torch.as_tensor(np.asarray(m.omega, dtype=float), device=dev)
         self.dt = float(dt)
         self._run = None
         self.checkpoints = []
         self.timings = {}
+        self.runs = 0  # forward runs so far: the tape, checkpoints and design cb belong to the latest one
 
     def use_checkpoints(self, on: bool):
         """Switch between the stored tape (no recompute) and checkpoint recompute (one interval of tape in use);
@@ -145,7 +182,7 @@ class AdjointProblem:
         cb = (self.fwd.cdx / eps)[self._vox].to(torch.float32)
         self.fwd._mat[self._nodes] = cb
         self.cb_design.copy_(cb)
-        self._cb_e.copy_(self.fwd._mat[self._e_nodes])
+        self._cb_e.copy_(self.fwd._mat[self._e_rows.entry_target])
 
     # --- runs -----------------------------------------------------------------------------------------
     def _zero(self, eng: Engine):
@@ -156,6 +193,7 @@ class AdjointProblem:
         """Forward run for density rho: the monitor's (acc_e, acc_h) after n_steps (copies)."""
         self.set_density(rho)
         self._zero(self.fwd)
+        self.runs += 1
         fwd = self.fwd
         self.base.fill_(0 if self.store_all else -(1 << 62))
         self.checkpoints = []
@@ -182,7 +220,8 @@ class AdjointProblem:
         t = self.n_steps * self.dt
         ph = torch.exp(1j * self._omega * t).to(torch.complex128)
         src = torch.real(self.W_E.to(torch.complex128) * ph[:, None]).sum(0)
-        adj.E_flat.index_add_(0, self._e_nodes, (src * self._cb_e.to(torch.float64)).to(torch.float32))
+        per_node = self._e_rows.row_sum(src, self._cb_e.to(torch.float64))
+        adj.E_flat[self._e_nodes] += per_node.to(torch.float32)
         if self.store_all:
             self.base.fill_(0)
             adj.steps(self.n_steps)
@@ -214,11 +253,20 @@ class _MonitorSpectra(torch.autograd.Function):
         ctx.prob = prob
         ctx.shape = rho.shape
         ctx.dtype = rho.dtype
-        return prob.forward(rho.detach())
+        out = prob.forward(rho.detach())
+        ctx.run = prob.runs
+        ctx.save_for_backward(rho.detach().clone())
+        return out
 
     @staticmethod
     def backward(ctx, grad_e, grad_h):
-        g = ctx.prob.backward(grad_e, grad_h)
+        prob = ctx.prob
+        if prob.runs != ctx.run:
+            # the problem ran another density since this output was made (two designs in one loss, a graph kept
+            # across a line search): the tape, checkpoints and design cb are that run's, so redo this one first
+            (rho,) = ctx.saved_tensors
+            prob.forward(rho)
+        g = prob.backward(grad_e, grad_h)
         return g.reshape(ctx.shape).to(ctx.dtype), None
 
 

@@ -221,7 +221,7 @@ def synthetic_table_source(ns, shape, seed=0, m=3, n_rows=40, n_pts=None):
 # --- adjoint ------------------------------------------------------------------------------------------
 
 def adjoint_case(shape=(34, 28, 22), pml=6, steps=240, stride=8, seed=4, design=((13, 9, 8), (21, 19, 14)),
-                 wavelength=400.0):
+                 wavelength=400.0, monitor="plane"):
     """A small SOI-like adjoint problem: clad eps 1.44^2 with a core bar along x (eps 3.48^2), CPML with kappa
     and alpha, a mode-injection soft source on an x plane (E_y, Gaussian profile, Gaussian pulse), a design box
     of the given cells, and a two-frequency DFT monitor (E_y, E_z and face-averaged H_z, H_y) on an x plane
@@ -254,6 +254,10 @@ def adjoint_case(shape=(34, 28, 22), pml=6, steps=240, stride=8, seed=4, design=
     e_mon = np.concatenate([flat_index(1, im, jm, km, shape), flat_index(2, im, jm, km, shape)])
     h1 = np.concatenate([flat_index(2, im - 1, jm, km, shape), flat_index(1, im - 1, jm, km, shape)])
     h2 = np.concatenate([flat_index(2, im, jm, km, shape), flat_index(1, im, jm, km, shape)])
+    if monitor == "box":
+        # a closed flux box downstream of the design: its edge and corner nodes are shared by two or three faces
+        from lightsmith_amd.fdtd.run import flux_points
+        e_mon, h1, h2, _ = flux_points((i_mon - 3, jc - 5, kc - 4), (i_mon + 1, jc + 5, kc + 4), shape, dx)
     omega = omega_c * np.array([0.95, 1.05])
     rho = rng.uniform(0, 1, size=tuple(h - l for l, h in zip(lo, hi)))
     w_e = rng.standard_normal(len(e_mon)) + 1j * rng.standard_normal(len(e_mon))
