@@ -1,0 +1,336 @@
+"""Plane-wave scattering by particles, solved end to end on the HIP engine (scattering.py:232-500 of the
+reference, for a homogeneous host on a uniform grid):
+
+    plan   -> cell size, grid layout, dt, DFT stride, step cap            (scattering.py:232-280)
+    set-up -> ADE media, voxelised materials (subpixel / conformal),       (scattering.py:301-346)
+              CPML, the table-fed plane wave, scattered- and total-field flux boxes
+    run    -> run_until_decayed on the engine (HIP graphs per DFT chunk)  (setup.py:722-773)
+    post   -> C_sca = outward flux of the scattered field / I, C_abs = -(net total-field flux - the background's
+              own) / I, C_ext = C_sca + C_abs, I = n |E_inc|^2 / 2          (scattering.py:428-464)
+
+Not on this path (the backend's ``supports`` declines them): substrates, Gaussian beams, near/far fields,
+resonances, mirror symmetry (runs are full grids), graded grids, tabulated materials (no Drude-Lorentz fit).
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .engine import DFTMonitor, Engine, PMLSpec
+from .layered import OMEGA_H, ObliqueLayeredPlaneWave
+from .media import HBARC_EV_NM, Medium, Pole, time_step
+from .run import FS, flux, flux_points, run_until_decayed
+from .voxel import ParticleSlots, voxelise
+
+
+# --- media (media.py:60-129) --------------------------------------------------------------------------------------
+
+def _from_drude_lorentz(eps_inf, wp_ev, gamma_ev, lorentz, label) -> Medium:
+    poles = []
+    if wp_ev > 0:
+        poles.append(Pole((wp_ev / HBARC_EV_NM) ** 2, 0.0, gamma_ev / HBARC_EV_NM))
+    for de, w0_ev, g_ev in lorentz:
+        w0 = w0_ev / HBARC_EV_NM
+        poles.append(Pole(de * w0 ** 2, w0, g_ev / HBARC_EV_NM))
+    return Medium(float(eps_inf), tuple(poles), label)
+
+
+def _matched_constant(eps: complex, omega_c: float, label: str) -> Medium:
+    """A constant complex eps as a Drude term exact at omega_c (media.py:77-91): lossless eps >= 1 is a plain
+    dielectric; otherwise eps_inf = max(1, eps' + eps''/20) and a Drude pole makes up the rest."""
+    er, ei = eps.real, eps.imag
+    if ei == 0 and er >= 1:
+        return Medium(er, (), label)
+    eps_inf = max(1.0, er + ei / 20)
+    delta = eps_inf - er
+    gamma = omega_c * ei / delta
+    wp2 = omega_c ** 2 * (delta ** 2 + ei ** 2) / delta
+    return Medium(eps_inf, (Pole(wp2, 0.0, gamma),), label)
+
+
+def to_medium(material, wavelengths_nm, warnings: list) -> Medium:
+    """An ADE medium for one material over the band (media.py:94-129)."""
+    wl = np.asarray(wavelengths_nm, dtype=float)
+    kind = getattr(material, "kind", type(material).__name__)
+    if kind == "drude":
+        return _from_drude_lorentz(material.eps_inf, material.wp_ev, material.gamma_ev, (), kind)
+    if kind == "drude_lorentz":
+        m = _from_drude_lorentz(material.eps_inf, material.wp_ev, material.gamma_ev,
+                                [(p.delta_eps, p.omega0_ev, p.gamma_ev) for p in material.poles], kind)
+        if any(p.sigma < 0 for p in m.poles):
+            warnings.append("DrudeLorentz with negative oscillator strength: the FDTD update may be unstable")
+        return m
+    if kind in ("constant_index", "constant_permittivity"):
+        eps = complex(material.eps(np.array([wl[0]]))[0])
+        omega = 2 * np.pi / wl
+        omega_c = 0.5 * (omega.min() + omega.max())
+        m = _matched_constant(eps, omega_c, kind)
+        if m.dispersive and wl.size > 1:
+            dev = float(np.max(np.abs(m.eps(omega) - eps)) / abs(eps))
+            if dev > 1e-3:
+                warnings.append(f"{kind} eps={eps:.4g} is not causal as a constant; FDTD models it as a Drude term "
+                                f"exact at {2 * np.pi / omega_c:.0f} nm (max deviation {dev:.1%} over the band). "
+                                "Use a Drude/DrudeLorentz material for broadband accuracy.")
+        return m
+    raise ValueError(f"FDTD on this path does not support material kind {kind!r} (give Drude or DrudeLorentz "
+                     "parameters; tabulated materials need a Drude-Lorentz fit)")
+
+
+def particle_media(particles, medium_index: float, wl, warnings) -> tuple[list[Medium], list[list[int]]]:
+    """ADE media (0 = the host) and each particle's region -> medium slots; equal materials share a medium
+    (scattering.py:102-116)."""
+    media = [Medium(medium_index ** 2, (), "medium")]
+    keys: dict[str, int] = {}
+    slot_of = []
+    for p in particles:
+        slots = []
+        for m in p.region_materials():
+            key = m.model_dump_json()
+            if key not in keys:
+                keys[key] = len(media)
+                media.append(to_medium(m, wl, warnings))
+            slots.append(keys[key])
+        slot_of.append(slots)
+    return media, slot_of
+
+
+# --- planning ------------------------------------------------------------------------------------------------------
+
+def band(sim) -> tuple[float, float, float]:
+    """(omega_c, omega_half, omega_max) of the pulse covering the wavelengths (scattering.py:143-152)."""
+    om = 2 * np.pi / sim.wavelength_array()
+    omega_c = 0.5 * (om.min() + om.max())
+    return float(omega_c), float(max(0.5 * (om.max() - om.min()), 0.15 * omega_c)), float(om.max())
+
+
+def _n_max(materials, wl) -> tuple[float, bool]:
+    n_max, metal = 0.0, False
+    for m in materials:
+        eps = np.asarray(m.eps(wl), dtype=complex)
+        n_max = max(n_max, float(np.abs(np.sqrt(eps)).max()))
+        metal |= bool(np.any(eps.real < 0))
+    return n_max, metal
+
+
+def spacing(sim, opts) -> float:
+    """Cell size: spacing_nm, else the shortest wavelength over cells_per_wavelength in the densest material,
+    at most the smallest particle half-extent over cells_per_radius (10, or 20 with a metal) (scattering.py:119-133)."""
+    if opts.spacing_nm:
+        return float(opts.spacing_nm)
+    wl = sim.wavelength_array()
+    n_max, metal = _n_max([m for p in sim.particles for m in p.region_materials()], wl)
+    n_max = max(n_max, sim.medium_index)
+    cpr = opts.cells_per_radius or (20.0 if metal else 10.0)
+    dx = wl.min() / (n_max * opts.cells_per_wavelength)
+    return min(dx, min(float(np.min(p.shape.bounding_half_extent())) for p in sim.particles) / cpr)
+
+
+@dataclass
+class Layout:
+    """Grid around the particles (setup.py:572-647): node (0,0,0) at origin; TFSF box; total-field flux box 2 cells
+    inside it, scattered-field box 2 cells outside; region examined by the rasteriser; CPML thickness."""
+
+    dx: float
+    shape: tuple
+    origin: np.ndarray
+    center: np.ndarray
+    tfsf: tuple
+    tot: tuple
+    sca: tuple
+    roi: tuple
+    pml: int
+
+    @property
+    def cells(self) -> int:
+        return int(np.prod(self.shape))
+
+
+def layout(sim, opts, dx: float) -> Layout:
+    """setup.py:627-647: odd shape centred on the particles' bounding box; padding and gap 4 cells, at least 3 % of
+    the longest wavelength in the host; CPML 10 cells by default."""
+    lo = np.min([np.asarray(p.position_nm) - p.shape.bounding_half_extent() for p in sim.particles], axis=0)
+    hi = np.max([np.asarray(p.position_nm) + p.shape.bounding_half_extent() for p in sim.particles], axis=0)
+    ctr = (lo + hi) / 2
+    half_cells = np.ceil((hi - lo) / 2 / dx - 1e-9).astype(int) + 1
+    floor = 0.03 * float(np.max(sim.wavelength_array())) / sim.medium_index
+    n_pad = max(4, math.ceil((floor if opts.padding_nm is None else opts.padding_nm) / dx - 1e-9))
+    n_gap = max(4, math.ceil((floor if opts.pml_gap_nm is None else opts.pml_gap_nm) / dx - 1e-9))
+    half_tfsf = half_cells + n_pad
+    n_pml = opts.pml_cells or 10
+    half_tot = half_tfsf + 2 + n_gap + n_pml + 1
+    shape = tuple(int(v) for v in 2 * half_tot + 1)
+    c = half_tot.astype(int)
+    roi = (np.maximum(c - half_cells - 2, 1), np.minimum(c + half_cells + 3, np.asarray(shape) - 1))
+    return Layout(dx=float(dx), shape=shape, origin=ctr - c * dx, center=c, tfsf=(c - half_tfsf, c + half_tfsf),
+                  tot=(c - half_tfsf + 2, c + half_tfsf - 2), sca=(c - half_tfsf - 2, c + half_tfsf + 2), roi=roi,
+                  pml=n_pml)
+
+
+def plan(sim, opts, media=None, warnings=None) -> dict:
+    """Everything about the run that is cheap to compute (scattering.py:232-280)."""
+    warnings = [] if warnings is None else warnings
+    if media is None:
+        media, _ = particle_media(sim.particles, sim.medium_index, sim.wavelength_array(), warnings)
+    dx = spacing(sim, opts)
+    lay = layout(sim, opts, dx)
+    dt = time_step(dx, opts.courant, media)
+    omega_c, omega_half, omega_max = band(sim)
+    # sampling every `stride` steps aliases content above omega_c + 6 omega_half onto the band's top, never into it
+    # (the table-fed wave is compared with its exact background: the e^-36 margin of 6 omega_half)
+    stride = opts.dft_stride or max(1, int(2 * np.pi / (omega_c + 6 * omega_half + omega_max) / dt))
+    max_t = (opts.run_time_fs or opts.max_time_fs) * FS
+    max_steps = int(math.ceil(max_t / dt / stride)) * stride
+    tau = 2 / omega_half
+    t_pulse = 7 * tau + lay.shape[0] * dx * sim.medium_index
+    gammas = [p.gamma for m in media for p in m.poles if p.sigma > 0]
+    t_ring = math.log(1 / opts.decay_by) / max(min(gammas), 1e-6) if gammas else 4 * t_pulse / 7
+    est_steps = max_steps if opts.run_time_fs else min(max_steps, int((t_pulse + t_ring) / dt))
+    n_flux = 2 * 12 * int(np.prod(lay.sca[1] - lay.sca[0] + 1) ** (2 / 3))
+    n_f = len(sim.wavelength_array())
+    ext = lay.tfsf[1] - lay.tfsf[0]
+    n_k = int(ext.max()) + 5
+    t_tab = 2 * max(5 * tau, 30 / (2 * np.pi / float(np.max(sim.wavelength_array())))) + 4 * tau + \
+        (4 * n_k + 2 * float(np.linalg.norm(ext))) * dx * sim.medium_index
+    n_t = 1 << int(np.ceil(np.log2(t_tab / dt)))
+    m = max(1, int(OMEGA_H / ((omega_c + 3 * omega_half) * dt)))
+    mem = 4 * (12 * lay.cells + 4 * n_flux * n_f) + 4 * 6 * n_k * (n_t // m + 8)
+    return {"dx": dx, "layout": lay, "dt": dt, "stride": stride, "max_steps": max_steps, "est_steps": est_steps,
+            "omega_c": omega_c, "omega_half": omega_half, "media": media, "memory_bytes": int(mem),
+            "warnings": warnings}
+
+
+# --- solve ---------------------------------------------------------------------------------------------------------
+
+@dataclass
+class Scene:
+    """The host inputs of one scattering run (what the engine is built from): layout, time step, DFT stride, step
+    cap, material arrays and groups, CPML, the table-fed source and the monitors' index sets."""
+
+    media: list
+    layout: Layout
+    dt: float
+    stride: int
+    max_steps: int
+    inv_eps: np.ndarray
+    groups: list
+    offdiag: object
+    pml: PMLSpec
+    source: ObliqueLayeredPlaneWave
+    omega: np.ndarray
+    sca: tuple
+    tot: tuple
+    stats: dict
+    mode: str
+    warnings: list
+    voxelise_s: float
+
+    def monitors(self):
+        """(scattered-field box, total-field box, incident reference) DFT monitors, energy groups 1 and 2."""
+        return [DFTMonitor(self.omega, self.sca[0], self.sca[1], self.sca[2], energy=1),
+                DFTMonitor(self.omega, self.tot[0], self.tot[1], self.tot[2], energy=2),
+                DFTMonitor(self.omega, np.arange(3), source=self.source.view)]
+
+    def engine(self, device, **kw) -> Engine:
+        """The HIP engine of the scene, with fresh monitors."""
+        return Engine(device, shape=self.layout.shape, dx=self.layout.dx, dt=self.dt, single=True,
+                      inv_eps=self.inv_eps, groups=self.groups, pml=self.pml, sources=[self.source],
+                      monitors=self.monitors(), dft_stride=self.stride, offdiag=self.offdiag, **kw)
+
+
+def _raster_device(device):
+    """The torch device the rasteriser classifies points on: the engine's GPU, or the CPU for host-only set-up."""
+    if device is None:
+        return torch.device("cpu")
+    if hasattr(device, "framework") and hasattr(device, "name"):
+        return torch.device(device.name)
+    return torch.device(device)
+
+
+def setup(sim, opts, device) -> Scene:
+    """Plan, media, voxelised materials (on the device), CPML, table-fed source and flux boxes of ``sim``
+    (scattering.py:301-390)."""
+    warnings: list[str] = []
+    wl = sim.wavelength_array()
+    media, slot_of = particle_media(sim.particles, sim.medium_index, wl, warnings)
+    p = plan(sim, opts, media, warnings)
+    lay, dx, dt, stride = p["layout"], p["dx"], p["dt"], p["stride"]
+    if lay.cells > opts.max_cells:
+        raise ValueError(f"grid {lay.shape} has {lay.cells} cells > max_cells={opts.max_cells}; "
+                         "increase spacing_nm or raise max_cells")
+    khat, ehat = sim.illumination.unit_vectors()
+    eps_b = sim.medium_index ** 2
+    mode = opts.mesh_refinement or ("conformal" if opts.subpixel else "staircase")
+    t0 = time.perf_counter()
+    inv_eps, groups, vstats, offdiag = voxelise(ParticleSlots(sim.particles, slot_of), media, lay.shape, lay.origin,
+                                                dx, lay.roi[0], lay.roi[1], mode=mode, samples=opts.subpixel_samples,
+                                                device=_raster_device(device))
+    t_vox = time.perf_counter() - t0
+    lam_max = float(wl.max())
+    pml = PMLSpec(cells=((lay.pml, lay.pml),) * 3, kappa_max=opts.pml_kappa,
+                  alpha_max=0.05 * 2 * np.pi * sim.medium_index / lam_max, eps_ref=eps_b)
+    src = ObliqueLayeredPlaneWave(shape=lay.shape, origin=lay.origin, spacing=dx, dt=dt, lo=lay.tfsf[0],
+                                  hi=lay.tfsf[1], khat=khat, ehat=ehat, inv_eps=inv_eps, host=media[0],
+                                  omega_c=p["omega_c"], omega_half=p["omega_half"],
+                                  max_steps=p["max_steps"] + 4 * stride, omega_min=2 * np.pi / lam_max)
+    return Scene(media=media, layout=lay, dt=dt, stride=stride, max_steps=p["max_steps"], inv_eps=inv_eps,
+                 groups=groups, offdiag=offdiag, pml=pml, source=src, omega=2 * np.pi / wl,
+                 sca=flux_points(*lay.sca, lay.shape, dx), tot=flux_points(*lay.tot, lay.shape, dx), stats=vstats,
+                 mode=mode, warnings=warnings, voxelise_s=t_vox)
+
+
+def cross_sections(scene: Scene, e_ref, es, hs, et, ht) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(c_sca, c_abs, c_ext) from the monitors' spectra (scattering.py:428-464): incident intensity at the
+    reference point n |E|^2 / 2; the background's own flux through the total-field box from the same flux formula."""
+    src, omega, tot = scene.source, scene.omega, scene.tot
+    intensity = 0.5 * src.n_inc * np.sum(np.abs(e_ref) ** 2, axis=1)
+    unit = src.incident_reference(omega)
+    amp = np.sum(e_ref * np.conj(unit), axis=1) / np.sum(np.abs(unit) ** 2, axis=1)
+    e_bg = amp[:, None] * src.background(omega, tot[0], "E")
+    h_bg = amp[:, None] * 0.5 * (src.background(omega, tot[1], "H") + src.background(omega, tot[2], "H"))
+    bg_out = flux(e_bg, h_bg, tot[3])
+    c_sca = flux(es, hs, scene.sca[3]) / intensity
+    c_abs = -(flux(et, ht, tot[3]) - bg_out) / intensity
+    return c_sca, c_abs, c_sca + c_abs
+
+
+def solve(sim, opts, device) -> dict:
+    """Cross sections of ``sim`` (c_sca, c_abs, c_ext in nm^2 per wavelength) with provenance and warnings."""
+    t_start = time.perf_counter()
+    scene = setup(sim, opts, device)
+    warnings = scene.warnings
+    lay, dt, stride = scene.layout, scene.dt, scene.stride
+    eng = scene.engine(device)
+    mon_sca, mon_tot, mon_inc = eng.monitors
+    t_loop = time.perf_counter()
+    try:
+        steps, level, graphs = run_until_decayed(eng, opts, max_steps=scene.max_steps,
+                                                 min_steps=int(scene.source.t_passed / dt),
+                                                 window=math.ceil(float(2 * np.pi / scene.omega.min()) / (stride * dt)),
+                                                 warnings=warnings)
+        t_loop = time.perf_counter() - t_loop
+        e_ref, _ = mon_inc.results()
+        es, hs = mon_sca.results()
+        et, ht = mon_tot.results()
+    finally:
+        eng.close()
+    c_sca, c_abs, c_ext = cross_sections(scene, e_ref, es, hs, et, ht)
+    if np.any(c_abs < -1e-3 * np.max(np.abs(c_ext))):
+        warnings.append("negative absorption beyond 0.1% of extinction: increase resolution or decay time")
+    prov = {"device": str(eng._dev), "grid": list(lay.shape), "cells": lay.cells, "spacing_nm": lay.dx,
+            "dt_fs": dt / FS, "steps": steps, "simulated_time_fs": steps * dt / FS, "dft_stride": stride,
+            "precision": "single", "hip_graphs": graphs, "subpixel": scene.mode != "staircase", "pml_cells": lay.pml,
+            "tfsf_box_nm": float((lay.tfsf[1] - lay.tfsf[0])[0] * lay.dx), "decay_level": level,
+            "materials": [m.describe() for m in scene.media], **scene.stats, "symmetry": "off",
+            "tfsf_table_stride": scene.source.m, "source": "table-fed plane wave",
+            "runtime_s": time.perf_counter() - t_start, "time_loop_s": t_loop, "voxelise_s": scene.voxelise_s}
+    return {"c_sca": c_sca, "c_abs": c_abs, "c_ext": c_ext, "near_field": None, "far_field": None,
+            "resonances": None, "provenance": prov, "warnings": warnings}
+
+
+__all__ = ["Layout", "Scene", "band", "cross_sections", "layout", "particle_media", "plan", "setup", "solve",
+           "spacing", "to_medium"]

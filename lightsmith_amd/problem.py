@@ -1,0 +1,340 @@
+"""Problem and result types of the FDTD front door: the subset of lightsmith's typed specification that plane-wave
+scattering by particles uses (spec.py, materials.py, results.py, backends/_fdtd/options.py of the reference), with
+the same field names, defaults and validation, so that a reference ``Simulation`` serialised to JSON parses here
+and a ``Result`` produced here has the reference's fields.
+
+Units: nm (vacuum wavelengths), exp(-i omega t), cross sections in nm^2, energies in eV.
+"""
+
+from __future__ import annotations
+
+from typing import Annotated, Any, Literal, Union
+
+import numpy as np
+from pydantic import BaseModel, ConfigDict, Field, field_validator, model_validator
+
+HC_EV_NM = 1239.841984  # h c in eV nm (materials.py:32)
+
+
+class _Model(BaseModel):
+    model_config = ConfigDict(extra="forbid", frozen=True)
+
+
+# --- materials (materials.py:49-113) -----------------------------------------------------------------------------
+
+class ConstantIndex(_Model):
+    kind: Literal["constant_index"] = "constant_index"
+    n_real: float
+    n_imag: float = Field(0.0, ge=0)
+
+    def eps(self, wavelength_nm):
+        wl = np.asarray(wavelength_nm, dtype=float)
+        return np.full(wl.shape, complex(self.n_real, self.n_imag) ** 2)
+
+
+class ConstantPermittivity(_Model):
+    kind: Literal["constant_permittivity"] = "constant_permittivity"
+    eps_real: float
+    eps_imag: float = Field(0.0, ge=0)
+
+    def eps(self, wavelength_nm):
+        wl = np.asarray(wavelength_nm, dtype=float)
+        return np.full(wl.shape, complex(self.eps_real, self.eps_imag))
+
+
+class Drude(_Model):
+    """eps(w) = eps_inf - wp^2 / (w^2 + i gamma w), energies in eV."""
+
+    kind: Literal["drude"] = "drude"
+    eps_inf: float = 1.0
+    wp_ev: float = Field(gt=0)
+    gamma_ev: float = Field(ge=0)
+
+    def eps(self, wavelength_nm):
+        w = HC_EV_NM / np.asarray(wavelength_nm, dtype=float)
+        return self.eps_inf - self.wp_ev ** 2 / (w ** 2 + 1j * self.gamma_ev * w)
+
+
+class LorentzPole(_Model):
+    delta_eps: float
+    omega0_ev: float = Field(gt=0)
+    gamma_ev: float = Field(ge=0)
+
+
+class DrudeLorentz(_Model):
+    """eps(w) = eps_inf - wp^2/(w^2 + i g w) + sum_j d_j w0_j^2 / (w0_j^2 - w^2 - i G_j w)."""
+
+    kind: Literal["drude_lorentz"] = "drude_lorentz"
+    eps_inf: float = 1.0
+    wp_ev: float = Field(0.0, ge=0)
+    gamma_ev: float = Field(0.0, ge=0)
+    poles: list[LorentzPole] = Field(default_factory=list)
+
+    def eps(self, wavelength_nm):
+        w = HC_EV_NM / np.asarray(wavelength_nm, dtype=float)
+        e = self.eps_inf + 0j * w
+        if self.wp_ev > 0:
+            e = e - self.wp_ev ** 2 / (w ** 2 + 1j * self.gamma_ev * w)
+        for p in self.poles:
+            e = e + p.delta_eps * p.omega0_ev ** 2 / (p.omega0_ev ** 2 - w ** 2 - 1j * p.gamma_ev * w)
+        return e
+
+
+Material = Annotated[Union[ConstantIndex, ConstantPermittivity, Drude, DrudeLorentz], Field(discriminator="kind")]
+
+
+# --- shapes (spec.py:29-121) -------------------------------------------------------------------------------------
+
+class Sphere(_Model):
+    kind: Literal["sphere"] = "sphere"
+    radius_nm: float = Field(gt=0)
+
+    def volume(self) -> float:
+        return 4 / 3 * np.pi * self.radius_nm ** 3
+
+    def bounding_half_extent(self) -> np.ndarray:
+        return np.full(3, self.radius_nm)
+
+
+class Spheroid(_Model):
+    kind: Literal["spheroid"] = "spheroid"
+    semi_axis_xy_nm: float = Field(gt=0)
+    semi_axis_z_nm: float = Field(gt=0)
+
+    def volume(self) -> float:
+        return 4 / 3 * np.pi * self.semi_axis_xy_nm ** 2 * self.semi_axis_z_nm
+
+    def bounding_half_extent(self) -> np.ndarray:
+        return np.array([self.semi_axis_xy_nm, self.semi_axis_xy_nm, self.semi_axis_z_nm])
+
+
+class Cylinder(_Model):
+    kind: Literal["cylinder"] = "cylinder"
+    radius_nm: float = Field(gt=0)
+    length_nm: float = Field(gt=0)
+
+    def volume(self) -> float:
+        return np.pi * self.radius_nm ** 2 * self.length_nm
+
+    def bounding_half_extent(self) -> np.ndarray:
+        return np.array([self.radius_nm, self.radius_nm, self.length_nm / 2])
+
+
+class Shell(_Model):
+    thickness_nm: float = Field(gt=0)
+    material: Material
+
+
+class LayeredSphere(_Model):
+    kind: Literal["layered_sphere"] = "layered_sphere"
+    core_radius_nm: float = Field(gt=0)
+    shells: list[Shell] = Field(min_length=1)
+
+    @property
+    def radii_nm(self) -> list[float]:
+        r = [self.core_radius_nm]
+        for s in self.shells:
+            r.append(r[-1] + s.thickness_nm)
+        return r
+
+    @property
+    def radius_nm(self) -> float:
+        return self.radii_nm[-1]
+
+    def volume(self) -> float:
+        return 4 / 3 * np.pi * self.radius_nm ** 3
+
+    def bounding_half_extent(self) -> np.ndarray:
+        return np.full(3, self.radius_nm)
+
+
+Shape = Annotated[Union[Sphere, Spheroid, Cylinder, LayeredSphere], Field(discriminator="kind")]
+
+
+class Particle(_Model):
+    shape: Shape
+    material: Material
+    position_nm: tuple[float, float, float] = (0.0, 0.0, 0.0)
+
+    def region_materials(self) -> list:
+        """Materials by region: [core/body, shell 1, shell 2, ...]."""
+        return [self.material, *[s.material for s in getattr(self.shape, "shells", [])]]
+
+
+# --- illumination, wavelengths, outputs (spec.py:135-246) ----------------------------------------------------------
+
+class PlaneWave(_Model):
+    kind: Literal["plane_wave"] = "plane_wave"
+    direction: tuple[float, float, float] = (0.0, 0.0, 1.0)
+    polarization: tuple[float, float, float] = (1.0, 0.0, 0.0)
+
+    @model_validator(mode="after")
+    def _orthogonal(self):
+        k, e = np.asarray(self.direction), np.asarray(self.polarization)
+        if np.linalg.norm(k) == 0 or np.linalg.norm(e) == 0:
+            raise ValueError("direction and polarization must be non-zero")
+        if abs(k @ e) > 1e-9 * np.linalg.norm(k) * np.linalg.norm(e):
+            raise ValueError("polarization must be orthogonal to direction")
+        return self
+
+    def unit_vectors(self) -> tuple[np.ndarray, np.ndarray]:
+        k, e = np.asarray(self.direction, float), np.asarray(self.polarization, float)
+        return k / np.linalg.norm(k), e / np.linalg.norm(e)
+
+
+class WavelengthRange(_Model):
+    start_nm: float = Field(gt=0)
+    stop_nm: float = Field(gt=0)
+    num: int = Field(ge=1)
+
+    def values(self) -> np.ndarray:
+        return np.linspace(self.start_nm, self.stop_nm, self.num)
+
+
+class Outputs(_Model):
+    """Only cross sections are produced on this path; near-/far-field and resonance requests are declined by the
+    backend's ``supports``."""
+
+    cross_sections: bool = True
+    near_field: dict | None = None
+    far_field: dict | None = None
+    resonances: dict | None = None
+
+
+class Simulation(_Model):
+    """A plane-wave scattering problem (spec.py:251-296)."""
+
+    kind: Literal["scattering"] = "scattering"
+    schema_version: Literal["0.1"] = "0.1"
+    particles: list[Particle] = Field(min_length=1)
+    medium_index: float = Field(1.0, gt=0)
+    illumination: PlaneWave = PlaneWave()
+    wavelengths: list[float] | WavelengthRange
+    substrate: dict | None = None
+    orientation_average: bool = False
+    outputs: Outputs = Outputs()
+    options: dict[str, dict[str, Any]] = Field(default_factory=dict)
+
+    @field_validator("wavelengths")
+    @classmethod
+    def _positive(cls, v):
+        if isinstance(v, list) and (not v or min(v) <= 0):
+            raise ValueError("wavelengths must be a non-empty list of positive values")
+        return v
+
+    def options_for(self, backend_name: str, method: str) -> dict[str, Any]:
+        return {**self.options.get(method, {}), **self.options.get(backend_name, {})}
+
+    def wavelength_array(self) -> np.ndarray:
+        w = self.wavelengths
+        return w.values() if isinstance(w, WavelengthRange) else np.asarray(w, dtype=float)
+
+    def equivalent_radius_nm(self) -> float:
+        v = sum(p.shape.volume() for p in self.particles)
+        return (3 * v / (4 * np.pi)) ** (1 / 3)
+
+
+def options_for(problem, backend_name: str, method: str) -> dict[str, Any]:
+    """Method-level options overlaid with backend-specific ones (problems.py:348-350)."""
+    opts = getattr(problem, "options", {}) or {}
+    return {**opts.get(method, {}), **opts.get(backend_name, {})}
+
+
+# --- FDTD options (backends/_fdtd/options.py:28-92) -----------------------------------------------------------------
+
+class FDTDOptions(BaseModel):
+    """The reference's FDTDOptions fields. Values this path does not run are accepted by the model and declined by
+    the backend: precision "double", grid "auto", mesh_override, symmetry "auto" is run as "off" (full grid)."""
+
+    model_config = ConfigDict(extra="forbid")
+
+    spacing_nm: float | None = Field(None, gt=0)
+    cells_per_wavelength: float = Field(20.0, ge=6)
+    cells_per_radius: float | None = Field(None, ge=2)
+    pml_cells: int | None = Field(None, ge=4)
+    pml_kappa: float = Field(1.0, ge=1)
+    padding_nm: float | None = Field(None, ge=0)
+    pml_gap_nm: float | None = Field(None, ge=0)
+    courant: float = Field(0.95, gt=0, le=1)
+    run_time_fs: float | None = Field(None, gt=0)
+    decay_by: float = Field(1e-5, gt=0, lt=1)
+    max_time_fs: float = Field(2000.0, gt=0)
+    precision: Literal["auto", "single", "double"] = "auto"
+    subpixel: bool = True
+    mesh_refinement: Literal["staircase", "volume_average", "subpixel", "conformal"] | None = None
+    subpixel_samples: int = Field(8, ge=2, le=24)
+    dft_stride: int | None = Field(None, ge=1)
+    symmetry: Literal["auto", "off"] = "auto"
+    polarization: Literal["te", "tm"] | None = None
+    grid: Literal["uniform", "auto"] = "uniform"
+    mesh_override: list[dict] = Field(default_factory=list)
+    background_cells_per_wavelength: float = Field(60.0, ge=10)
+    grading_ratio: float = Field(1.25, ge=1.05, le=1.5)
+    cuda_graphs: bool = True
+    max_cells: int = Field(300_000_000, ge=1)
+
+
+def mesh_mode(opts: FDTDOptions) -> str:
+    """The interface treatment: mesh_refinement, else conformal (subpixel=True) or staircase (setup.py:101-105)."""
+    if opts.mesh_refinement is not None:
+        return opts.mesh_refinement
+    return "conformal" if opts.subpixel else "staircase"
+
+
+# --- result (results.py:39-88) ---------------------------------------------------------------------------------------
+
+class Result(BaseModel):
+    """Cross sections of a scattering problem."""
+
+    model_config = ConfigDict(extra="forbid")
+
+    kind: Literal["cross_sections"] = "cross_sections"
+    method: str
+    backend: str
+    wavelengths_nm: list[float]
+    c_ext_nm2: list[float]
+    c_sca_nm2: list[float]
+    c_abs_nm2: list[float]
+    equivalent_radius_nm: float
+    near_field: list | None = None
+    far_field: list | None = None
+    resonances: list | None = None
+    provenance: dict[str, Any] = Field(default_factory=dict)
+    warnings: list[str] = Field(default_factory=list)
+
+    @classmethod
+    def from_arrays(cls, *, method, backend, wavelengths_nm, c_ext, c_sca, c_abs, r_eq, **kw) -> "Result":
+        f = lambda a: np.asarray(a, dtype=float).tolist()  # noqa: E731
+        return cls(method=method, backend=backend, wavelengths_nm=f(wavelengths_nm), c_ext_nm2=f(c_ext),
+                   c_sca_nm2=f(c_sca), c_abs_nm2=f(c_abs), equivalent_radius_nm=float(r_eq), **kw)
+
+    def _q(self, c) -> np.ndarray:
+        return np.asarray(c) / (np.pi * self.equivalent_radius_nm ** 2)
+
+    @property
+    def q_ext(self) -> np.ndarray:
+        return self._q(self.c_ext_nm2)
+
+    @property
+    def q_sca(self) -> np.ndarray:
+        return self._q(self.c_sca_nm2)
+
+    @property
+    def q_abs(self) -> np.ndarray:
+        return self._q(self.c_abs_nm2)
+
+
+def parse_problem(obj) -> Simulation:
+    """A Simulation from a model (ours, or the reference's, through its JSON), a dict or a JSON string."""
+    if isinstance(obj, Simulation):
+        return obj
+    if isinstance(obj, BaseModel):
+        return Simulation.model_validate_json(obj.model_dump_json())
+    if isinstance(obj, str):
+        return Simulation.model_validate_json(obj)
+    return Simulation.model_validate(obj)
+
+
+__all__ = ["ConstantIndex", "ConstantPermittivity", "Cylinder", "Drude", "DrudeLorentz", "FDTDOptions", "LayeredSphere",
+           "LorentzPole", "Outputs", "Particle", "PlaneWave", "Result", "Shell", "Simulation", "Sphere", "Spheroid",
+           "WavelengthRange", "mesh_mode", "options_for", "parse_problem"]

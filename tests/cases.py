@@ -368,3 +368,29 @@ def tensor_sphere(ns, device=None, shape=(24, 24, 24), steps=300, periodic=(Fals
               pml=ns.PMLSpec(cells=cells, eps_ref=1.77), sources=[src], monitors=[mon], dft_stride=5,
               periodic=periodic, bloch=bloch)
     return make(ns, device, **kw)
+
+
+def oracle_scattering_engine(scene):
+    """The FP64 oracle Engine of a lightsmith_amd.fdtd.scattering.Scene: the same material arrays, groups (the
+    vertex tensor as the oracle's TensorMesh), off-diagonal rows, CPML, source tables and monitor index sets."""
+    from oracle import fdtd, sources
+    from oracle.tensor import TensorMesh
+    groups = []
+    for g in scene.groups:
+        if hasattr(g, "slot_vert"):
+            g = TensorMesh(media=g.media, frac=g.frac, normal=g.normal, node_comp=g.node_comp, node_idx3=g.node_idx3,
+                           vert_nodes=g.vert_nodes, shift=g.shift, slot_vert=g.slot_vert, slot_medium=g.slot_medium,
+                           shape=g.shape, vert_idx3=g.vert_idx3)
+        groups.append(g)
+    s = scene.source
+    src = sources.TableSource(s.T.reshape(-1), s.m, s.cmax, s.points, inc=s.inc, T_inc=np.asarray(s.T_inc).reshape(-1))
+    p = scene.pml
+    pml = fdtd.PMLSpec(cells=p.cells, m=p.m, kappa_max=p.kappa_max, alpha_max=p.alpha_max, eps_ref=p.eps_ref)
+    mons = [fdtd.DFTMonitor(scene.omega, scene.sca[0], scene.sca[1], scene.sca[2], energy=1),
+            fdtd.DFTMonitor(scene.omega, scene.tot[0], scene.tot[1], scene.tot[2], energy=2),
+            fdtd.DFTMonitor(scene.omega, np.arange(3), source=src)]
+    lay = scene.layout
+    od = scene.offdiag
+    return fdtd.Engine(shape=lay.shape, dx=lay.dx, dt=scene.dt, inv_eps=scene.inv_eps, groups=groups, pml=pml,
+                       sources=[src], monitors=mons, dft_stride=scene.stride,
+                       offdiag=None if od is None else fdtd.OffDiagonal(od.idx3, od.comp, od.coef))

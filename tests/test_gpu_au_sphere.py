@@ -1,0 +1,101 @@
+"""The plain Au sphere end to end on the MI355X (r = 40 nm Drude + 2 Lorentz gold in water, 97^3 nodes at 2 nm with a
+12-cell CPML, table-fed plane wave over 400-800 nm, 81-wavelength flux boxes):
+
+* at the full grid, 3 DFT chunks (321 steps) of the HIP engine against the FP64 oracle on the same inputs (the
+  product's set-up: conformal voxelisation with the vertex tensor, source tables, flux boxes): fields and every
+  monitor spectrum within 1e-5 relative L2;
+* 4000 steps (DFT every 100) against the oracle's committed fixture (tests/golden/au_sphere_4000.npz): the incident
+  reference, both boxes' net fluxes and 64 seeded points of each box's spectra within 1e-5;
+* the decay-terminated solve through ``api.solve(sim, "fdtd", "gpu")`` against Mie (oracle/mie.py) with the same
+  eps(omega), at the reference's bounds for its gold sphere (tests/test_fdtd.py:446-456): peak within 5 nm, peak
+  height within 3 %, C_ext within 7 % over the band; C_abs > 0.
+"""
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOL = 1e-5
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def test_au_sphere_full_grid_matches_the_fp64_oracle(gpu):
+    from lightsmith_amd.fdtd import workloads
+    sim, opts, scene = workloads.au_sphere_scene(gpu)
+    g = scene.engine(gpu)
+    o = cases.oracle_scattering_engine(scene)
+    chunks = 3
+    for q in range(chunks):
+        g.chunk()
+        o.chunk()
+        print(f"chunk {q + 1}/{chunks} ({scene.stride} steps)", flush=True)
+    Eg, Hg = cases.fields(g)
+    Eo, Ho = cases.fields(o)
+    errs = {"E": _rel(Eg, Eo), "H": _rel(Hg, Ho)}
+    for name, mg, mo in zip(("sca", "tot", "inc"), g.monitors, o.monitors):
+        eg, hg = mg.results()
+        eo, ho = mo.results()
+        errs[name + "_e"] = _rel(eg, eo)
+        if ho is not None:
+            errs[name + "_h"] = _rel(hg, ho)
+    print(f"Au sphere {scene.layout.shape}, {chunks * scene.stride} steps, {scene.stats}: " +
+          ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    g.close()
+    assert all(v < TOL for v in errs.values()), errs
+
+
+def test_au_sphere_4000_steps_match_the_oracle_fixture(gpu):
+    from lightsmith_amd.fdtd import scattering, workloads
+    from lightsmith_amd.fdtd.run import flux
+    ref = np.load(os.path.join(HERE, "golden", "au_sphere_4000.npz"))
+    steps, stride = int(ref["steps"]), int(ref["stride"])
+    sim, opts, scene = workloads.au_sphere_scene(gpu, dft_stride=stride)
+    g = scene.engine(gpu)
+    run, used = g.make_runner(True)
+    for _ in range(steps // stride):
+        run()
+    g.release()
+    (es, hs), (et, ht), (e_ref, _) = (m.results() for m in g.monitors)
+    c_sca, c_abs, c_ext = scattering.cross_sections(scene, e_ref, es, hs, et, ht)
+    errs = {"e_ref": _rel(e_ref, ref["e_ref"]), "flux_sca": _rel(flux(es, hs, scene.sca[3]), ref["flux_sca"]),
+            "flux_tot": _rel(flux(et, ht, scene.tot[3]), ref["flux_tot"]),
+            "es": _rel(es[:, ref["pick_sca"]], ref["es"]), "hs": _rel(hs[:, ref["pick_sca"]], ref["hs"]),
+            "et": _rel(et[:, ref["pick_tot"]], ref["et"]), "ht": _rel(ht[:, ref["pick_tot"]], ref["ht"])}
+    print(f"Au sphere, {steps} steps (graphs {used}): " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()) +
+          f"; C_ext rel L2 {_rel(c_ext, ref['c_ext']):.2e}")
+    g.close()
+    assert all(v < TOL for v in errs.values()), errs
+
+
+def test_au_sphere_solve_matches_mie(gpu):
+    from lightsmith_amd import api
+    from lightsmith_amd.fdtd import workloads
+    from oracle.mie import mie_q
+    sim = workloads.au_sphere_sim()
+    res = api.solve(sim, method="fdtd", device="gpu")
+    wl = np.asarray(res.wavelengths_nm)
+    mat = sim.particles[0].material
+    n_h = sim.medium_index
+    r = sim.particles[0].shape.radius_nm
+    q = np.array([mie_q(np.sqrt(complex(mat.eps(np.array([w]))[0])) / n_h, 2 * np.pi * n_h * r / w)[0] for w in wl])
+    mie = q * np.pi * r ** 2
+    fd = np.asarray(res.c_ext_nm2)
+    i, j = int(np.argmax(mie)), int(np.argmax(fd))
+    rel = np.abs(fd - mie) / mie
+    print(f"Au sphere via api.solve: {res.provenance['steps']} steps, grid {res.provenance['grid']}, "
+          f"{res.provenance['tensor_vertices']} tensor vertices; peak {wl[j]:.0f} nm (Mie {wl[i]:.0f} nm), height "
+          f"{fd[j] / mie[i] - 1:+.2%}, max |C_ext - Mie| / Mie {rel.max():.2%}, decay {res.provenance['decay_level']:.1e}")
+    assert res.backend == "lightsmith-fdtd" and res.provenance["hip_graphs"]
+    assert abs(wl[i] - wl[j]) <= 5.0
+    assert abs(fd[j] / mie[i] - 1) < 0.03
+    assert rel.max() < 0.07
+    assert min(res.c_abs_nm2) > 0
