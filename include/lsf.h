@@ -175,6 +175,13 @@ int lsf_set_tiling(lsf_engine* eng, int rows_per_lane, int waves, int x_cells, i
 
 /* Run n_steps steps (Engine.step, engine.py:665-678) on `stream` (a hipStream_t; NULL = default). */
 int lsf_step(lsf_engine* eng, int n_steps, void* stream);
+/* Run n_steps steps as lsf_step does, allocating nothing. No counterpart in the reference: for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid hipMalloc). The two kernel-argument blocks
+ * that lsf_step keeps in a library allocation are staged instead, on `stream`, in one file-scope device array of
+ * 8 KiB per GPU, shared by every call on that GPU: calls that overlap on one GPU (other streams or host threads)
+ * race. Refused with an error: engines with sub-engines (their arguments exceed the array) and single-pass
+ * engines. */
+int lsf_step_noalloc(lsf_engine* eng, int n_steps, void* stream);
 /* As lsf_step, bracketing each H and E kernel with HIP events on `stream`; adds the summed kernel times (ms)
  * to ms[0] (H kernels) and ms[1] (E kernels; the single-pass kernel when the engine steps with it, ms[0] then
  * gaining 0). Synchronises the stream at the end. For measurement. */

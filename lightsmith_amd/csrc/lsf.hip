@@ -131,6 +131,8 @@ struct lsf_engine {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   lsf::StepArgs* dev_args = nullptr;  // library scratch: [0] E kernel, [1] H kernel arguments
+  const lsf::StepArgs* args_override = nullptr;  // during lsf_step_noalloc: the arguments staged in `scratch`
+  lsf::StepArgs host_args[2];          // lsf_step_noalloc: host source of the staged arguments (outlives the copy)
   int nb[2] = {0, 0}, nbp[2] = {0, 0};  // tiles of the E [0] and H [1] launches (and padded to a multiple of 8)
   bool dirty = true;                   // dev_args out of date
   bool launched = false;               // kernels may be reading dev_args
@@ -388,7 +390,7 @@ void launch_fused_step(lsf_engine* e, hipStream_t s) {
 
 lsf::KLaunch own_launch(lsf_engine* e, int field) {
   lsf::KLaunch L{};
-  L.args = e->dev_args + field;
+  L.args = (e->args_override ? e->args_override : e->dev_args) + field;
   L.sc = lsf::Sched{nullptr, nullptr, e->nb[field], e->nbp[field]};
   L.grid = e->nbp[field];
   L.ry = e->ry;
@@ -1096,6 +1098,70 @@ int lsf_step(lsf_engine* e, int n, void* stream) {
   if (int rc1 = sync_args(e)) return rc1;
   hipStream_t s = (hipStream_t)stream;
   for (int q = 0; q < n; ++q) one_step(e, s);
+  return hip_check(hipGetLastError(), "step launch");
+}
+
+}  // extern "C"
+
+namespace {
+
+// lsf_step_noalloc's device scratch: the two kernel-argument blocks of the engine being stepped, staged here on
+// the caller's stream instead of in a library allocation. One array per GPU for the whole process; every call on
+// a GPU shares it, so calls that overlap on one GPU (different streams, or host threads) race.
+constexpr size_t kNoAllocScratchBytes = 8192;
+static_assert(2 * sizeof(lsf::StepArgs) <= kNoAllocScratchBytes, "scratch too small for the kernel arguments");
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` per device (host code never uses the name)
+
+// The device address of `scratch` on the current device, looked up once per device.
+int scratch_address(void** out) {
+  int dev = 0;
+  if (int rc = hip_check(hipGetDevice(&dev), "hipGetDevice")) return rc;
+  if (dev < 0 || dev >= kMaxDevices) return fail(LSF_EINVAL, "device index beyond the scratch table");
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (int rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return rc;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return LSF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lsf_step_noalloc(lsf_engine* e, int n, void* stream) {
+  if (valid(e)) return LSF_EINVAL;
+  if (!e->cbd && !e->mid) return fail(LSF_ESTATE, "no material set");
+  if (n < 0) return fail(LSF_EINVAL, "negative step count");
+  for (const Op& o : e->ops)
+    if (o.kind == OP_SUB_H)
+      return fail(LSF_EINVAL, "lsf_step_noalloc: an engine with sub-engines needs more kernel-argument scratch "
+                              "than the 8 KiB it holds; use lsf_step");
+  if (fused_eligible(e))
+    return fail(LSF_EINVAL, "lsf_step_noalloc: single-pass stepping is not supported; use lsf_step");
+  const size_t need = 2 * sizeof(StepArgs);
+  if (need > kNoAllocScratchBytes) return fail(LSF_EINVAL, "lsf_step_noalloc: kernel arguments exceed the scratch");
+  void* p = nullptr;
+  if (int rc = scratch_address(&p)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const Tiling t = own_tiling(e);
+  e->host_args[0] = make_args(e, 0, t);
+  e->host_args[1] = make_args(e, 1, t);
+  for (int f = 0; f < 2; ++f) {
+    e->nb[f] = e->host_args[f].nb;
+    e->nbp[f] = e->host_args[f].nbp;
+  }
+  // stream-ordered upload: kernels of an earlier call on this stream have read the previous contents
+  if (int rc = hip_check(hipMemcpyAsync(p, e->host_args, need, hipMemcpyHostToDevice, s), "argument staging"))
+    return rc;
+  e->use_fused = false;
+  e->args_override = (const StepArgs*)p;
+  for (int q = 0; q < n; ++q) one_step(e, s);
+  e->args_override = nullptr;
   return hip_check(hipGetLastError(), "step launch");
 }
 
