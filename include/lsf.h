@@ -91,6 +91,7 @@ int lsf_set_ade(lsf_engine* eng, const int32_t* dev_chunk_base, void* dev_P, int
 #define LSF_PH_CORRECT_E 3  /* src.correct_E */
 #define LSF_PH_GROUPS 4     /* groups.step for interface (mixed) groups */
 #define LSF_PH_ADVANCE_E 5  /* src.advance_E */
+#define LSF_PH_BEFORE_H 6   /* before the H update (adjoint engines: the transposed D-form update) */
 
 /* Gather-add rows: F[target[r]] += sum_k w[r, k] * src[src_idx[r, k]] (field 0 = E, 1 = H).
  * Replaces the off-diagonal rows (engine.py:653-655), the 1D-fed TFSF corrections (tfsf.py:195-204)
@@ -214,6 +215,14 @@ int lsf_add_tape(lsf_engine* eng, int64_t n_nodes, const int64_t* dev_nodes, flo
 int lsf_add_adjoint_gradient(lsf_engine* eng, int64_t n_nodes, const int64_t* dev_nodes, const float* dev_cb,
                              const float* dev_tape, int64_t n_slots, const int64_t* dev_base, int64_t n_steps,
                              double* dev_g);
+/* Adjoint engine: the transposed D-form (ADE) update of a forward engine's pure-group nodes (engine.py:759-793
+ * run backwards): n nodes (native flat E indices, sorted), their medium, and the media table of the forward engine
+ * (n_media x {1/eps_inf, poles, (c1, c2, c3) x max_poles}); state holds the adjoint pole pairs (n x max_poles x 2
+ * floats, zeroed by the caller), save n floats of working space. Before each adjoint H kernel the nodes' adjoint
+ * fields become e~ / eps_inf (what the transposed curl reads) and the pole recursion advances; after it the new
+ * adjoint fields are put back. No reference counterpart (the reference has no adjoint). */
+int lsf_add_adjoint_ade(lsf_engine* eng, int64_t n, const int64_t* dev_nodes, const int32_t* dev_medium, int n_media,
+                        int max_poles, const float* dev_media, float* dev_state, float* dev_save);
 /* Adjoint engine: the source of a DFT monitor objective. dev_W[nf][n_pts] complex64 = conj(dL/dacc) (torch's
  * gradient of the monitor's E or H accumulator). The targets are given as CSR rows, one per distinct node:
  * row r adds to node dev_target[r] the sum over its entries q (dev_row_ptr[r] .. dev_row_ptr[r+1]) of

@@ -60,6 +60,7 @@ _SIGNATURES = {
     "lsf_set_adjoint": [c_vp, c_int],
     "lsf_add_tape": [c_vp, c_i64, c_vp, c_vp, c_i64, c_vp],
     "lsf_add_adjoint_gradient": [c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_i64, c_vp],
+    "lsf_add_adjoint_ade": [c_vp, c_i64, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp],
     "lsf_add_adjoint_source": [c_vp, c_int, c_int, c_i64, c_vp, c_dbl, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64,
                                c_int],
     "lsf_batch_create": [ctypes.POINTER(c_vp), c_int, ctypes.POINTER(c_vp)],

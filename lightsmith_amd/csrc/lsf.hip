@@ -35,7 +35,7 @@ int hip_check(hipError_t e, const char* what) {
 
 enum OpKind {
   OP_GATHER, OP_MODAL, OP_TABLE, OP_VIEW, OP_LINE_H, OP_LINE_E, OP_SUB_H, OP_SUB_E, OP_MIXED,
-  OP_TAPE, OP_ADJ_GRAD, OP_ADJ_SRC, OP_TENSOR
+  OP_TAPE, OP_ADJ_GRAD, OP_ADJ_SRC, OP_TENSOR, OP_ADJ_ADE_PRE, OP_ADJ_ADE_POST
 };
 
 struct Op {
@@ -73,6 +73,7 @@ struct Op {
   int dft_stride = 1, h_side = 0;
   double* gacc = nullptr;
   const long long* t2 = nullptr;
+  const int* ade_medium = nullptr;
   const double* omega = nullptr;
   double dt = 0.0;
   // conformal vertex tensor
@@ -215,7 +216,7 @@ void set_parity(lsf_engine* e, int p) {
 
 bool has_h_phase_ops(const lsf_engine* e) {
   for (const Op& o : e->ops)
-    if (o.phase <= LSF_PH_ADVANCE_H) return true;
+    if (o.phase <= LSF_PH_ADVANCE_H || o.phase == LSF_PH_BEFORE_H) return true;
   return false;
 }
 
@@ -465,6 +466,7 @@ void run_op(lsf_engine* e, const Op& o, hipStream_t s) {
                          o.m_src, o.table, o.table_len, (const long long*)e->counter, slot);
       break;
     case OP_SUB_H:
+      run_phase(o.sub, LSF_PH_BEFORE_H, s);
       launch_h(o.sub, s);
       run_phase(o.sub, LSF_PH_CORRECT_H, s);
       run_phase(o.sub, LSF_PH_ADVANCE_H, s);
@@ -483,6 +485,15 @@ void run_op(lsf_engine* e, const Op& o, hipStream_t s) {
       lsf::GradJob J{o.n, o.target, (const float*)e->E, o.coef, o.table, o.n_slots, o.base,
                      (const long long*)e->counter, o.n_steps, o.gacc};
       if (o.n > 0) hipLaunchKernelGGL(lsf::k_adj_grad, dim3(nblk(o.n, 256)), dim3(256), 0, s, J);
+      break;
+    }
+    case OP_ADJ_ADE_PRE:
+    case OP_ADJ_ADE_POST: {
+      lsf::AdjAdeJob J{o.n, o.target, o.ade_medium, o.coef, 2 + 3 * o.maxp, o.maxp, (float*)o.state, o.out, (float*)e->E};
+      if (o.n > 0) {
+        if (o.kind == OP_ADJ_ADE_PRE) hipLaunchKernelGGL(lsf::k_adj_ade_pre, dim3(nblk(o.n, 256)), dim3(256), 0, s, J);
+        else hipLaunchKernelGGL(lsf::k_adj_ade_post, dim3(nblk(o.n, 256)), dim3(256), 0, s, J);
+      }
       break;
     }
     case OP_ADJ_SRC: {
@@ -524,6 +535,7 @@ void one_step(lsf_engine* e, hipStream_t s) {
     for (int ph = LSF_PH_OFFDIAG; ph <= LSF_PH_ADVANCE_E; ++ph) run_phase(e, ph, s);
     return;
   }
+  run_phase(e, LSF_PH_BEFORE_H, s);
   launch_h(e, s);
   run_phase(e, LSF_PH_CORRECT_H, s);
   run_phase(e, LSF_PH_ADVANCE_H, s);
@@ -600,7 +612,7 @@ int configurable(lsf_engine* e) {
 
 int add_op(lsf_engine* e, const Op& o) {
   if (int rc0 = configurable(e)) return rc0;
-  if (o.phase < LSF_PH_CORRECT_H || o.phase > LSF_PH_ADVANCE_E) return fail(LSF_EINVAL, "bad phase");
+  if (o.phase < LSF_PH_CORRECT_H || o.phase > LSF_PH_BEFORE_H) return fail(LSF_EINVAL, "bad phase");
   if (o.field != 0 && o.field != 1) return fail(LSF_EINVAL, "field must be 0 (E) or 1 (H)");
   if (e->exec) return fail(LSF_ESTATE, "the engine has a captured graph; reset it before adding work");
   e->ops.push_back(o);
@@ -940,6 +952,30 @@ int lsf_add_adjoint_gradient(lsf_engine* e, int64_t n, const int64_t* nodes, con
   return add_op(e, o);
 }
 
+int lsf_add_adjoint_ade(lsf_engine* e, int64_t n, const int64_t* nodes, const int32_t* medium, int n_media,
+                        int max_poles, const float* media, float* state, float* save) {
+  if (valid(e)) return LSF_EINVAL;
+  if (!e->adjoint) return fail(LSF_ESTATE, "adjoint D-form updates belong to an adjoint engine (lsf_set_adjoint)");
+  if (n < 1 || !nodes || !medium || n_media < 1 || max_poles < 1 || max_poles > 4 || !media || !state || !save)
+    return fail(LSF_EINVAL, "bad adjoint D-form group");
+  for (int ph = 0; ph < 2; ++ph) {
+    Op o;
+    o.kind = ph == 0 ? OP_ADJ_ADE_PRE : OP_ADJ_ADE_POST;
+    o.phase = ph == 0 ? LSF_PH_BEFORE_H : LSF_PH_ADVANCE_H;
+    o.field = 0;
+    o.n = n;
+    o.target = (const long long*)nodes;
+    o.ade_medium = medium;
+    o.coef = media;
+    o.maxp = max_poles;
+    o.S = n_media;
+    o.state = state;
+    o.out = save;
+    if (int rc = add_op(e, o)) return rc;
+  }
+  return LSF_OK;
+}
+
 int lsf_add_adjoint_source(lsf_engine* e, int h_side, int nf, int64_t n_pts, const double* omega, double dt,
                            const void* W, int64_t n_rows, const int32_t* row_ptr, const int64_t* target,
                            const int64_t* point, const float* scale, int64_t n_steps, int stride) {
@@ -1177,6 +1213,7 @@ int lsf_step_timed(lsf_engine* e, int n, void* stream, double* ms) {
     if ((rc = hip_check(hipEventCreate(&x), "event create"))) break;
   if (!rc) {
     for (int q = 0; q < n; ++q) {
+      run_phase(e, LSF_PH_BEFORE_H, s);
       (void)hipEventRecord(ev[4 * q + 0], s);
       if (!e->use_fused) launch_h(e, s);
       (void)hipEventRecord(ev[4 * q + 1], s);
@@ -1333,7 +1370,7 @@ struct lsf_batch;
 namespace {
 
 struct KGroup {
-  int cplx = 0, per = 0, mat = 0, np = 0;
+  int cplx = 0, per = 0, mat = 0, np = 0, adj = 0;
   std::vector<lsf_engine*> mem;
   lsf::KLaunch L{};
 };
@@ -1363,7 +1400,7 @@ struct DftLaunch {
 struct Level {
   std::vector<lsf_engine*> mem;
   std::vector<KGroup> hg, eg;
-  std::vector<Slot> slots[LSF_PH_ADVANCE_E + 1];
+  std::vector<Slot> slots[LSF_PH_BEFORE_H + 1];
   std::vector<DftLaunch> dft;
   lsf::EnergyJob* energy = nullptr;
   int n_energy = 0;
@@ -1422,9 +1459,10 @@ int build_groups(Level& L, const Tiling& t, int grid_blocks, std::vector<void*>&
       key.per = any_periodic(e) ? 1 : 0;
       key.mat = field ? 0 : (e->mid ? 1 : 0);
       key.np = field ? 0 : pole_regs(e);
+      key.adj = e->adjoint ? 1 : 0;
       KGroup* g = nullptr;
       for (KGroup& x : groups)
-        if (x.cplx == key.cplx && x.per == key.per && x.mat == key.mat && x.np == key.np) g = &x;
+        if (x.cplx == key.cplx && x.per == key.per && x.mat == key.mat && x.np == key.np && x.adj == key.adj) g = &x;
       if (!g) {
         groups.push_back(key);
         g = &groups.back();
@@ -1432,10 +1470,13 @@ int build_groups(Level& L, const Tiling& t, int grid_blocks, std::vector<void*>&
       g->mem.push_back(e);
     }
     for (KGroup& g : groups) {
+      // the adjoint kernels take 64-cell rows and one block per tile
+      Tiling tg = t;
+      if (g.adj) tg.xw = 64;
       std::vector<StepArgs> args;
       std::vector<int> counts;
       for (lsf_engine* e : g.mem) {
-        args.push_back(make_args(e, field, t));
+        args.push_back(make_args(e, field, tg));
         counts.push_back(args.back().nb);
       }
       void* da = nullptr;
@@ -1444,10 +1485,10 @@ int build_groups(Level& L, const Tiling& t, int grid_blocks, std::vector<void*>&
       g.L.args = (const StepArgs*)da;
       if ((rc = make_sched(counts, &g.L.sc, owned))) return rc;
       g.L.grid = g.L.sc.n_pad;
-      if (grid_blocks > 0) g.L.grid = std::min(g.L.sc.n_pad, (grid_blocks + 7) / 8 * 8);
-      g.L.ry = t.ry;
-      g.L.waves = t.waves;
-      g.L.xw = t.xw;
+      if (grid_blocks > 0 && !g.adj) g.L.grid = std::min(g.L.sc.n_pad, (grid_blocks + 7) / 8 * 8);
+      g.L.ry = tg.ry;
+      g.L.waves = tg.waves;
+      g.L.xw = tg.xw;
       g.L.cplx = g.cplx;
       g.L.per = g.per;
       g.L.mat = g.mat;
@@ -1545,7 +1586,7 @@ int build_sampling(Level& L, std::vector<void*>& owned) {
 int build_level(Level& L, const Tiling& t, int grid_blocks, bool sample, std::vector<void*>& owned,
                 std::vector<void*>& owned_groups) {
   int rc = build_groups(L, t, grid_blocks, owned_groups);
-  for (int ph = LSF_PH_CORRECT_H; ph <= LSF_PH_ADVANCE_E && !rc; ++ph) {
+  for (int ph = LSF_PH_CORRECT_H; ph <= LSF_PH_BEFORE_H && !rc; ++ph) {
     std::vector<std::vector<std::pair<lsf_engine*, Op>>> by_pos;
     for (lsf_engine* e : L.mem) {
       size_t q = 0;
@@ -1589,11 +1630,17 @@ void run_slot(Slot& sl, bool h_half, hipStream_t s) {
 }
 
 void level_h_kernels(Level& L, hipStream_t s) {
-  for (KGroup& g : L.hg) lsf::launch_update_h(g.L, s);
+  for (KGroup& g : L.hg) {
+    if (g.adj) lsf::launch_adjoint_h(g.L, s);
+    else lsf::launch_update_h(g.L, s);
+  }
 }
 
 void level_e_kernels(Level& L, hipStream_t s) {
-  for (KGroup& g : L.eg) lsf::launch_update_e(g.L, s);
+  for (KGroup& g : L.eg) {
+    if (g.adj) lsf::launch_adjoint_e(g.L, s);
+    else lsf::launch_update_e(g.L, s);
+  }
 }
 
 void level_h_ops(Level& L, hipStream_t s) {
@@ -1607,6 +1654,7 @@ void level_e_ops(Level& L, hipStream_t s) {
 }
 
 void level_h(Level& L, hipStream_t s) {
+  for (Slot& sl : L.slots[LSF_PH_BEFORE_H]) run_slot(sl, true, s);
   level_h_kernels(L, s);
   level_h_ops(L, s);
 }
@@ -1658,7 +1706,6 @@ int lsf_batch_create(lsf_engine* const* members, int n, lsf_batch** out) {
     if (!e) return fail(LSF_EINVAL, "null engine in batch");
     if (!e->tick) return fail(LSF_EINVAL, "batch members are top-level engines (tick = 1)");
     if (!e->cbd && !e->mid) return fail(LSF_ESTATE, "a batch member has no material");
-    if (e->adjoint) return fail(LSF_EINVAL, "adjoint engines are stepped alone");
     for (int r = 0; r < q; ++r)
       if (members[r] == e || members[r]->counter == e->counter)
         return fail(LSF_EINVAL, "batch members must be distinct engines with their own step counters");
@@ -1729,6 +1776,7 @@ int lsf_batch_step_timed(lsf_batch* b, int n, void* stream, double* ms) {
     if ((rc = hip_check(hipEventCreate(&x), "event create"))) break;
   if (!rc) {
     for (int q = 0; q < n; ++q) {
+      for (Slot& sl : b->root.slots[LSF_PH_BEFORE_H]) run_slot(sl, true, s);
       (void)hipEventRecord(ev[4 * q + 0], s);
       level_h_kernels(b->root, s);
       (void)hipEventRecord(ev[4 * q + 1], s);

@@ -545,6 +545,54 @@ __global__ void k_adj_grad(GradJob J) {
   J.g[q] += (double)(J.E[J.node[q]] / J.cb[q]) * (double)J.tape[slot * J.n + q];
 }
 
+// Adjoint of the D-form (ADE) update of pure-group nodes (engine.py:759-793 transposed). Forward, per node with
+// eps_inf and poles (c1, c2, c3): P'_p = c1 P_p + c2 P_p^prev + c3 E, E' = E + (X - sum_p (P'_p - P_p)) / eps_inf,
+// X = cb (curl- H + CPML terms). With the adjoint pole pair (a_p, b_p) = (dL/dP'_p, dL/dP_p^prev') and
+// t_p = a_p - E'bar / eps_inf, the transposed step is
+//   Ebar = E'bar + sum_p c3 t_p,  a_p <- E'bar / eps_inf + b_p + c1 t_p,  b_p <- c2 t_p,  Xbar = E'bar / eps_inf.
+// In the adjoint engine's variable e~ = cb Ebar (cb constant, dt/dx, at D-form nodes) every term scales alike.
+// pre (before the adjoint H kernel): keep Ebar's new value aside and leave cb Xbar = e~ / eps_inf in the field,
+// which is what the H kernel's transposed curl (and CPML) must read there; post (after it): restore Ebar.
+struct AdjAdeJob {
+  long long n;
+  const long long* node;   // native flat E index, sorted
+  const int* medium;       // per node
+  const float* media;      // per medium: {1/eps_inf, poles, (c1, c2, c3) x max_poles}
+  int stride;              // 2 + 3 max_poles
+  int max_poles;
+  float* state;            // [n][max_poles][2]: (a, b) per pole
+  float* save;             // [n]
+  float* E;
+};
+
+__global__ void k_adj_ade_pre(AdjAdeJob J) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= J.n) return;
+  const float* m = J.media + (long long)J.medium[q] * J.stride;
+  const float inv = m[0];
+  const int np = (int)m[1];
+  const long long nd = J.node[q];
+  const float e1 = J.E[nd];
+  const float x = e1 * inv;
+  float acc = 0.f;
+  float* st = J.state + q * 2 * J.max_poles;
+  for (int p = 0; p < np; ++p) {
+    const float c1 = m[2 + 3 * p], c2 = m[3 + 3 * p], c3 = m[4 + 3 * p];
+    const float t = st[2 * p] - x;
+    acc = fmaf(c3, t, acc);
+    st[2 * p] = fmaf(c1, t, x + st[2 * p + 1]);
+    st[2 * p + 1] = c2 * t;
+  }
+  J.save[q] = e1 + acc;
+  J.E[nd] = x;
+}
+
+__global__ void k_adj_ade_post(AdjAdeJob J) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= J.n) return;
+  J.E[J.node[q]] = J.save[q];
+}
+
 // Adjoint DFT source: F[node] += sum over the points p that touch the node of scale * sum_f Re(W[f, p] e^{i omega_f t})
 // at sampling steps, where W = conj(dL/dacc) in torch's gradient convention for the monitor accumulator acc[f, p].
 // E side (h_side = 0): after an adjoint E half-step the state is E^n, n = n_steps - 1 - counter[1]; sampled when

@@ -8,7 +8,8 @@ inverse/__init__.py:14-40); this module adds one behind ``torch.autograd``:
     loss = objective(acc_e, acc_h); loss.backward()    # rho.grad by one adjoint run
 
 Density rho in [0, 1] per design voxel sets eps = eps_lo + rho (eps_hi - eps_lo) at the E_x, E_y and E_z nodes of
-that cell index (staircase). The forward engine records, per step, X = curl- H at the design nodes (a tape); the
+that cell index (staircase). The scene may hold pure D-form (Drude-Lorentz ADE) groups outside the design region;
+the adjoint engine runs their transposed pole recursion (lsf_add_adjoint_ade). The forward engine records, per step, X = curl- H at the design nodes (a tape); the
 adjoint engine (lsf_set_adjoint) runs the transposed update backwards from the monitor's adjoint sources and
 accumulates dL/dcb = sum_n Ebar^{n+1} X^{n+1/2} per node; dL/drho follows from cb = (dt/dx) / eps(rho).
 
@@ -91,7 +92,7 @@ class AdjointProblem:
 
     def __init__(self, device, *, shape, dx, dt: float, inv_eps: np.ndarray, pml: PMLSpec, sources, monitor: DFTMonitor,
                  design: DesignRegion, n_steps: int, dft_stride: int, checkpoint_every: int = 100,
-                 tape_budget: int = 4 << 30, use_graphs: bool = True):
+                 tape_budget: int = 4 << 30, use_graphs: bool = True, groups=()):
         self.design = design
         self.shape = tuple(int(v) for v in shape)
         self.stride = int(dft_stride)
@@ -103,12 +104,26 @@ class AdjointProblem:
             cells = pml.cells[a]
             if design.lo[a] < max(1, cells[0]) or design.hi[a] > self.shape[a] - max(1, cells[1]) - 1:
                 raise ValueError("the design region must lie inside the CPML-free interior")
-        self.fwd = Engine(device, shape=shape, dx=dx, dt=dt, single=True, inv_eps=inv_eps, pml=pml, sources=sources,
-                          monitors=[monitor], dft_stride=self.stride, material="dense", fused=False)
+        groups = list(groups)
+        if any(getattr(g, "w", None) is not None or hasattr(g, "slot_vert") for g in groups):
+            raise ValueError("the adjoint transposes pure D-form groups only (no mixed or vertex-tensor groups)")
+        self.fwd = Engine(device, shape=shape, dx=dx, dt=dt, single=True, inv_eps=inv_eps, groups=groups, pml=pml,
+                          sources=sources, monitors=[monitor], dft_stride=self.stride, material="dense", fused=False)
         self.monitor = monitor
         dev = self.fwd._dev
         self._dev = dev
         nodes, vox = design.nodes(self.shape)
+        if self.fwd.n_ade:
+            if np.isin(nodes, self.fwd.ade_nodes).any():
+                raise ValueError("the design region must not hold D-form (dispersive) nodes")
+            # the transposed CPML recursions read the adjoint field of their node; D-form nodes keep out of the slabs
+            n0, n1, _ = self.shape
+            cell = self.fwd.ade_nodes % self.fwd.N
+            idx = (cell % n0, (cell // n0) % n1, cell // (n0 * n1))
+            for a in range(3):
+                lo_c, hi_c = pml.cells[a]
+                if idx[a].min() < max(1, lo_c) or idx[a].max() > self.shape[a] - max(1, hi_c) - 1:
+                    raise ValueError("D-form (dispersive) nodes must lie inside the CPML-free interior")
         self.n_nodes = len(nodes)
         self._nodes = torch.as_tensor(nodes, device=dev)
         self._vox = torch.as_tensor(vox, device=dev)
@@ -122,11 +137,25 @@ class AdjointProblem:
         check(lib.lsf_add_tape(self.fwd._h, self.n_nodes, ptr(self._nodes), ptr(self.tape), self.n_slots,
                                ptr(self.base)), "lsf_add_tape")
 
-        # the adjoint engine: same grid, material (shared) and CPML profiles; its own fields and chi arrays
+        # the adjoint engine: same grid, material and CPML profiles; its own fields and chi arrays. Its material is
+        # the forward one with the D-form nodes' code replaced by their coefficient dt/dx (inv_eps = 1 there);
+        # set_density updates both at the design nodes
         self.adj = Engine(device, shape=shape, dx=dx, dt=dt, single=True, inv_eps=inv_eps, pml=pml, sources=[],
                           monitors=[], dft_stride=1, material="dense", fused=False)
-        check(lib.lsf_set_material_dense(self.adj._h, ptr(self.fwd._mat)), "lsf_set_material_dense")
+        self._adj_mat = self.fwd._mat.clone()
+        if self.fwd.n_ade:
+            self._adj_mat[torch.as_tensor(self.fwd.ade_nodes, device=dev)] = float(self.fwd.cdx)
+        check(lib.lsf_set_material_dense(self.adj._h, ptr(self._adj_mat)), "lsf_set_material_dense")
         check(lib.lsf_set_adjoint(self.adj._h, 1), "lsf_set_adjoint")
+        if self.fwd.n_ade:
+            f = self.fwd
+            self._ade_nodes = torch.as_tensor(f.ade_nodes, device=dev)
+            self._ade_medium = torch.as_tensor(f.ade_medium.astype(np.int32), device=dev)
+            self._ade_state = self.adj.zeros(f.n_ade * 2 * f.ade_max_poles, complex_=False)
+            self._ade_save = torch.zeros(f.n_ade, dtype=torch.float32, device=dev)
+            check(lib.lsf_add_adjoint_ade(self.adj._h, f.n_ade, ptr(self._ade_nodes), ptr(self._ade_medium),
+                                          len(f.ade_media), f.ade_max_poles, ptr(f._ade_tab), ptr(self._ade_state),
+                                          ptr(self._ade_save)), "lsf_add_adjoint_ade")
         self.g = torch.zeros(self.n_nodes, dtype=torch.float64, device=dev)
         check(lib.lsf_add_adjoint_gradient(self.adj._h, self.n_nodes, ptr(self._nodes), ptr(self.cb_design),
                                            ptr(self.tape), self.n_slots, ptr(self.base), self.n_steps, ptr(self.g)),
@@ -136,7 +165,7 @@ class AdjointProblem:
         e_nat = self.fwd.native_index(m.e_idx)
         self._e_rows = _Rows(e_nat, np.arange(len(e_nat)), dev)
         self._e_nodes = self._e_rows.target
-        self._cb_e = self.fwd._mat[self._e_rows.entry_target].clone()
+        self._cb_e = self._adj_mat[self._e_rows.entry_target].clone()
         self.W_E = torch.zeros((nf, len(e_nat)), dtype=torch.complex64, device=dev)
         r = self._e_rows
         check(lib.lsf_add_adjoint_source(self.adj._h, 0, nf, len(e_nat), ptr(m._omega), float(dt), ptr(self.W_E),
@@ -181,8 +210,9 @@ class AdjointProblem:
         eps = self.eps(rho.to(self._dev))
         cb = (self.fwd.cdx / eps)[self._vox].to(torch.float32)
         self.fwd._mat[self._nodes] = cb
+        self._adj_mat[self._nodes] = cb
         self.cb_design.copy_(cb)
-        self._cb_e.copy_(self.fwd._mat[self._e_rows.entry_target])
+        self._cb_e.copy_(self._adj_mat[self._e_rows.entry_target])
 
     # --- runs -----------------------------------------------------------------------------------------
     def _zero(self, eng: Engine):
@@ -207,32 +237,32 @@ class AdjointProblem:
         acc_h = m.acc_h.clone() if m.acc_h is not None else torch.zeros(0, dtype=torch.complex64, device=self._dev)
         return m.acc_e.clone(), acc_h
 
-    def backward(self, grad_e: torch.Tensor | None, grad_h: torch.Tensor | None) -> torch.Tensor:
-        """dL/drho (float64, the shape of rho's flattening) for the objective whose gradients with respect to the
-        monitor's accumulators are grad_e, grad_h (torch's convention for complex tensors)."""
-        adj, fwd = self.adj, self.fwd
+    def begin_backward(self, grad_e: torch.Tensor | None, grad_h: torch.Tensor | None):
+        """Zero the adjoint state, load the monitor's adjoint source weights and set the adjoint's initial condition
+        (the sample of E^{n_steps})."""
+        adj = self.adj
         self._zero(adj)
         self.g.zero_()
         self.W_E.copy_(torch.conj(grad_e) if grad_e is not None else torch.zeros_like(self.W_E))
         if self.W_H is not None:
             self.W_H.copy_(torch.conj(grad_h) if grad_h is not None and grad_h.numel() else torch.zeros_like(self.W_H))
-        # the sample of E^{n_steps}: the adjoint's initial condition
         t = self.n_steps * self.dt
         ph = torch.exp(1j * self._omega * t).to(torch.complex128)
         src = torch.real(self.W_E.to(torch.complex128) * ph[:, None]).sum(0)
         per_node = self._e_rows.row_sum(src, self._cb_e.to(torch.float64))
         adj.E_flat[self._e_nodes] += per_node.to(torch.float32)
-        if self.store_all:
-            self.base.fill_(0)
-            adj.steps(self.n_steps)
-        else:
-            for seg in reversed(range(len(self.checkpoints))):
-                n_seg = min(self.K, self.n_steps - seg * self.K)
-                for tt, snap in zip(fwd.state_tensors(), self.checkpoints[seg]):
-                    tt.copy_(snap)
-                self.base.fill_(seg * self.K)
-                fwd.steps(n_seg)
-                adj.steps(n_seg)
+
+    def restore_checkpoint(self, seg: int):
+        """The forward state at the start of checkpoint interval seg, with the tape rows of that interval next."""
+        for tt, snap in zip(self.fwd.state_tensors(), self.checkpoints[seg]):
+            tt.copy_(snap)
+        self.base.fill_(seg * self.K)
+
+    def interval(self, seg: int) -> int:
+        return min(self.K, self.n_steps - seg * self.K)
+
+    def finish_backward(self) -> torch.Tensor:
+        """dL/drho (float64) from the accumulated dL/dcb of the design nodes."""
         d = self.design
         cb = self.cb_design.to(torch.float64)
         # cb = C / eps  ->  dcb/deps = -cb^2 / C ; deps/drho = eps_hi - eps_lo
@@ -240,11 +270,129 @@ class AdjointProblem:
         # nodes are component-major over the voxels (DesignRegion.nodes): a fixed-order sum, no atomics
         return (self.g * dcb_drho).view(3, d.n_voxels).sum(0)
 
+    def backward(self, grad_e: torch.Tensor | None, grad_h: torch.Tensor | None) -> torch.Tensor:
+        """dL/drho (float64, the shape of rho's flattening) for the objective whose gradients with respect to the
+        monitor's accumulators are grad_e, grad_h (torch's convention for complex tensors)."""
+        self.begin_backward(grad_e, grad_h)
+        if self.store_all:
+            self.base.fill_(0)
+            self.adj.steps(self.n_steps)
+        else:
+            for seg in reversed(range(len(self.checkpoints))):
+                self.restore_checkpoint(seg)
+                self.fwd.steps(self.interval(seg))
+                self.adj.steps(self.interval(seg))
+        return self.finish_backward()
+
     def close(self):
         if self._run is not None:
             self.fwd.release()
         self.fwd.close()
         self.adj.close()
+
+
+class BatchedAdjoint:
+    """Forward and adjoint runs of several AdjointProblems (multi-instance inverse design: uneven grids, each with
+    its own design region, sources, dispersive groups and monitor) stepped together: the forward engines as one
+    Batch, the adjoint engines as another, so every half-step is one launch per kernel configuration over all
+    instances' tiles (and one HIP graph per forward DFT chunk). With checkpoints the adjoint pass recomputes each
+    interval of every instance in one batched forward run right before reversing it in one batched adjoint run.
+    Each instance's result equals its run alone bit for bit (the batched kernels run the same tile code)."""
+
+    def __init__(self, problems, use_graphs: bool = True, schedule=None):
+        from .batch import Batch
+        self.problems = list(problems)
+        if not self.problems:
+            raise ValueError("a batched adjoint needs at least one problem")
+        p0 = self.problems[0]
+        for p in self.problems:
+            if (p.stride, p.K, p.n_steps, p.store_all) != (p0.stride, p0.K, p0.n_steps, p0.store_all):
+                raise ValueError("batched adjoint problems share the DFT stride, checkpoint interval, step count "
+                                 "and tape mode")
+        self.stride, self.K, self.n_steps, self.store_all = p0.stride, p0.K, p0.n_steps, p0.store_all
+        self.use_graphs = use_graphs
+        self.fwd = Batch([p.fwd for p in self.problems], schedule=schedule)
+        self.adj = Batch([p.adj for p in self.problems], schedule=schedule)
+        self._run = None
+        self.runs = 0
+
+    def forward(self, rhos) -> list:
+        """[(acc_e, acc_h)] of every problem after a batched forward run with the given densities."""
+        for p, rho in zip(self.problems, rhos):
+            p.set_density(rho)
+            p._zero(p.fwd)
+            p.runs += 1
+            p.base.fill_(0 if p.store_all else -(1 << 62))
+            p.checkpoints = []
+        self.runs += 1
+        if self._run is None:
+            self._run, _ = self.fwd.make_runner(self.use_graphs)
+        for step in range(0, self.n_steps, self.stride):
+            if not self.store_all and step % self.K == 0:
+                for p in self.problems:
+                    p.checkpoints.append([t.clone() for t in p.fwd.state_tensors()])
+            self._run()
+        out = []
+        for p in self.problems:
+            m = p.monitor
+            acc_h = m.acc_h.clone() if m.acc_h is not None else torch.zeros(0, dtype=torch.complex64, device=p._dev)
+            out.append((m.acc_e.clone(), acc_h))
+        return out
+
+    def backward(self, grads) -> list:
+        """[dL/drho] for the objectives whose accumulator gradients are grads = [(grad_e, grad_h)]."""
+        for p, (ge, gh) in zip(self.problems, grads):
+            p.begin_backward(ge, gh)
+        if self.store_all:
+            for p in self.problems:
+                p.base.fill_(0)
+            self.adj.steps(self.n_steps)
+        else:
+            for seg in reversed(range(len(self.problems[0].checkpoints))):
+                for p in self.problems:
+                    p.restore_checkpoint(seg)
+                n_seg = self.problems[0].interval(seg)
+                self.fwd.steps(n_seg)
+                self.adj.steps(n_seg)
+        return [p.finish_backward() for p in self.problems]
+
+    def release(self):
+        if self._run is not None:
+            self.fwd.release()
+            self._run = None
+
+    def close(self):
+        self.release()
+        self.fwd.close()
+        self.adj.close()
+
+
+class _BatchedSpectra(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, batch, *rhos):
+        ctx.batch = batch
+        ctx.shapes = [r.shape for r in rhos]
+        ctx.dtypes = [r.dtype for r in rhos]
+        outs = batch.forward([r.detach() for r in rhos])
+        ctx.run = batch.runs
+        ctx.save_for_backward(*[r.detach().clone() for r in rhos])
+        return tuple(t for pair in outs for t in pair)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        batch = ctx.batch
+        if batch.runs != ctx.run:
+            batch.forward(list(ctx.saved_tensors))
+        pairs = [(grads[2 * q], grads[2 * q + 1]) for q in range(len(batch.problems))]
+        gs = batch.backward(pairs)
+        return (None, *[g.reshape(s).to(d) for g, s, d in zip(gs, ctx.shapes, ctx.dtypes)])
+
+
+def batched_monitor_spectra(rhos, batch: BatchedAdjoint) -> list:
+    """[(acc_e, acc_h)] of every problem of the batch after one batched forward run with densities rhos;
+    differentiable in every rho through one batched adjoint run."""
+    flat = _BatchedSpectra.apply(batch, *rhos)
+    return [(flat[2 * q], flat[2 * q + 1]) for q in range(len(rhos))]
 
 
 class _MonitorSpectra(torch.autograd.Function):

@@ -281,7 +281,7 @@ def adjoint_problem(scene: dict, device="cuda:0", **kw):
     return AdjointProblem(device, shape=scene["shape"], dx=scene["dx"], dt=scene["dt"], inv_eps=scene["inv_eps"],
                           pml=pml, sources=[src], monitor=mon,
                           design=DesignRegion(scene["lo"], scene["hi"], scene["eps_lo"], scene["eps_hi"]),
-                          n_steps=scene["n_steps"], dft_stride=scene["stride"], **kw)
+                          n_steps=scene["n_steps"], dft_stride=scene["stride"], groups=scene.get("groups", ()), **kw)
 
 
 def mode_power(acc_e, w):
@@ -323,3 +323,54 @@ def au_sphere_scene(device="cuda:0", **kw):
     sim = au_sphere_sim(**kw)
     opts = FDTDOptions.model_validate(options_for(sim, "lightsmith-fdtd", "fdtd"))
     return sim, opts, scattering.setup(sim, opts, device)
+
+
+# --- batched adjoint with dispersion (multi-instance inverse design) -------------------------------------------------
+
+# Ag as Drude + 3 Lorentz poles (energies in eV): a fixed parameter set of this workload.
+AG_DRUDE_3LORENTZ = {"eps_inf": 1.0, "wp_ev": 9.01, "gamma_ev": 0.048,
+                     "lorentz": [(0.79, 4.43, 0.65), (0.35, 5.10, 1.40), (0.60, 8.00, 2.50)]}
+
+
+def batched_adjoint_scenes(n_inst: int = 16, seed: int = 6, n_steps: int = 2500, dx: float = 20.0, pml: int = 12,
+                           stride: int = 25) -> list:
+    """The multi-instance inverse-design workload: n_inst adjoint scenes of seeded uneven sizes
+    (128-256 x 96-192 x 64-96 cells, seed 6), each the waveguide scene of adjoint_scene (a strip waveguide along x
+    broken by a seeded density design region of up to 60 x 60 x 11 cells) with an Ag nanostructure -- a 6-cell
+    cube of Drude + 3 Lorentz silver (pure D-form nodes) on the waveguide axis between the design region and the
+    output monitor; dt from the reference's rule with the dispersive stability limit (setup.py:78-91); 12-cell
+    CPML; n_steps steps, DFT every 25 steps."""
+    rng = np.random.default_rng(seed)
+    from .engine import DispersiveGroup
+    from .media import drude_lorentz
+    ag = drude_lorentz(AG_DRUDE_3LORENTZ["eps_inf"], AG_DRUDE_3LORENTZ["wp_ev"], AG_DRUDE_3LORENTZ["gamma_ev"],
+                       AG_DRUDE_3LORENTZ["lorentz"], "Ag")
+    dt = time_step(dx, 0.99, [Medium(1.44 ** 2), Medium(3.48 ** 2), ag])
+    out = []
+    for q in range(n_inst):
+        shape = (int(rng.integers(128, 257)), int(rng.integers(96, 193)), int(rng.integers(64, 97)))
+        design = (min(60, shape[0] // 4), min(60, shape[1] // 3), 11)
+        sc = adjoint_scene(shape=shape, dx=dx, design=design, seed=int(rng.integers(1 << 30)), pml=pml,
+                           n_steps=n_steps, stride=stride)
+        sc["dt"] = dt
+        sc["t_src"] = _mode_pulse(n_steps, dt, float(sc["omega"][0]))
+        n0, n1, n2 = shape
+        # the Ag cube: 6 cells on the waveguide axis, halfway between the design region and the monitor plane
+        i_mon = n0 - pml - 14
+        ic = (sc["hi"][0] + i_mon) // 2
+        jc, kc = n1 // 2, n2 // 2
+        box = [np.arange(ic - 3, ic + 3), np.arange(jc - 3, jc + 3), np.arange(kc - 3, kc + 3)]
+        ii, jj, kk = (a.ravel() for a in np.meshgrid(*box, indexing="ij"))
+        idx = np.sort(np.concatenate([flat_index(c, ii, jj, kk, shape) for c in range(3)]))
+        sc["inv_eps"].reshape(-1)[idx] = 1.0
+        sc["groups"] = [DispersiveGroup(idx=idx, media=[ag], frac=np.ones((1, len(idx))), w=None)]
+        sc["ag_cells"] = int(len(ii))
+        out.append(sc)
+    return out
+
+
+def _mode_pulse(n_steps: int, dt: float, omega: float) -> np.ndarray:
+    """The mode source's Gaussian pulse (half-width 0.1 omega) sampled at the source's half steps."""
+    tau = 2.0 / (0.1 * omega)
+    t = (np.arange(n_steps + 2) + 0.5) * dt
+    return (np.exp(-((t - 3.5 * tau) / tau) ** 2) * np.sin(omega * (t - 3.5 * tau)))[:, None]

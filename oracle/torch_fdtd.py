@@ -2,8 +2,8 @@
 adjoint gradient, by autograd through every step (test infrastructure only).
 
 Restates Engine.update_H / update_E with CPML (engine.py:630-663), the per-step mode-source corrections
-(ports.py:362-372) and the running DFT sample (engine.py:564-592) of ``oracle.fdtd.Engine`` -- real fields, no
-periodic axes, no dispersive groups -- with the E coefficients of a design box taken from a density rho:
+(ports.py:362-372), pure D-form (ADE) groups (engine.py:759-793) and the running DFT sample (engine.py:564-592) of
+``oracle.fdtd.Engine`` -- real fields, no periodic axes, no mixed or vertex-tensor groups -- with the E coefficients of a design box taken from a density rho:
 eps = eps_lo + rho (eps_hi - eps_lo) at the E_x, E_y, E_z nodes of each design cell, cb = (dt/dx) / eps. Every
 update is out of place so that autograd can differentiate the whole run.
 """
@@ -29,8 +29,11 @@ class TorchRun:
     """Forward run of an oracle Engine's configuration in torch. `eng` must be freshly built (fields zero)."""
 
     def __init__(self, eng: Engine, lo, hi, eps_lo: float, eps_hi: float, device="cpu"):
-        if eng.complex or any(eng.periodic) or eng.groups or eng.od is not None:
-            raise ValueError("TorchRun restates real fields without periodic axes, groups or off-diagonal terms")
+        from .fdtd import PureGroup
+        if eng.complex or any(eng.periodic) or eng.od is not None or \
+                any(not isinstance(g, PureGroup) for g in eng.groups):
+            raise ValueError("TorchRun restates real fields without periodic axes, mixed/tensor groups or "
+                             "off-diagonal terms")
         self.eng = eng
         self.dev = torch.device(device)
         self.lo, self.hi = tuple(lo), tuple(hi)
@@ -41,6 +44,8 @@ class TorchRun:
         conv = lambda a: None if a is None else torch.as_tensor(a, dtype=T, device=d)  # noqa: E731
         self.pml_H = [(c, s_, dd, Rs, conv(bb), conv(cp), conv(kp)) for c, s_, dd, Rs, bb, cp, kp, _ in eng.pml_H]
         self.pml_E = [(c, s_, dd, Rs, conv(bb), conv(cp), conv(kp)) for c, s_, dd, Rs, bb, cp, kp, _ in eng.pml_E]
+        self.groups = [(torch.as_tensor(g.idx, device=d), float(g.inv_eps_inf), [tuple(float(v) for v in c)
+                        for c in g.coef]) for g in eng.groups]
         self.src = []
         for s in eng.sources:
             self.src.append((torch.as_tensor(np.asarray(s.h_idx, dtype=np.int64), device=d), conv(s.mH @ s.tE.T).T
@@ -70,6 +75,9 @@ class TorchRun:
         H = torch.zeros((3,) + eng.shape, dtype=T, device=d)
         psiE = [torch.zeros(t[5].shape, dtype=T, device=d) for t in self.pml_E]
         psiH = [torch.zeros(t[5].shape, dtype=T, device=d) for t in self.pml_H]
+        gstate = [(torch.zeros(len(idx), dtype=T, device=d), torch.zeros(len(idx), dtype=T, device=d),
+                   [(torch.zeros(len(idx), dtype=T, device=d), torch.zeros(len(idx), dtype=T, device=d))
+                    for _ in coef]) for idx, _, coef in self.groups]
         m = eng.monitors[0]
         omega = torch.as_tensor(np.asarray(m.omega, dtype=float), dtype=T, device=d)
         e_idx = torch.as_tensor(np.asarray(m.e_idx), device=d)
@@ -118,6 +126,19 @@ class TorchRun:
             for _, _, e_src, me in self.src:
                 if me is not None:
                     E = E.reshape(-1).index_add(0, e_src, me[n]).reshape(E.shape)
+            # pure D-form groups: D += E_now - E_prev; poles from E_prev; E = (D - sum P) / eps_inf
+            for q, (idx, inv, coef) in enumerate(self.groups):
+                D, E_prev, P = gstate[q]
+                D = D + E.reshape(-1)[idx] - E_prev
+                tot = 0.0
+                newP = []
+                for (P0, P1), (c1, c2, c3) in zip(P, coef):
+                    Pn = c1 * P0 + c2 * P1 + c3 * E_prev
+                    newP.append((Pn, P0))
+                    tot = tot + Pn
+                e_new = (D - tot) * inv
+                E = E.reshape(-1).index_copy(0, idx, e_new).reshape(E.shape)
+                gstate[q] = (D, e_new, newP)
             # DFT sample after each chunk (engine.py:564-592)
             if (n + 1) % eng.stride == 0:
                 te = (n + 1) * eng.dt

@@ -221,7 +221,7 @@ def synthetic_table_source(ns, shape, seed=0, m=3, n_rows=40, n_pts=None):
 # --- adjoint ------------------------------------------------------------------------------------------
 
 def adjoint_case(shape=(34, 28, 22), pml=6, steps=240, stride=8, seed=4, design=((13, 9, 8), (21, 19, 14)),
-                 wavelength=400.0, monitor="plane"):
+                 wavelength=400.0, monitor="plane", ade=False):
     """A small SOI-like adjoint problem: clad eps 1.44^2 with a core bar along x (eps 3.48^2), CPML with kappa
     and alpha, a mode-injection soft source on an x plane (E_y, Gaussian profile, Gaussian pulse), a design box
     of the given cells, and a two-frequency DFT monitor (E_y, E_z and face-averaged H_z, H_y) on an x plane
@@ -260,11 +260,22 @@ def adjoint_case(shape=(34, 28, 22), pml=6, steps=240, stride=8, seed=4, design=
         e_mon, h1, h2, _ = flux_points((i_mon - 3, jc - 5, kc - 4), (i_mon + 1, jc + 5, kc + 4), shape, dx)
     omega = omega_c * np.array([0.95, 1.05])
     rho = rng.uniform(0, 1, size=tuple(h - l for l, h in zip(lo, hi)))
+    ade_idx, ade_medium = None, None
+    if ade:
+        # a Drude + Lorentz block (silver-like: eps_inf 2, wp 9 eV, one Lorentz pole at 4 eV) in the core between
+        # the design box and the monitor plane: pure D-form nodes (inv_eps 1) whose update the adjoint transposes
+        from oracle.media import HBARC_EV_NM
+        w0 = 4.0 / HBARC_EV_NM
+        ade_medium = (2.0, ((9.0 / HBARC_EV_NM) ** 2, 0.0, 0.05 / HBARC_EV_NM), (1.5 * w0 ** 2, w0, 0.5 / HBARC_EV_NM))
+        blk = (slice(hi[0] + 1, hi[0] + 3), slice(jc - 3, jc + 3), slice(kc - 2, kc + 2))
+        ii, jj2, kk2 = np.meshgrid(*[np.arange(b.start, b.stop) for b in blk], indexing="ij")
+        ade_idx = np.sort(np.concatenate([flat_index(c, ii.ravel(), jj2.ravel(), kk2.ravel(), shape) for c in range(3)]))
+        inv_eps.reshape(-1)[ade_idx] = 1.0
     w_e = rng.standard_normal(len(e_mon)) + 1j * rng.standard_normal(len(e_mon))
     w_h = rng.standard_normal(len(h1)) + 1j * rng.standard_normal(len(h1))
     return dict(shape=shape, dx=dx, dt=dt, inv_eps=inv_eps, pml=pml, steps=steps, stride=stride, lo=lo, hi=hi,
                 eps_lo=e_clad, eps_hi=e_core, e_src=e_src, m_src=prof[:, None], t_src=pulse[:, None], omega=omega,
-                e_mon=e_mon, h1=h1, h2=h2, rho=rho, w_e=w_e, w_h=w_h)
+                e_mon=e_mon, h1=h1, h2=h2, rho=rho, w_e=w_e, w_h=w_h, ade_idx=ade_idx, ade_medium=ade_medium)
 
 
 def adjoint_objective(acc_e, acc_h, case):
@@ -297,7 +308,18 @@ def adjoint_oracle(case, rho):
     return fdtd.Engine(shape=case["shape"], dx=case["dx"], dt=case["dt"], inv_eps=adjoint_inv_eps(case, rho),
                        pml=fdtd.PMLSpec(cells=((p, p),) * 3, kappa_max=case.get("kappa_max", 2.0),
                                         alpha_max=case.get("alpha_max", 0.02), eps_ref=case["eps_lo"]),
-                       sources=[src], monitors=[mon], dft_stride=case["stride"])
+                       sources=[src], monitors=[mon], dft_stride=case["stride"],
+                       groups=_adjoint_groups(oracle_ns(), case))
+
+
+def _adjoint_groups(ns, case):
+    """The case's pure D-form group (none without ade) in namespace ns."""
+    if case.get("ade_idx") is None:
+        return []
+    eps_inf, *poles = case["ade_medium"]
+    med = ns.Medium(eps_inf, tuple(ns.Pole(*p) for p in poles))
+    idx = case["ade_idx"]
+    return [ns.DispersiveGroup(idx=idx, media=[med], frac=np.ones((1, len(idx))), w=None)]
 
 
 def adjoint_problem(case, device, **kw):
@@ -313,7 +335,8 @@ def adjoint_problem(case, device, **kw):
                           pml=PMLSpec(cells=((p, p),) * 3, kappa_max=2.0, alpha_max=0.02, eps_ref=case["eps_lo"]),
                           sources=[src], monitor=mon,
                           design=DesignRegion(case["lo"], case["hi"], case["eps_lo"], case["eps_hi"]),
-                          n_steps=case["steps"], dft_stride=case["stride"], **kw)
+                          n_steps=case["steps"], dft_stride=case["stride"], groups=_adjoint_groups(product_ns(), case),
+                          **kw)
 
 
 # --- conformal vertex tensor ---------------------------------------------------------------------------
@@ -370,7 +393,7 @@ def tensor_sphere(ns, device=None, shape=(24, 24, 24), steps=300, periodic=(Fals
     return make(ns, device, **kw)
 
 
-def oracle_scattering_engine(scene):
+def oracle_scattering_engine(scene, dtype=np.float64):
     """The FP64 oracle Engine of a lightsmith_amd.fdtd.scattering.Scene: the same material arrays, groups (the
     vertex tensor as the oracle's TensorMesh), off-diagonal rows, CPML, source tables and monitor index sets."""
     from oracle import fdtd, sources
@@ -392,5 +415,5 @@ def oracle_scattering_engine(scene):
     lay = scene.layout
     od = scene.offdiag
     return fdtd.Engine(shape=lay.shape, dx=lay.dx, dt=scene.dt, inv_eps=scene.inv_eps, groups=groups, pml=pml,
-                       sources=[src], monitors=mons, dft_stride=scene.stride,
+                       sources=[src], monitors=mons, dft_stride=scene.stride, dtype=dtype,
                        offdiag=None if od is None else fdtd.OffDiagonal(od.idx3, od.comp, od.coef))

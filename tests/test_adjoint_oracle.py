@@ -2,6 +2,7 @@
 same forward spectra, and autograd gradients equal to central finite differences of the oracle."""
 
 import numpy as np
+import pytest
 import torch
 
 from oracle.torch_fdtd import TorchRun
@@ -18,8 +19,9 @@ def _oracle_loss(case, rho):
     return cases.adjoint_objective(e, h, case), e, h
 
 
-def test_torch_run_reproduces_the_oracle_spectra():
-    case = cases.adjoint_case(**SMALL)
+@pytest.mark.parametrize("ade", [False, True])
+def test_torch_run_reproduces_the_oracle_spectra(ade):
+    case = cases.adjoint_case(**SMALL, ade=ade)
     rho = case["rho"]
     _, e, h = _oracle_loss(case, rho)
     run = TorchRun(cases.adjoint_oracle(case, rho), case["lo"], case["hi"], case["eps_lo"], case["eps_hi"])
@@ -29,8 +31,11 @@ def test_torch_run_reproduces_the_oracle_spectra():
     assert np.abs(th.numpy() - h).max() < 1e-12 * np.abs(h).max()
 
 
-def test_torch_autograd_equals_oracle_finite_differences():
-    case = cases.adjoint_case(**SMALL)
+@pytest.mark.parametrize("ade", [False, True])
+def test_torch_autograd_equals_oracle_finite_differences(ade):
+    """With ade, a Drude + Lorentz D-form block sits between the design box and the monitor: the gradient flows
+    through the pole recursion."""
+    case = cases.adjoint_case(**SMALL, ade=ade)
     rho0 = case["rho"]
     run = TorchRun(cases.adjoint_oracle(case, rho0), case["lo"], case["hi"], case["eps_lo"], case["eps_hi"])
     rho = torch.as_tensor(rho0).requires_grad_(True)
