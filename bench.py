@@ -16,6 +16,8 @@ scene; the FP64 NumPy oracle (one thread) alongside.
 batched: the oblique sweep (64 uneven Bloch-periodic runs) stepped as one autotuned Batch against the same runs
 stepped one after another on the plain engine; aggregate Gcell-updates/s of both and their ratio.
 adjoint: time of dL/drho (forward + adjoint run) over one forward run on the 256x192x96 adjoint scene.
+au_sphere: the plain Au sphere (97^3, vertex tensor, table-fed source, 81-wavelength flux boxes), 4000 steps.
+batched_adjoint: 16 dispersive inverse-design instances, one batched gradient against 16 sequential ones.
 """
 
 from __future__ import annotations
@@ -245,6 +247,114 @@ def read_traffic():
         return json.load(f)
 
 
+def au_sphere_leg(dev) -> dict:
+    """The plain Au sphere (workloads.au_sphere_sim: 97^3 nodes at 2 nm, 12-cell CPML, Drude + 2 Lorentz gold with
+    the conformal vertex tensor, table-fed plane wave 400-800 nm, 81-wavelength flux boxes, DFT every 100 steps):
+    4000 steps through HIP graphs, the whole step timed (bulk kernels, table source, vertex tensor, pure ADE, DFT
+    of 71,664 flux points x 81 wavelengths); the bulk kernels timed alone with HIP events for the roofline."""
+    import torch
+
+    from lightsmith_amd.fdtd import workloads
+
+    t0 = time.perf_counter()
+    sim, opts, scene = workloads.au_sphere_scene(dev, dft_stride=100)
+    eng = scene.engine(dev)
+    t_set = time.perf_counter() - t0
+    steps = 4000
+    run, used = eng.make_runner(True)
+    run()
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    for _ in range(steps // eng.stride - 1):
+        run()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - a
+    timed = steps - eng.stride
+    eng.release()
+    ms = (ctypes.c_double * 2)(0.0, 0.0)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(eng._dev).cuda_stream)
+    from lightsmith_amd import _lib
+    _lib.check(eng.lib.lsf_step_timed(eng._h, 400, stream, ms), "lsf_step_timed")
+    kern_s = (ms[0] + ms[1]) / 400 / 1e3
+    b = eng.bytes_per_step()
+    n = eng.N
+    achieved = b["total"] / kern_s / 1e9
+    out = {"workload": "plain Au sphere: r = 40 nm Drude + 2 Lorentz gold in water, 97^3 nodes at 2 nm (12-cell CPML), "
+                       "conformal vertex tensor, table-fed plane wave 400-800 nm, 81-wavelength flux boxes, "
+                       f"{steps} steps, DFT every 100, float32, HIP graphs",
+           "cells": n, "steps": timed, "ms_per_step": round(wall / timed * 1e3, 4),
+           "gcell_per_s": round(n * timed / wall / 1e9, 3), "hip_graphs": used, "setup_s": round(t_set, 2),
+           "voxelise_s": round(scene.voxelise_s, 3), "tensor_vertices": scene.stats["tensor_vertices"],
+           "dispersive_cells": scene.stats["dispersive_cells"],
+           "bulk_kernels_us": {"h": round(ms[0] / 400 * 1e3, 2), "e": round(ms[1] / 400 * 1e3, 2)},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_step": int(b["total"])}}
+    log(f"Au sphere: {out['gcell_per_s']} Gcell-updates/s ({out['ms_per_step'] * 1e3:.1f} us per step), bulk kernels "
+        f"{achieved:.0f} GB/s, set-up {t_set:.1f} s")
+    eng.close()
+    return out
+
+
+def batched_adjoint_leg(dev, steps: int = 2500) -> dict:
+    """Batched forward + adjoint with dispersion (workloads.batched_adjoint_scenes: 16 uneven waveguide instances,
+    each with a density design region and a Drude + 3 Lorentz silver cube, checkpoint recompute every 100 steps):
+    one batched gradient of all instances against the same 16 gradients computed one instance after another."""
+    import torch
+
+    from lightsmith_amd.fdtd import workloads
+    from lightsmith_amd.fdtd.adjoint import BatchedAdjoint, batched_monitor_spectra, monitor_spectra
+
+    scenes = workloads.batched_adjoint_scenes(n_steps=steps)
+    probs = [workloads.adjoint_problem(sc, dev, checkpoint_every=100, tape_budget=1) for sc in scenes]
+    cells = sum(sc["n_cells"] for sc in scenes)
+
+    def seq():
+        gs = []
+        for p, sc in zip(probs, scenes):
+            rho = torch.tensor(sc["rho"], dtype=torch.float64, device=dev, requires_grad=True)
+            acc_e, _ = monitor_spectra(rho, p)
+            workloads.mode_power(acc_e, sc["w"]).backward()
+            gs.append(rho.grad)
+        return gs
+
+    seq()  # warm-up: each instance's chunk graph
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    g_seq = seq()
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - a
+    for p in probs:
+        p.release_graphs()
+    batch = BatchedAdjoint(probs)
+
+    def bat():
+        rhos = [torch.tensor(sc["rho"], dtype=torch.float64, device=dev, requires_grad=True) for sc in scenes]
+        outs = batched_monitor_spectra(rhos, batch)
+        sum(workloads.mode_power(e, sc["w"]) for (e, _), sc in zip(outs, scenes)).backward()
+        return [r.grad for r in rhos]
+
+    bat()
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    g_bat = bat()
+    torch.cuda.synchronize()
+    t_bat = time.perf_counter() - a
+    same = all(bool(torch.equal(x, y)) for x, y in zip(g_seq, g_bat))
+    batch.close()
+    for p in probs:
+        p.close()
+    log(f"batched adjoint: 16 gradients {t_bat * 1e3:.0f} ms batched, {t_seq * 1e3:.0f} ms one after another "
+        f"({t_seq / t_bat:.2f}x); identical {same}")
+    return {"workload": "batched adjoint with dispersion: 16 uneven waveguide instances (128-256 x 96-192 x 64-96 "
+                        "cells, seed 6), density design region + Drude + 3 Lorentz silver cube each, "
+                        f"{steps} steps, checkpoint recompute every 100 steps, float32",
+            "instances": len(scenes), "cells": int(cells), "steps": steps,
+            "batched_ms": round(t_bat * 1e3, 1), "sequential_ms": round(t_seq * 1e3, 1),
+            "speedup": round(t_seq / t_bat, 3), "batched_equals_sequential": same,
+            "gradient_cell_updates_per_s": {"batched": round(cells * steps / t_bat / 1e9, 3),
+                                            "sequential": round(cells * steps / t_seq / 1e9, 3)}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +372,8 @@ def main() -> int:
                     help="steps per run of the batched-vs-sequential oblique sweep (0: skip)")
     ap.add_argument("--batched-only", action="store_true", help="run only the oblique sweep comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
+    ap.add_argument("--no-au-sphere", action="store_true", help="skip the plain Au sphere leg")
+    ap.add_argument("--no-batched-adjoint", action="store_true", help="skip the batched dispersive adjoint leg")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint cost measurement")
     ap.add_argument("--no-tune", action="store_true", help="batched sweep: keep the default schedule")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -381,6 +493,8 @@ def main() -> int:
     if args.batched_steps > 0:
         batched = batched_leg(dev, args.batched_steps, tune=not args.no_tune)
     adjoint = None if args.no_adjoint else adjoint_leg(dev)
+    au = None if args.no_au_sphere else au_sphere_leg(dev)
+    badj = None if args.no_batched_adjoint else batched_adjoint_leg(dev)
 
     out = {"metric": METRIC, "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": K,
            "warmup": W, "steps_requested": args.steps, "warmup_requested": args.warmup,
@@ -397,7 +511,8 @@ def main() -> int:
                       "tiling_note": "rows per lane, wavefronts per block, x cells per row, z planes per tile; the E "
                                      "kernel of an engine with pole state marches twice the z planes",
                       "hip_graphs": graphs, "parallelism": "single GPU"},
-           "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "adjoint": adjoint}
+           "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "adjoint": adjoint, "au_sphere": au,
+           "batched_adjoint": badj}
     print(json.dumps(out), flush=True)
     return 0
 

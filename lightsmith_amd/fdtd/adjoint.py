@@ -207,7 +207,15 @@ class AdjointProblem:
     def set_density(self, rho: torch.Tensor):
         if rho.numel() != self.design.n_voxels:
             raise ValueError(f"rho has {rho.numel()} values, the design region {self.design.n_voxels}")
-        eps = self.eps(rho.to(self._dev))
+        self.set_permittivity(self.eps(rho.to(self._dev)))
+
+    def set_permittivity(self, eps: torch.Tensor):
+        """Permittivity per design voxel (C order over the box), set at the voxel's E_x, E_y, E_z nodes."""
+        eps = eps.to(device=self._dev, dtype=torch.float64).reshape(-1)
+        if eps.numel() != self.design.n_voxels:
+            raise ValueError(f"eps has {eps.numel()} values, the design region {self.design.n_voxels}")
+        if float(eps.min()) <= 0:
+            raise ValueError("the design permittivity must be positive")
         cb = (self.fwd.cdx / eps)[self._vox].to(torch.float32)
         self.fwd._mat[self._nodes] = cb
         self._adj_mat[self._nodes] = cb
@@ -222,6 +230,14 @@ class AdjointProblem:
     def forward(self, rho: torch.Tensor):
         """Forward run for density rho: the monitor's (acc_e, acc_h) after n_steps (copies)."""
         self.set_density(rho)
+        return self._forward_run()
+
+    def forward_permittivity(self, eps: torch.Tensor):
+        """Forward run with the design voxels' permittivity given directly: the monitor's (acc_e, acc_h)."""
+        self.set_permittivity(eps)
+        return self._forward_run()
+
+    def _forward_run(self):
         self._zero(self.fwd)
         self.runs += 1
         fwd = self.fwd
@@ -261,6 +277,16 @@ class AdjointProblem:
     def interval(self, seg: int) -> int:
         return min(self.K, self.n_steps - seg * self.K)
 
+    def finish_backward_permittivity(self) -> torch.Tensor:
+        """dL/deps per design voxel (float64) from the accumulated dL/dcb: cb = C / eps, dcb/deps = -cb^2 / C."""
+        cb = self.cb_design.to(torch.float64)
+        return (self.g * (-(cb * cb) / self.fwd.cdx)).view(3, self.design.n_voxels).sum(0)
+
+    def backward_permittivity(self, grad_e: torch.Tensor | None, grad_h: torch.Tensor | None) -> torch.Tensor:
+        """dL/deps per design voxel for the last forward run (forward or forward_permittivity)."""
+        self._adjoint_pass(grad_e, grad_h)
+        return self.finish_backward_permittivity()
+
     def finish_backward(self) -> torch.Tensor:
         """dL/drho (float64) from the accumulated dL/dcb of the design nodes."""
         d = self.design
@@ -273,6 +299,10 @@ class AdjointProblem:
     def backward(self, grad_e: torch.Tensor | None, grad_h: torch.Tensor | None) -> torch.Tensor:
         """dL/drho (float64, the shape of rho's flattening) for the objective whose gradients with respect to the
         monitor's accumulators are grad_e, grad_h (torch's convention for complex tensors)."""
+        self._adjoint_pass(grad_e, grad_h)
+        return self.finish_backward()
+
+    def _adjoint_pass(self, grad_e, grad_h):
         self.begin_backward(grad_e, grad_h)
         if self.store_all:
             self.base.fill_(0)
@@ -282,11 +312,15 @@ class AdjointProblem:
                 self.restore_checkpoint(seg)
                 self.fwd.steps(self.interval(seg))
                 self.adj.steps(self.interval(seg))
-        return self.finish_backward()
 
-    def close(self):
+    def release_graphs(self):
+        """Drop the forward engine's captured chunk graph (before the engine joins a batch)."""
         if self._run is not None:
             self.fwd.release()
+            self._run = None
+
+    def close(self):
+        self.release_graphs()
         self.fwd.close()
         self.adj.close()
 
