@@ -117,7 +117,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
   const bool has_r = x0 + 64 < nx && jin;                       // the column x + 64 exists
   const unsigned oxl = ((unsigned)jc * nx + (unsigned)(has_l ? x0 - 1 : 0)) * 4u;
   const unsigned oxr = ((unsigned)jc * nx + (unsigned)(has_r ? x0 + 64 : 0)) * 4u;
-  const unsigned long long below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  // lanes of this lane's 32-cell chunk below it (the rank of a D-form node within its chunk)
+  const unsigned long long below = (lane & 31) == 0 ? 0ull : (((~0ull) >> (64 - (lane & 31))) << ((lane >> 5) * 32));
 
   gptr<const float> __restrict__ Ex = as_global(a.E_in);
   gptr<const float> __restrict__ Ey = as_global(a.E_in + g.N);
@@ -218,7 +219,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
 #pragma unroll
       for (int q = 0; q < 3; ++q)
         ab[q] = ldi(as_global(a.chunk_base),
-                    ((((unsigned)q * nz + (unsigned)k) * ny + (unsigned)jc) * a.nchunk + (unsigned)tx * 2u) * 4u);
+                    ((((unsigned)k * ny + (unsigned)jc) * a.nchunk + (unsigned)tx * 2u + (unsigned)(lane >> 5)) * 3u +
+                     (unsigned)q) * 4u);
   };
   load_mat(z0, idc, cbc, abc);
   __syncthreads();  // LDS tables ready

@@ -420,7 +420,9 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
   const int il = tx * XW - 1;
   const bool xl_ok = (xl == 0) && il >= 0;
   // lanes of this lane's x row below it
-  const unsigned long long below = xl == 0 ? 0ull : (((~0ull) >> (64 - xl)) << (yl * 32));
+  // lanes of this lane's 32-cell chunk below it (the rank of a D-form node within its chunk)
+  const int xc = xl & 31;
+  const unsigned long long below = xc == 0 ? 0ull : (((~0ull) >> (64 - xc)) << ((yl + (xl >> 5)) * 32));
   const float4 px = (a.pml[0].prof && inx) ? ((const float4*)as_global(a.pml[0].prof))[i] : no_pml();
   float4 py[RY];
 #pragma unroll
@@ -434,12 +436,14 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
   uint32_t id_c[RY], id_n[RY], id_nn[RY];
   float cbd_c[RY][MAT == 0 ? 3 : 1], cbd_n[RY][MAT == 0 ? 3 : 1], cbd_nn[RY][MAT == 0 ? 3 : 1];
   int ab_c[RY][3], ab_n[RY][3], ab_nn[RY][3];  // first D-form slot of each row's x chunk per component
-  // first D-form slot of each row's 32-cell x chunk per component: chunk_base[((c n2 + k) ny + j) nchunk + x / 32],
-  // one vector load per row and component (the index sits in a VGPR), so that these loads join the plane's other
-  // loads instead of waiting one by one in the scalar cache
+  // first D-form slot of this lane's 32-cell x chunk per component: chunk_base[((k ny + j) nchunk + x / 32) 3 + c]
+  // (slots ordered by plane, row, chunk, component, x: a wavefront row's pole pairs of all three components are one
+  // contiguous block), one vector load per row and component (the index sits in a VGPR), so that these loads join
+  // the plane's other loads instead of waiting one by one in the scalar cache
   const gptr<const int> cbase = as_global(a.chunk_base);
   const unsigned cb_k = ny * (unsigned)a.nchunk;  // per plane
-  const unsigned cb_j = (unsigned)vregi((int)((unsigned)j0 * (unsigned)a.nchunk + (unsigned)tx * (XW / 32)));
+  const unsigned cb_j = (unsigned)vregi((int)((unsigned)j0 * (unsigned)a.nchunk + (unsigned)tx * (XW / 32) +
+                                               (unsigned)(xl >> 5)));
   auto load_bases = [&](int k, int (*ab)[3]) {
 #pragma unroll
     for (int r = 0; r < RY; ++r)
@@ -447,7 +451,7 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
       for (int c = 0; c < 3; ++c) {
         ab[r][c] = 0;
         if (NP > 0 && k < g.n[2] && j0 + r < g.n[1])
-          ab[r][c] = ((const int*)cbase)[cb_j + (unsigned)r * (unsigned)a.nchunk + ((unsigned)c * g.n[2] + (unsigned)k) * cb_k];
+          ab[r][c] = ((const int*)cbase)[(cb_j + (unsigned)r * (unsigned)a.nchunk + (unsigned)k * cb_k) * 3u + (unsigned)c];
       }
   };
   auto load_mat = [&](int k, uint32_t* ids, float (*cb)[MAT == 0 ? 3 : 1]) {

@@ -345,8 +345,11 @@ class BatchedAdjoint:
                                  "and tape mode")
         self.stride, self.K, self.n_steps, self.store_all = p0.stride, p0.K, p0.n_steps, p0.store_all
         self.use_graphs = use_graphs
-        self.fwd = Batch([p.fwd for p in self.problems], schedule=schedule)
-        self.adj = Batch([p.adj for p in self.problems], schedule=schedule)
+        # the members' own tile shape (one block per tile) unless told otherwise: the batch default suits the
+        # pole-free complex sweep; engines with up to 4 poles per node need one row per lane to keep registers
+        sched = schedule or (*p0.fwd.tiling, 0)
+        self.fwd = Batch([p.fwd for p in self.problems], schedule=sched)
+        self.adj = Batch([p.adj for p in self.problems], schedule=sched)
         self._run = None
         self.runs = 0
 

@@ -191,15 +191,24 @@ def to_reference(native, shape) -> np.ndarray:
 CHUNK = 32  # x cells per chunk of the D-form slot table (the narrowest wavefront row of the E kernel)
 
 
-def ade_chunk_base(nodes, shape) -> np.ndarray:
-    """First compact slot of each 32-cell chunk (c, k, j, x // 32) for D-form nodes sorted by native index: the
-    E kernel finds a node's slot as the base of its row's chunk + (D-form lanes below it in that row)."""
+def ade_slot_key(nodes, shape) -> np.ndarray:
+    """Slot order of D-form nodes: plane k, row j, 32-cell x chunk, component, then x -- a wavefront row's pole
+    pairs of all three components form one contiguous block."""
     n0, n1, n2 = shape
     N = n0 * n1 * n2
     nchunk = (n0 + CHUNK - 1) // CHUNK
     c, cell = np.divmod(np.asarray(nodes, dtype=np.int64), N)
-    keys = (c * (n1 * n2) + cell // n0) * nchunk + (cell % n0) // CHUNK
-    return np.searchsorted(keys, np.arange(3 * n1 * n2 * nchunk), side="left").astype(np.int32)
+    x = cell % n0
+    return (((cell // n0) * nchunk + x // CHUNK) * 3 + c) * CHUNK + x % CHUNK
+
+
+def ade_chunk_base(nodes, shape) -> np.ndarray:
+    """First compact slot of each (k, j, x // 32, component) for D-form nodes in slot order (ade_slot_key): the E
+    kernel finds a node's slot as the base of its chunk and component + (D-form lanes below it in that chunk)."""
+    n0, n1, n2 = shape
+    nchunk = (n0 + CHUNK - 1) // CHUNK
+    keys = ade_slot_key(nodes, shape) // CHUNK
+    return np.searchsorted(keys, np.arange(n1 * n2 * nchunk * 3), side="left").astype(np.int32)
 
 
 def encode_material(cb: np.ndarray):
@@ -376,6 +385,14 @@ class Engine:
             arr = (ctypes.c_void_p * 3)(*[None if t is None else t.data_ptr() for t in self._psiH_second])
             check(self.lib.lsf_set_fused(self._h, ptr(self._E_bufs[1]), ptr(self._H_bufs[1]), arr), "lsf_set_fused")
 
+    def set_tiling(self, rows_per_lane: int, waves: int, x_cells: int, z_chunk: int):
+        """Launch tiling of this engine's bulk kernels (lsf_set_tiling); before a graph is captured."""
+        if self._graphed:
+            raise ValueError("release the captured graph before changing the tiling")
+        check(self.lib.lsf_set_tiling(self._h, int(rows_per_lane), int(waves), int(x_cells), int(z_chunk)),
+              "lsf_set_tiling")
+        self.tiling = (int(rows_per_lane), int(waves), int(x_cells), int(z_chunk))
+
     # --- fields: the current buffer set ------------------------------------------------------
     @property
     def parity(self) -> int:
@@ -481,6 +498,8 @@ class Engine:
         self.ade_nodes, self.ade_medium = self.ade_nodes[order], self.ade_medium[order]
         if len(self.ade_nodes) and np.any(np.diff(self.ade_nodes) == 0):
             raise ValueError("an E node belongs to two dispersive groups")
+        # pole-state slots follow ade_slot_key; ade_nodes stays sorted by native index for lookups
+        self.ade_slot_order = np.argsort(ade_slot_key(self.ade_nodes, self.shape), kind="stable")
         n = len(self.ade_nodes)
         self.n_ade = n
         if not n:
@@ -494,7 +513,7 @@ class Engine:
             tab[q, 0], tab[q, 1] = 1.0 / m.eps_inf, len(m.poles)
             for p, pole in enumerate(m.poles):
                 tab[q, 2 + 3 * p:5 + 3 * p] = pole_coefficients(pole, self.dt)
-        chunk_base = ade_chunk_base(self.ade_nodes, self.shape)
+        chunk_base = ade_chunk_base(self.ade_nodes[self.ade_slot_order], self.shape)
         self._ade_P = torch.zeros(max_p * n * 2, dtype=self.fdt, device=self._dev)
         self._keep.append(self._ade_P)
         self._state.append(self._ade_P)

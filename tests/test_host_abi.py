@@ -79,26 +79,34 @@ def test_native_index_round_trip_and_layout():
 @pytest.mark.parametrize("xw", [64, 32])
 @pytest.mark.parametrize("shape", [(70, 9, 6), (64, 3, 4), (130, 5, 5), (33, 4, 3)])
 def test_ade_chunk_base_reproduces_the_compact_slots(shape, xw):
-    """Emulate the E kernel: a wavefront row covers xw x cells; a node's slot = the base of the row's first
-    32-cell chunk + number of D-form nodes at lower x in the same row (the ballot restricted to the row). It
-    must equal the node's position in native order."""
+    """Emulate the E kernel: a wavefront row covers xw x cells; a node's slot = the base of its (plane, row, 32-cell
+    chunk, component) + the number of D-form nodes of that component at lower x in the same chunk (the ballot
+    restricted to the chunk). It must equal the node's position in slot order, which puts a wavefront row's three
+    components' nodes in one contiguous block."""
     rng = np.random.default_rng(1)
     N = int(np.prod(shape))
     nodes = np.sort(rng.choice(3 * N, size=3 * N // 7, replace=False))
-    base = peng.ade_chunk_base(nodes, shape)
+    order = np.argsort(peng.ade_slot_key(nodes, shape), kind="stable")
+    base = peng.ade_chunk_base(nodes[order], shape)
     n0, n1, n2 = shape
-    assert len(base) == 3 * n1 * n2 * ((n0 + 31) // 32)
+    nchunk = (n0 + 31) // 32
+    assert len(base) == 3 * n1 * n2 * nchunk
     c, cell = np.divmod(nodes, N)
     i = cell % n0
-    row = c * (n1 * n2) + cell // n0
-    tile = i // xw
-    chunk = row * ((n0 + 31) // 32) + tile * (xw // 32)
-    group = row * ((n0 + xw - 1) // xw) + tile
+    row = cell // n0  # k * n1 + j
+    chunk = (row * nchunk + i // 32) * 3 + c
     rank = np.zeros(len(nodes), int)
-    for q in np.unique(group):
-        sel = np.flatnonzero(group == q)
+    for q in np.unique(chunk):
+        sel = np.flatnonzero(chunk == q)
         rank[sel] = np.argsort(np.argsort(i[sel]))
-    assert np.array_equal(base[chunk] + rank, np.arange(len(nodes)))
+    slot = np.empty(len(nodes), int)
+    slot[order] = np.arange(len(nodes))
+    assert np.array_equal(base[chunk] + rank, slot)
+    # a wavefront row's nodes (all components) occupy one contiguous range of slots
+    group = row * ((n0 + xw - 1) // xw) + i // xw
+    for q in np.unique(group):
+        s = np.sort(slot[group == q])
+        assert s[-1] - s[0] + 1 == len(s)
 
 
 def test_material_encoding_is_exact():
