@@ -288,13 +288,13 @@ lsf::FusedArgs fused_args(const lsf_engine* e, int p) {
   a.counter = e->counter;
   a.tick = e->tick;
   a.tiles_x = (e->g.n[0] + 63) / 64;
-  a.tiles_y = (e->g.n[1] + e->waves - 2) / (e->waves - 1);  // waves - 1 owned rows per block
+  a.tiles_y = (e->g.n[1] + e->waves - 1) / e->waves;  // one row per wavefront
   a.zc = e->zc;
   return a;
 }
 
 // The single-pass tiles in three lists by the CPML axes they touch: a tile reads psi on the x span [x0 - 1, x0 + 64),
-// the rows [j0 - 1, j0 - 1 + waves) and the planes [z0 - 1, z1) (the halo cells it recomputes included). Lists: no
+// the rows [j0 - 1, j0 + waves) and the planes [z0 - 1, z1) (the halo cells it recomputes included). Lists: no
 // CPML, x CPML only, any other.
 int fused_tiles(lsf_engine* e, const lsf::FusedArgs& a) {
   const int n[3] = {e->g.n[0], e->g.n[1], e->g.n[2]};
@@ -321,8 +321,8 @@ int fused_tiles(lsf_engine* e, const lsf::FusedArgs& a) {
   for (int tz = 0; tz < tz_n; ++tz)
     for (int ty = 0; ty < a.tiles_y; ++ty)
       for (int tx = 0; tx < a.tiles_x; ++tx) {
-        const int z0 = tz * a.zc, z1 = std::min(z0 + a.zc, n[2]), j0 = ty * (e->waves - 1), x0 = tx * 64;
-        const int m = (touches(0, x0 - 1, x0 + 64) ? 1 : 0) | (touches(1, j0 - 1, j0 - 1 + e->waves) ? 2 : 0) |
+        const int z0 = tz * a.zc, z1 = std::min(z0 + a.zc, n[2]), j0 = ty * e->waves, x0 = tx * 64;
+        const int m = (touches(0, x0 - 1, x0 + 64) ? 1 : 0) | (touches(1, j0 - 1, j0 + e->waves) ? 2 : 0) |
                       (touches(2, z0 - 1, z1) ? 4 : 0);
         lists[m == 0 ? 0 : (m == 1 ? 1 : 2)].push_back((tz * a.tiles_y + ty) * a.tiles_x + tx);
       }

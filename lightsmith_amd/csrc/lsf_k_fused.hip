@@ -1,5 +1,5 @@
 // Instantiations of the single-pass step kernel (real fields, no periodic axes, one row per wavefront, 64 x cells):
-// material mode x pole registers x wavefronts per block (4 or 8) x CPML class (none, x only, any).
+// material mode x poles per D-form node x wavefronts per block (4 or 8) x CPML class (none, x only, any).
 
 #include "lsf_fused.h"
 #include "lsf_launch.h"
@@ -29,9 +29,9 @@ void by_waves(const FusedLaunch& L, hipStream_t s) {
 
 template <int MAT>
 void by_poles(const FusedLaunch& L, hipStream_t s) {
-  switch (L.np) {
+  switch (L.np) {  // the pole count is the slot stride: exact
     case 0: return by_waves<MAT, 0>(L, s);
-    case 1:
+    case 1: return by_waves<MAT, 1>(L, s);
     case 2: return by_waves<MAT, 2>(L, s);
     case 3: return by_waves<MAT, 3>(L, s);
     default: return by_waves<MAT, 4>(L, s);

@@ -442,8 +442,10 @@ __device__ __forceinline__ void e_tile(const StepArgs& a, int t, const float* ta
   // the plane's other loads instead of waiting one by one in the scalar cache
   const gptr<const int> cbase = as_global(a.chunk_base);
   const unsigned cb_k = ny * (unsigned)a.nchunk;  // per plane
-  const unsigned cb_j = (unsigned)vregi((int)((unsigned)j0 * (unsigned)a.nchunk + (unsigned)tx * (XW / 32) +
-                                               (unsigned)(xl >> 5)));
+  // (a lane past the row's last chunk, in a tile that ends within 32 cells of x0, reads the last chunk's entry:
+  // it holds no node, and the index stays inside the table at the grid's last row and plane)
+  const unsigned cb_j = (unsigned)vregi((int)((unsigned)j0 * (unsigned)a.nchunk +
+                                               (unsigned)min(tx * (XW / 32) + (xl >> 5), a.nchunk - 1)));
   auto load_bases = [&](int k, int (*ab)[3]) {
 #pragma unroll
     for (int r = 0; r < RY; ++r)
