@@ -14,8 +14,9 @@
 //
 // Instruction economy: field loads use clamped, in-grid addresses given as 32-bit byte offsets from a wave-uniform
 // base (no branches around them, no 64-bit address arithmetic); values outside a component's stepped box are masked
-// with selects; the x - 1 and x + 64 halo cells of the row are wave-uniform (E_in and H_in are read-only here, so
-// they come through the scalar cache); tests on the row and the plane are wave-uniform.
+// with selects; tests on the row and the plane are wave-uniform. The x - 1 and x + 64 halo cells of the row are
+// wave-uniform addresses read by vector loads: as scalar loads (constant address space) they share the lgkm
+// counter with the LDS table lookups and the lane shuffles, whose waits then wait for them too (slower).
 #pragma once
 
 #include "lsf_step.h"
@@ -161,7 +162,6 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
     return base + ((unsigned)p * ny + (unsigned)row) * R4 + lane4;
   };
   const unsigned ic4 = (unsigned)ic * 4u;
-  const int pxq = pxH >= 0 ? pxH : 0;  // the lane's x slab index, clamped in range
 
   // h_new = h_old - cH (s_d1 da - s_d2 db) + CPML terms in ascending axis order (the H kernel's arithmetic);
   // use1 / use2: the term of axis d1 / d2 applies (in its slab)
@@ -254,9 +254,9 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
       qx1 = ld(as_global(a.psiHi[2]), psi_off(1, 0, 2, pzH, k, j, ic4));
       qy0 = ld(as_global(a.psiHi[2]), psi_off(1, 1, 2, pzH, k, j, ic4));
     }
-    if (xslab) {  // x slab: H_y (d2 = x) and H_z (d1 = x), per lane
-      qy1 = ld(as_global(a.psiHi[0]), psi_off(1, 1, 0, pxq, k, j, 0u));
-      qz0 = ld(as_global(a.psiHi[0]), psi_off(1, 2, 0, pxq, k, j, 0u));
+    if (xslab && pxH >= 0) {  // x slab: H_y (d2 = x) and H_z (d1 = x), in the slab's lanes only
+      qy1 = ld(as_global(a.psiHi[0]), psi_off(1, 1, 0, pxH, k, j, 0u));
+      qz0 = ld(as_global(a.psiHi[0]), psi_off(1, 2, 0, pxH, k, j, 0u));
     }
     // the same for H_x, H_z of the row below (read only: their owner advances them)
     float rx0 = 0.f, rx1 = 0.f, rz0 = 0.f, rz1 = 0.f;
@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
         rz1 = ld(as_global(a.psiHi[1]), psi_off(1, 2, 1, pyB, k, jb, ic4));
       }
       if (pzH >= 0) rx1 = ld(as_global(a.psiHi[2]), psi_off(1, 0, 2, pzH, k, jb, ic4));
-      if (xslab) rz0 = ld(as_global(a.psiHi[0]), psi_off(1, 2, 0, pxq, k, jb, 0u));
+      if (xslab && pxH >= 0) rz0 = ld(as_global(a.psiHi[0]), psi_off(1, 2, 0, pxH, k, jb, 0u));
     }
 
     // ---- the cell state of E(k): E-side psi, pole state, next material
@@ -283,8 +283,8 @@ __global__ void __launch_bounds__(64 * WAVES) k_step_fused(const FusedArgs* __re
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
           const int ax = (c + 1 + s) % 3;
-          if (ax == 0 && xslab)
-            pse[c][s] = ld(as_global((const float*)a.psiE[0]), psi_off(0, c, 0, pxE >= 0 ? pxE : 0, k, j, 0u));
+          if (ax == 0 && xslab && pxE >= 0)
+            pse[c][s] = ld(as_global((const float*)a.psiE[0]), psi_off(0, c, 0, pxE, k, j, 0u));
           if (ax == 1 && pyE >= 0) pse[c][s] = ld(as_global((const float*)a.psiE[1]), psi_off(0, c, 1, pyE, k, j, ic4));
           if (ax == 2 && pzE >= 0) pse[c][s] = ld(as_global((const float*)a.psiE[2]), psi_off(0, c, 2, pzE, k, j, ic4));
         }

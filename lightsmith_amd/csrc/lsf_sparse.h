@@ -413,14 +413,18 @@ __device__ __forceinline__ void dft_block(const DftJob<S>& J, int blk, c32* ph, 
       }
     }
   }
-  if (J.partial) {
-    red[threadIdx.x] = e2;
+  if (J.partial) {  // the block's sum of |E|^2 (+ |H|^2): a butterfly within each wavefront, then the wavefronts'
+                    // sums in order (a fixed order: the same partials on every run)
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) e2 += __shfl_xor(e2, s, 64);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if ((threadIdx.x & 63) == 0) red[w] = e2;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
+    if (threadIdx.x == 0) {
+      float s = 0.f;
+      for (int q = 0; q < nw; ++q) s += red[q];
+      J.partial[blk] = s;
     }
-    if (threadIdx.x == 0) J.partial[blk] = red[0];
   }
 }
 
