@@ -114,7 +114,10 @@ int lsf_add_table(lsf_engine* eng, int phase, int field, int64_t n_rows, const i
                   const int32_t* dev_row_ptr, const float* dev_T, int m, int64_t cmax, const int64_t* dev_base,
                   const int64_t* dev_i0, const float* dev_u0, const float* dev_coef, const float* dev_scale);
 
-/* Incident-wave view of a table source (layered.py:481-482): out[r] = quintic(T, t + i0[r], ...), r < n. */
+/* Incident-wave view of a table source (layered.py:481-482): out[r] = quintic(T, t + i0[r], ...), r < n.
+ * Its readers are monitors, so it is evaluated when the engine samples them (lsf_sample, the end of each
+ * chunk), with the step counters of the E half of the step just taken; phase must be an E-side phase
+ * (LSF_PH_OFFDIAG .. LSF_PH_ADVANCE_E). */
 int lsf_add_table_view(lsf_engine* eng, int phase, int n, float* dev_out, const float* dev_T, int m, int64_t cmax,
                        const int64_t* dev_base, const int64_t* dev_i0, const float* dev_u0);
 

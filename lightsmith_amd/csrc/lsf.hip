@@ -509,8 +509,7 @@ void run_op(lsf_engine* e, const Op& o, hipStream_t s) {
       else J = o.tjc;
       J.E = (T*)e->E;
       hipLaunchKernelGGL(lsf::k_tensor_d<T>, dim3(nblk(J.nN, 256)), dim3(256), 0, s, J);
-      hipLaunchKernelGGL(lsf::k_tensor_slot<T>, dim3(nblk(2 * J.nN, 256)), dim3(256), 0, s, J);
-      hipLaunchKernelGGL(lsf::k_tensor_e<T>, dim3(nblk(J.nN, 256)), dim3(256), 0, s, J);
+      hipLaunchKernelGGL(lsf::k_tensor_slot_e<T>, dim3(nblk(2 * J.nN, 256)), dim3(256), 0, s, J);
       break;
     }
     case OP_MIXED:
@@ -521,9 +520,11 @@ void run_op(lsf_engine* e, const Op& o, hipStream_t s) {
   }
 }
 
+// A table view's only readers are monitors, which read it when they sample: it is evaluated then (run_views), with
+// the step counters of its phase, which are the same then.
 void run_phase(lsf_engine* e, int phase, hipStream_t s) {
   for (const Op& o : e->ops) {
-    if (o.phase != phase) continue;
+    if (o.phase != phase || o.kind == OP_VIEW) continue;
     if (e->cplx) run_op<c32>(e, o, s);
     else run_op<float>(e, o, s);
   }
@@ -588,7 +589,16 @@ void sample_time(lsf_engine* e, hipStream_t s) {
   }
 }
 
+void run_views(lsf_engine* e, hipStream_t s) {
+  for (const Op& o : e->ops) {
+    if (o.kind != OP_VIEW) continue;
+    if (e->cplx) run_op<c32>(e, o, s);
+    else run_op<float>(e, o, s);
+  }
+}
+
 void sample(lsf_engine* e, hipStream_t s) {
+  run_views(e, s);
   for (const Monitor& m : e->mons) {
     const size_t shm = sizeof(c32) * 2 * m.nf;
     if (mon_complex(e, m))
@@ -823,6 +833,8 @@ int lsf_add_table_view(lsf_engine* e, int phase, int n, float* out, const float*
                        const int64_t* base, const int64_t* i0, const float* u0) {
   if (valid(e)) return LSF_EINVAL;
   if (n < 1 || n > 64 || !out || !T || !base || !i0 || !u0 || m < 1) return fail(LSF_EINVAL, "bad table view");
+  if (phase < LSF_PH_OFFDIAG || phase > LSF_PH_ADVANCE_E)
+    return fail(LSF_EINVAL, "a table view is evaluated at sampling with the E half's counters: give an E-side phase");
   Op o;
   o.kind = OP_VIEW;
   o.phase = phase;
@@ -1591,7 +1603,7 @@ int build_level(Level& L, const Tiling& t, int grid_blocks, bool sample, std::ve
     for (lsf_engine* e : L.mem) {
       size_t q = 0;
       for (const Op& o : e->ops) {
-        if (o.phase != ph) continue;
+        if (o.phase != ph || o.kind == OP_VIEW) continue;  // views run when the batch samples
         if (by_pos.size() <= q) by_pos.resize(q + 1);
         by_pos[q++].push_back({e, o});
       }
@@ -1666,6 +1678,7 @@ void level_e(Level& L, hipStream_t s) {
 
 void batch_sample(lsf_batch* b, hipStream_t s) {
   Level& L = b->root;
+  for (lsf_engine* e : L.mem) run_views(e, s);
   for (const DftLaunch& d : L.dft) {
     const dim3 grid(d.sc.n_tiles), block(256);
     if (d.cplx)

@@ -48,6 +48,8 @@ template <> __device__ __forceinline__ c32 zero<c32>() { return mk(0.f, 0.f); }
 __device__ __forceinline__ float shfl_dn(float v) { return __shfl_down(v, 1, 64); }
 __device__ __forceinline__ c32 shfl_dn(c32 v) { return mk(__shfl_down(v.x, 1, 64), __shfl_down(v.y, 1, 64)); }
 __device__ __forceinline__ float shfl_up(float v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ float shfl_xor1(float v) { return __shfl_xor(v, 1, 64); }
+__device__ __forceinline__ c32 shfl_xor1(c32 v) { return mk(__shfl_xor(v.x, 1, 64), __shfl_xor(v.y, 1, 64)); }
 __device__ __forceinline__ c32 shfl_up(c32 v) { return mk(__shfl_up(v.x, 1, 64), __shfl_up(v.y, 1, 64)); }
 
 // Grid geometry shared by every kernel of an engine.

@@ -309,14 +309,12 @@ __global__ void k_tensor_d(TensorJob<T> J) {
   J.D[q] = add(J.D[q], sub(J.E[J.node[q]], J.Ep[q]));
 }
 
-// one slot: the tensor vertex's share, or half the plain medium's
+// One slot per thread, a node's two slots on a lane pair (lane 2m: slot k = 0 at the node's index, lane 2m + 1:
+// k = 1 at + e_c): the tensor vertex's share, or half the plain medium's; the even lane then sets E_p = the sum of
+// the node's two slots (slot 0 + slot 1, the order of the reference's sum).
 template <class T>
-__global__ void k_tensor_slot(TensorJob<T> J) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ T tensor_slot(const TensorJob<T>& J, long long q, int k, long long node) {
   const long long n = 2 * J.nN;
-  if (q >= n) return;
-  const int k = q >= J.nN ? 1 : 0;
-  const long long node = q - k * J.nN;
   const int v = J.slot_v[q];
   T contrib;
   if (v >= 0) {
@@ -349,17 +347,27 @@ __global__ void k_tensor_slot(TensorJob<T> J) {
   } else {
     contrib = scl(0.5f, tensor_harmonic(J.D[node], q, n, J.S, J.maxp, J.sm, J.ms, J.f, J.h0[q], J.Eh, J.Ph));
   }
-  J.out[q] = contrib;
+  return contrib;
 }
 
-// E_p = the sum of its two slots
 template <class T>
-__global__ void k_tensor_e(TensorJob<T> J) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= J.nN) return;
-  const T e = add(J.out[q], J.out[J.nN + q]);
-  J.E[J.node[q]] = e;
-  J.Ep[q] = e;
+__global__ void k_tensor_slot_e(TensorJob<T> J) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long node = t >> 1;
+  const int k = (int)(t & 1);
+  const bool in = node < J.nN;  // lanes past the end still take part in the pair exchange
+  T contrib = zero<T>();
+  if (in) {
+    const long long q = k * J.nN + node;
+    contrib = tensor_slot(J, q, k, node);
+    J.out[q] = contrib;
+  }
+  const T other = shfl_xor1(contrib);
+  if (in && k == 0) {
+    const T e = add(contrib, other);
+    J.E[J.node[node]] = e;
+    J.Ep[node] = e;
+  }
 }
 
 // Running DFT monitor (engine.py:564-592). Block size 256; phases for all frequencies in LDS, computed in

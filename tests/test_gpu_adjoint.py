@@ -127,11 +127,10 @@ def test_adjoint_refuses_what_it_does_not_transpose(gpu):
         cases.adjoint_problem(dict(case, lo=(2, 9, 8)), gpu)  # design region inside the CPML
 
 
-@pytest.mark.skipif(not os.environ.get("LSF_ADJOINT_FULL"),
-                    reason="full adjoint scene against float64 central differences takes minutes; LSF_ADJOINT_FULL=1")
 def test_full_scene_gradient_matches_float64_central_differences(gpu):
-    """The 256x192x96 adjoint scene (3000 steps): dL/drho on 32 seeded voxels (seed 5) against central
-    differences of the float64 torch restatement run on the GPU (h = 1e-3)."""
+    """The 256x192x96 adjoint scene (3000 steps): dL/drho on seeded voxels (seed 5) against central differences of
+    the float64 torch restatement run on the GPU (h = 1e-3): the first 6 of the 32 seeded voxels (two float64 runs
+    of about 4 s each per voxel), all 32 with LSF_ADJOINT_FULL=1."""
     from lightsmith_amd.fdtd import workloads
     scene = workloads.adjoint_scene()
     prob = workloads.adjoint_problem(scene, gpu)
@@ -145,6 +144,7 @@ def test_full_scene_gradient_matches_float64_central_differences(gpu):
     run = TorchRun(o, scene["lo"], scene["hi"], scene["eps_lo"], scene["eps_hi"], device=gpu)
     rng = np.random.default_rng(5)
     picks = [tuple(int(rng.integers(0, s)) for s in scene["rho"].shape) for _ in range(32)]
+    picks = picks if os.environ.get("LSF_ADJOINT_FULL") else picks[:6]
     h = 1e-3
     fd, ad = [], []
     with torch.no_grad():
@@ -160,6 +160,6 @@ def test_full_scene_gradient_matches_float64_central_differences(gpu):
             print(f"voxel {q} {v}: adjoint {ad[-1]:.6e}  central difference {fd[-1]:.6e}", flush=True)
     fd, ad = np.array(fd), np.array(ad)
     err = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
-    print(f"32 voxels: relative L2 error {err:.3e}; max per-voxel relative error "
+    print(f"{len(picks)} voxels: relative L2 error {err:.3e}; max per-voxel relative error "
           f"{np.max(np.abs(ad - fd) / np.abs(fd)):.3e}")
     assert err < GRAD_TOL
