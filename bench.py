@@ -218,7 +218,7 @@ def adjoint_leg(dev, repeats: int = 2) -> dict:
         acc_e, _ = monitor_spectra(rho, prob)
         loss = workloads.mode_power(acc_e, scene["w"])
         loss.backward()
-        return rho.grad, float(loss)
+        return rho.grad, float(loss.detach())
 
     t_grad, (g, loss) = timed(grad)
     prob.use_checkpoints(True)
@@ -380,7 +380,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=320, help="grid size per axis (the workload is 320)")
     ap.add_argument("--cpu-steps", type=int, default=3, help="timed steps of the NumPy CPU oracle (0: skip the CPU legs)")
-    ap.add_argument("--cpu-c-steps", type=int, default=100, help="timed steps of the C (OpenMP) CPU oracle")
+    ap.add_argument("--cpu-c-steps", type=int, default=300,
+                    help="timed steps of the C (OpenMP) CPU oracle (about 10 s on 16 threads)")
     ap.add_argument("--tiling", default=None, help="rows_per_lane,waves,x_cells,z_chunk")
     ap.add_argument("--single-pass", choices=["auto", "on", "off"], default="auto",
                     help="step with the single-pass kernel (on), the H and E kernels (off), or the engine default")
