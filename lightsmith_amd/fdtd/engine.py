@@ -347,6 +347,7 @@ class Engine:
         handle = ctypes.c_void_p()
         check(self.lib.lsf_create(ctypes.byref(g), ctypes.byref(handle)), "lsf_create")
         self._h = handle
+        self.tiling = tuple(int(v) for v in self.tiling)  # this engine's, whatever the class default becomes
         ry, waves, xw, zc = self.tiling
         check(self.lib.lsf_set_tiling(self._h, ry, waves, xw, zc), "lsf_set_tiling")
 
