@@ -262,7 +262,8 @@ class TableSource(_Native):
     table row at its own delay with quintic interpolation (tables sampled every m steps, read index clamped
     to cmax, zero rows past the end) and adds coef * value to its node. ``points[kind]`` = (reference flat
     node index, base, i0, u0, coef); ``inc`` = (base, i0, u0) of the incident-wave view and ``T_inc`` its
-    tables (layered.py:469-482)."""
+    tables (layered.py:469-482). The view's buffer (``E_flat``) is what monitors sample; it is evaluated when
+    they sample (each chunk's end, ``Engine.sample``), not on every step."""
 
     def __init__(self, T, m: int, cmax: int, points: dict, inc=None, T_inc=None):
         self.T, self.m, self.cmax = np.asarray(T, dtype=float), int(m), int(cmax)
