@@ -479,3 +479,43 @@ def test_tilings_agree_with_poles(gpu, tiling):
         o.chunk()
         g.chunk()
     _compare(o, g, f"tiling {tiling} with 3-pole nodes, {steps} steps")
+
+
+def test_table_source_over_a_substrate(gpu):
+    """The table-fed plane wave with a layered background (dielectric substrate crossing the box, a wave coming up
+    from it at 40 degrees): the HIP engine fed with the tables against the FP64 oracle fed with the same ones, fields
+    after 900 steps (the pulse peak reaches the box around step 700) and the incident-wave view's spectrum."""
+    from lightsmith_amd.fdtd import layered
+    from lightsmith_amd.fdtd.media import Medium
+    from oracle import sources as osrc
+    shape, dx, eps_h, eps_s = (30, 28, 34), 4.0, 1.69, 2.25
+    dt = 0.95 * dx / np.sqrt(3) * np.sqrt(eps_h)
+    z_int = 16.0 * dx
+    z = np.arange(shape[2]) * dx
+    f_t, f_n = np.clip((z_int - (z - dx / 2)) / dx, 0, 1), np.clip((z_int - z) / dx, 0, 1)
+    inv = np.empty((3,) + shape)
+    inv[0] = inv[1] = 1 / (f_t * eps_s + (1 - f_t) * eps_h)
+    inv[2] = f_n / eps_s + (1 - f_n) / eps_h
+    th = np.deg2rad(40.0)
+    khat = np.array([np.sin(th), 0.0, np.cos(th)])
+    om = 2 * np.pi / np.array([280.0, 400.0])
+    src = layered.ObliqueLayeredPlaneWave(shape=shape, origin=np.zeros(3), spacing=dx, dt=dt, lo=(9, 8, 10),
+                                          hi=(21, 20, 24), khat=khat, ehat=(0.0, 1.0, 0.0), inv_eps=inv,
+                                          host=Medium(eps_h), z_int=z_int, sub=Medium(eps_s),
+                                          omega_c=om.mean(), omega_half=0.3 * om.mean(), max_steps=2000,
+                                          omega_min=om.min())
+    so = osrc.TableSource(src.T.reshape(-1), src.m, src.cmax, src.points, inc=src.inc,
+                          T_inc=np.asarray(src.T_inc).reshape(-1))
+    on, pn = cases.oracle_ns(), cases.product_ns()
+    mo = on.DFTMonitor(om, np.arange(3), source=so)
+    mp = pn.DFTMonitor(om, np.arange(3), source=src)
+    kw = dict(shape=shape, dx=dx, dt=dt, inv_eps=inv, dft_stride=5)
+    o = cases.make(on, pml=on.PMLSpec(cells=((5, 5),) * 3, eps_ref=eps_s), sources=[so], monitors=[mo], **kw)
+    g = cases.make(pn, gpu, pml=pn.PMLSpec(cells=((5, 5),) * 3, eps_ref=eps_s), sources=[src], monitors=[mp], **kw)
+    for _ in range(900 // 5):
+        o.chunk()
+        g.chunk()
+    _compare(o, g, "table-fed TFSF over a substrate")
+    eo, _ = o.monitors[0].results()
+    eg, _ = g.monitors[0].results()
+    assert cases.rel_l2(eg, eo) < DFT_TOL

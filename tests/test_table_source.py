@@ -17,16 +17,31 @@ DX, EPS = 4.0, 1.69
 DT = 0.95 * DX / np.sqrt(3) * np.sqrt(EPS)
 
 
-def _run(khat, ehat, steps=None, monitors=(), src=None):
+def _layer_inv_eps(z_int, eps_sub):
+    """1/eps of every E node over a substrate filling z < z_int (world z = k DX): the substrate fraction f of the
+    node's cell along z, 1/(f eps_s + (1 - f) eps_h) for E_x, E_y (cell [z - DX/2, z + DX/2)), f / eps_s +
+    (1 - f) / eps_h for E_z (at z + DX/2, cell [z, z + DX))."""
+    z = np.arange(SHAPE[2]) * DX
+    f_t = np.clip((z_int - (z - DX / 2)) / DX, 0, 1)
+    f_n = np.clip((z_int - z) / DX, 0, 1)
+    inv = np.empty((3,) + SHAPE)
+    inv[0] = inv[1] = (1 / (f_t * eps_sub + (1 - f_t) * EPS))[None, None, :]
+    inv[2] = (f_n / eps_sub + (1 - f_n) / EPS)[None, None, :]
+    return inv
+
+
+def _run(khat, ehat, steps=None, monitors=(), src=None, sub=None):
+    """sub: (z_int, eps) of a dielectric substrate filling z < z_int, or None."""
     host = Medium(EPS)
     wl = np.array([260.0, 320.0, 400.0])
     om = 2 * np.pi / wl
     omega_c, omega_half = 0.5 * (om.min() + om.max()), max(0.5 * (om.max() - om.min()), 0.15 * 0.5 * (om.min() + om.max()))
-    inv = np.full((3,) + SHAPE, 1 / EPS)
+    inv = np.full((3,) + SHAPE, 1 / EPS) if sub is None else _layer_inv_eps(*sub)
     if src is None:
+        extra = {} if sub is None else {"z_int": sub[0], "sub": Medium(sub[1])}
         src = layered.ObliqueLayeredPlaneWave(shape=SHAPE, origin=np.zeros(3), spacing=DX, dt=DT, lo=LO, hi=HI,
                                               khat=khat, ehat=ehat, inv_eps=inv, host=host, omega_c=omega_c,
-                                              omega_half=omega_half, max_steps=4000, omega_min=om.min())
+                                              omega_half=omega_half, max_steps=4000, omega_min=om.min(), **extra)
     ts = sources.TableSource(src.T.reshape(-1), src.m, src.cmax, src.points, inc=src.inc,
                                T_inc=np.asarray(src.T_inc).reshape(-1))
     eng = fdtd.Engine(shape=SHAPE, dx=DX, dt=DT, inv_eps=inv, pml=fdtd.PMLSpec(cells=((6, 6),) * 3, eps_ref=EPS),
@@ -39,10 +54,10 @@ def _inside(a):
     return tuple(slice(int(LO[d]) + 1, int(HI[d])) for d in range(3))
 
 
-def _leak(khat, ehat, src=None):
+def _leak(khat, ehat, src=None, sub=None):
     """Largest |E| in the scattered-field region (outside the box, 2 cells off it, clear of the CPML) relative to
     the largest |E| inside, over the whole pulse."""
-    eng, src, n = _run(khat, ehat, src=src)
+    eng, src, n = _run(khat, ehat, src=src, sub=sub)
     out_max = in_max = 0.0
     mask = np.ones(SHAPE, bool)
     mask[tuple(slice(int(LO[d]) - 2, int(HI[d]) + 3) for d in range(3))] = False
@@ -107,4 +122,62 @@ def test_leak_pin_fails_with_the_continuum_wavenumber(monkeypatch):
 
     monkeypatch.setattr(layered.ObliqueLayeredPlaneWave, "_wave", continuum)
     leak, _ = _leak(khat, ehat)
+    assert leak > 1e-6
+
+
+# a glass-like substrate (eps 2.25 under the host's 1.69) crossing the box: at z_int = 17.5 cells the E_z nodes of one
+# plane are half filled; at 17.0 the E_x, E_y nodes of one plane are
+@pytest.mark.parametrize("khat,ehat,z_cells", [
+    ((0.0, 0.0, -1.0), (1.0, 0.0, 0.0), 17.5),         # normal incidence from the host
+    ((0.35, -0.4, -0.847), (0.0, 0.904, -0.427), 17.0),  # oblique from the host
+    ((0.3, 0.25, 0.92), (0.95, 0.0, -0.31), 17.5),     # oblique from the substrate
+    ((0.9, 0.2, 0.3873), (0.0, 0.9, -0.46), 17.0),     # from the substrate beyond the critical angle (60 deg): TIR
+])
+def test_table_fed_box_over_a_substrate_does_not_leak(khat, ehat, z_cells):
+    """The layered background (incident, reflected and transmitted or evanescent waves) solves the grid's own
+    equations over the layer-meshed substrate, so the box stays leak-free as in a uniform host."""
+    khat = np.asarray(khat) / np.linalg.norm(khat)
+    ehat = np.asarray(ehat) - (np.asarray(ehat) @ khat) * khat
+    leak, src = _leak(khat, ehat, sub=(z_cells * DX, 2.25))
+    print(f"substrate, khat {np.round(khat, 3)}: leak {leak:.2e}")
+    assert src.sub is not None and leak < 1e-7
+
+
+def test_substrate_background_matches_fresnel_at_normal_incidence():
+    """Far from the interface the layered background at normal incidence is the incident wave plus the reflected
+    one, r = (n1 - n2) / (n1 + n2) up to the grid's dispersion (< 1e-3 at 25 cells per wavelength), and the
+    transmitted wave below carries t = 2 n1 / (n1 + n2)."""
+    inv = _layer_inv_eps(17.5 * DX, 2.25)
+    w = np.array([2 * np.pi / 400.0])
+    src = layered.ObliqueLayeredPlaneWave(shape=SHAPE, origin=np.zeros(3), spacing=DX, dt=DT, lo=LO, hi=HI,
+                                          khat=(0, 0, -1.0), ehat=(1.0, 0, 0), inv_eps=inv, host=Medium(EPS),
+                                          z_int=17.5 * DX, sub=Medium(2.25), omega_c=w[0], omega_half=0.3 * w[0],
+                                          max_steps=4000, omega_min=0.5 * w[0])
+    f = src._layered(w)
+    n1, n2 = np.sqrt(EPS), 1.5
+    ex = f["Ex"][0]
+    # above the interface: a e^{-i k z} + b e^{i k z}; fit a, b on two nodes near the top, r = b / a referred to z_int
+    k1 = n1 * w[0]
+    zt = src.z_nodes[-3:]
+    Mx = np.stack([np.exp(-1j * k1 * (zt - 17.5 * DX)), np.exp(1j * k1 * (zt - 17.5 * DX))], axis=1)
+    a, b = np.linalg.lstsq(Mx, ex[-3:], rcond=None)[0]
+    r = b / a
+    print(f"r {r:.5f}, Fresnel {(n1 - n2) / (n1 + n2):.5f}")
+    assert abs(r - (n1 - n2) / (n1 + n2)) < 2e-3
+
+
+def test_substrate_leak_pin_fails_with_the_averaging_rules_swapped(monkeypatch):
+    """Planted: the tables' E_x, E_y take <1/eps> and E_z 1/<eps> over the half-filled plane (the engine's material
+    keeps the right ones): the box leaks."""
+    orig = layered.ObliqueLayeredPlaneWave.layer_inv_eps
+
+    def swapped(self, w, z, normal):
+        return orig(self, w, z, not normal)
+
+    monkeypatch.setattr(layered.ObliqueLayeredPlaneWave, "layer_inv_eps", swapped)
+    khat = np.array([0.35, -0.4, -0.847]) / np.linalg.norm([0.35, -0.4, -0.847])
+    ehat = np.array([0.0, 0.904, -0.427])
+    ehat = ehat - (ehat @ khat) * khat
+    leak, _ = _leak(khat, ehat, sub=(17.0 * DX, 2.25))
+    print(f"swapped averaging: leak {leak:.2e}")
     assert leak > 1e-6
