@@ -1,5 +1,6 @@
 """The drop-in at lightsmith's plugin boundary: a ``Backend`` named ``lightsmith-fdtd`` (method ``fdtd``) that solves
-scattering problems on the HIP engine (backends/base.py:20-64 and backends/fdtd.py:55-158 of the reference).
+scattering problems, layer stacks and periodic structures on the HIP engine (backends/base.py:20-64 and
+backends/fdtd.py:55-158 of the reference).
 
 Installed next to lightsmith, the class subclasses the reference's own ``Backend`` and registers through the
 ``lightsmith.backends`` entry-point group; plugins load after the built-ins and registration keys on ``name``
@@ -7,9 +8,9 @@ Installed next to lightsmith, the class subclasses the reference's own ``Backend
 reaches it unchanged. Without lightsmith it subclasses the mirror below (same class attributes and classmethods)
 and ``lightsmith_amd.api.solve`` is the front door.
 
-The backend takes lightsmith's objects as they are (a reference ``Simulation`` is read through its JSON, the
-schema of ``lightsmith_amd.problem``) and returns a ``Result`` with the reference's fields (the reference's own
-class when it is importable). It runs on a GPU only: a CPU device is refused, and the HIP library must load.
+The backend takes lightsmith's objects as they are (a reference ``Simulation``, ``LayerStackProblem`` or
+``PeriodicProblem`` is read through its JSON, the schemas of ``lightsmith_amd.problem``) and returns a ``Result`` or a
+``SpectrumResult`` with the reference's fields (the reference's own classes when they are importable). It runs on a GPU only: a CPU device is refused, and the HIP library must load.
 """
 
 from __future__ import annotations
@@ -116,9 +117,16 @@ def _unsupported(sim) -> str:
     return "does not support " + ", ".join(missing) if missing else ""
 
 
-def _as_simulation(sim):
-    if getattr(sim, "kind", "scattering") != "scattering":
-        raise ValueError(f"lightsmith-fdtd (HIP) solves scattering problems; got {sim.kind!r}")
+KINDS = ("scattering", "periodic", "layer_stack")
+
+
+def _kind(sim) -> str:
+    return sim.get("kind", "scattering") if isinstance(sim, dict) else getattr(sim, "kind", "scattering")
+
+
+def _as_problem(sim):
+    if _kind(sim) not in KINDS:
+        raise ValueError(f"lightsmith-fdtd (HIP) solves {', '.join(KINDS)} problems; got {_kind(sim)!r}")
     return P.parse_problem(sim)
 
 
@@ -130,8 +138,17 @@ def _result(**kw):
         return P.Result.from_arrays(**kw)
 
 
+def _spectrum(res: P.SpectrumResult):
+    try:
+        from lightsmith.results import SpectrumResult as RefSpectrum  # type: ignore
+        return RefSpectrum.model_validate(res.model_dump(mode="json"))
+    except Exception:  # noqa: BLE001 - lightsmith absent
+        return res
+
+
 class _NativeFDTD:
-    """lightsmith's FDTD backend on MI355X (fdtd.py:55-158): scattering problems, float32 fields on the GPU."""
+    """lightsmith's FDTD backend on MI355X (fdtd.py:55-158): scattering problems, layer stacks and periodic
+    structures, float32 fields on the GPU."""
 
     name = "lightsmith-fdtd"
     method = "fdtd"
@@ -140,7 +157,7 @@ class _NativeFDTD:
                 "Oskooi et al., CPC 181, 687 (2010)")
     gpu = True
     differentiable = False
-    problem_kinds = ("scattering",)
+    problem_kinds = KINDS
     options_model = P.FDTDOptions
 
     @classmethod
@@ -154,12 +171,22 @@ class _NativeFDTD:
 
     @classmethod
     def supports(cls, sim) -> tuple[bool, str]:
-        if getattr(sim, "kind", "scattering") != "scattering":
-            return False, "lightsmith-fdtd (HIP) solves scattering problems"
+        if _kind(sim) not in KINDS:
+            return False, "lightsmith-fdtd (HIP) solves scattering, periodic and layer_stack problems"
         try:
-            s = _as_simulation(sim)
+            s = _as_problem(sim)
         except Exception as e:  # noqa: BLE001 - a reference feature the mirror schema lacks
             return False, f"problem not on the HIP path: {e}"
+        if s.kind != "scattering":
+            from .fdtd import spectra
+            try:
+                cls._options(s)
+                ok, why = spectra.supports(s)
+                if ok:
+                    spectra.MediaTable(s, s.wavelength_array(), [])
+            except ValueError as e:
+                return False, str(e)
+            return ok, why
         why = _unsupported(s)
         if why:
             return False, why
@@ -181,9 +208,28 @@ class _NativeFDTD:
         return opts
 
     @classmethod
+    def _estimate_spectrum(cls, s) -> dict[str, Any]:
+        from .fdtd import spectra
+        from .fdtd.run import FS
+        opts = cls._options(s)
+        p = spectra.plan(s, opts)
+        c = p["cell"]
+        out = {"grid": list(c.shape), "real_cells": list(c.n_cells) + [c.shape[2]], "cells": c.cells,
+               "spacing_nm": list(c.spacing), "dt_fs": p["dt"] / FS, "steps_max": p["max_steps"], "runs": p["runs"],
+               "dft_stride": p["dft_stride"], "precision": "single", "memory_bytes": p["memory_bytes"],
+               "fits": c.cells <= opts.max_cells, "warnings": p["warnings"]}
+        if p["oblique_runs"]:
+            out["oblique_runs"] = p["oblique_runs"]
+            out["notes"] = ("oblique incidence: one fixed-k_par complex-field run per (angle, wavelength, "
+                            "polarisation), all stepped together as one batch")
+        return out
+
+    @classmethod
     def estimate(cls, sim, device) -> dict[str, Any]:
         from .fdtd import scattering
-        s = _as_simulation(sim)
+        s = _as_problem(sim)
+        if s.kind != "scattering":
+            return cls._estimate_spectrum(s)
         opts = cls._options(s)
         p = scattering.plan(s, opts)
         lay = p["layout"]
@@ -198,10 +244,13 @@ class _NativeFDTD:
         if not ok:
             raise ValueError(f"{cls.name} {why}")
         from .fdtd import scattering
-        s = _as_simulation(sim)
+        s = _as_problem(sim)
         name = _device_name(device)
         if not name.startswith("cuda"):
             raise ValueError(f"{cls.name} runs on a GPU (device 'gpu' or 'cuda:N'), got {name!r}")
+        if s.kind != "scattering":
+            from .fdtd import spectra
+            return _spectrum(spectra.solve(s, cls._options(s), name))
         out = scattering.solve(s, cls._options(s), name)
         res = _result(method=cls.method, backend=cls.name, wavelengths_nm=s.wavelength_array(), c_ext=out["c_ext"],
                       c_sca=out["c_sca"], c_abs=out["c_abs"], r_eq=s.equivalent_radius_nm(),
@@ -214,7 +263,8 @@ _Base = _RefBackend if _RefBackend is not None else Backend
 
 @register
 class NativeFDTD(_NativeFDTD, _Base):
-    """lightsmith's FDTD backend on MI355X (fdtd.py:55-158): scattering problems, float32 fields on the GPU."""
+    """lightsmith's FDTD backend on MI355X (fdtd.py:55-158): scattering problems, layer stacks and periodic
+    structures, float32 fields on the GPU."""
 
 
 __all__ = ["Backend", "NativeFDTD", "get_backend", "register"]

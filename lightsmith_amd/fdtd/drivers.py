@@ -11,17 +11,17 @@ monitors on the host, with the reference's formulas:
   incident intensity; C_ext = C_sca + C_abs (scattering.py:428-464).
 * ``solve_periodic``: one Bloch-periodic cell, normal incidence from a 1D-fed TFSF plane or oblique incidence at
   fixed k_par from a Bloch plane source; reflection and transmission per order from the FFT of the plane monitors
-  over the cell (periodic.py:347-363, 392-459).
+  over the cell (periodic.py:347-363, 392-459). ``periodic_run`` builds such a run without stepping it and
+  ``periodic_finish`` reads it out, so that ``spectra.solve`` can step many as one batch.
 * ``solve_emission``: a point dipole; the power it gives to the field, -1/2 dx^3 Re(E_src . J*), and the power
   through a closed box around it (emission.py:153-196).
 
-Scope of this module against the reference's drivers: a uniform grid and a homogeneous, lossless host; particles
-are given as voxelised permittivity (``staircase`` places spheres on the grid the way the reference's
-mesh_refinement="staircase" does: the medium at each E node's position). The plane wave of ``solve_scattering``
-comes from the 1D-fed TFSF source (tfsf.py:108-211), which the reference uses for periodic normal incidence; its
-scattering driver feeds the box from precomputed tables of the exact discrete background instead (layered.py), whose
-construction is not part of this package. Substrates, beams, near/far-field outputs, mirror symmetry and graded
-grids are not driven here.
+Scope of this module: a uniform grid and a homogeneous, lossless host; particles are given as voxelised permittivity
+(``staircase`` places spheres on the grid the way the reference's mesh_refinement="staircase" does: the medium at
+each E node's position). The plane wave of ``solve_scattering`` comes from the 1D-fed TFSF source
+(tfsf.py:108-211), which the reference uses for periodic normal incidence. The problem-level solves with the
+reference's schemas sit above these drivers: ``scattering.solve`` (table-fed TFSF from the exact discrete
+background, ``layered.py``) and ``spectra.solve`` (layer stacks and periodic structures).
 """
 
 from __future__ import annotations
@@ -239,6 +239,8 @@ class PeriodicCell:
     k_top: int
     eps_sup: float = 1.0
     eps_sub: float = 1.0
+    offdiag: object = None  # engine.OffDiagonal of conformal dielectric interfaces, or None
+    origin: np.ndarray | None = None  # nm; None: the grid's first node at 0
 
     @property
     def shape(self) -> tuple:
@@ -257,7 +259,7 @@ def periodic_time_step(cell: PeriodicCell, courant: float, media, kpar=(0.0, 0.0
         elif kpar[a]:
             inv2 += (math.sin(kpar[a] * d[a] / 2) / d[a]) ** 2
     dt0 = courant / math.sqrt(inv2)
-    scale = math.sqrt(min(m.eps_inf for m in media))
+    scale = math.sqrt(min([1.0] + [m.eps_inf for m in media]))  # periodic.py:479 (no faster than in vacuum)
     return limit_dt(dt0 * scale, courant, media, 4 * (courant / dt0) ** 2)
 
 
@@ -271,81 +273,141 @@ class PeriodicResult:
     warnings: list = field(default_factory=list)
 
 
-def solve_periodic(device, cell: PeriodicCell, wavelengths, *, pol: str = "s", kpar=(0.0, 0.0), phi_deg: float = 0.0,
-                   opts: DriverOptions | None = None) -> PeriodicResult:
-    """Reflection and transmission per diffraction order (periodic.py:392-459). kpar = 0: one broadband real
-    run fed by a 1D TFSF plane; otherwise one complex run at the first wavelength with a Bloch plane source."""
+@dataclass
+class PeriodicRun:
+    """One run of a periodic cell, built and not yet stepped: its engine, monitors and run-control limits."""
+
+    eng: Engine
+    src: object
+    mon_r: DFTMonitor
+    mon_t: DFTMonitor
+    mon_i: DFTMonitor
+    pol: str
+    kpar: tuple
+    oblique: bool
+    stride: int
+    max_steps: int
+    min_steps: int
+    window: int
+    area: float
+    n_cells: tuple
+    spacing: tuple
+    dt: float
+
+
+def periodic_plan(cell: PeriodicCell, wavelengths, kpar=(0.0, 0.0), opts: DriverOptions | None = None,
+                  media_extra=()) -> dict:
+    """dt, DFT stride and pulse band of one run (periodic.py:275-331): normal incidence one broadband pulse over the
+    band, oblique incidence a pulse aimed at the first wavelength and kept clear of the grazing frequency."""
     opts = opts or DriverOptions()
-    warnings: list[str] = []
+    wl = np.atleast_1d(np.asarray(wavelengths, dtype=float))
+    oblique = any(abs(k) > 0 for k in kpar)
+    media = [Medium(cell.eps_sup), Medium(cell.eps_sub)] + [m for g in cell.groups for m in g.media] + list(media_extra)
+    dt = periodic_time_step(cell, opts.courant, media, kpar if oblique else (0.0, 0.0))
+    if not oblique:
+        omega_c, omega_half, omega_max = band(wl)
+        stride = max(1, int(2 * np.pi / (omega_c + 4 * omega_half + omega_max) / dt))
+    else:
+        omega_c = 2 * np.pi / wl[0]
+        k = math.hypot(*kpar)
+        gaps = [abs(omega_c - k / math.sqrt(e)) for e in (cell.eps_sup, cell.eps_sub) if e > 0]
+        omega_half = float(np.clip(min(gaps) / 4, 0.01 * omega_c, 0.15 * omega_c))  # periodic.py:324-331
+        stride = max(1, int(2 * np.pi / (2 * omega_c + 4 * omega_half) / dt))
+    return {"dt": dt, "stride": opts.dft_stride or stride, "omega_c": omega_c, "omega_half": omega_half,
+            "media": media}
+
+
+def periodic_run(device, cell: PeriodicCell, wavelengths, *, pol: str = "s", kpar=(0.0, 0.0), phi_deg: float = 0.0,
+                 opts: DriverOptions | None = None, stride: int | None = None, media_extra=()) -> PeriodicRun:
+    """The engine of one run (periodic.py:392-459): kpar = 0 one broadband real run fed by a 1D TFSF plane;
+    otherwise one complex run at the first wavelength with a Bloch plane source. ``stride`` overrides the run's
+    own DFT stride (a batch of runs shares one; a smaller stride than the run's own is always valid)."""
+    opts = opts or DriverOptions()
     wl = np.atleast_1d(np.asarray(wavelengths, dtype=float))
     d = tuple(float(v) for v in cell.spacing)
     shape = cell.shape
     lam_max = float(wl.max())
     oblique = any(abs(k) > 0 for k in kpar)
     pml = PMLSpec(cells=((0, 0), (0, 0), (cell.pml, cell.pml)), kappa_max=opts.pml_kappa,
-                  alpha_max=0.05 * 2 * np.pi / lam_max, eps_ref=cell.eps_sup, eps_ref_high=cell.eps_sub)
-    media = [Medium(cell.eps_sup), Medium(cell.eps_sub)] + [m for g in cell.groups for m in g.media]
-    dt = periodic_time_step(cell, opts.courant, media, kpar if oblique else (0.0, 0.0))
-    transit = shape[2] * d[2] * max(math.sqrt(cell.eps_sup), math.sqrt(cell.eps_sub))
+                  alpha_max=0.05 * 2 * np.pi * math.sqrt(cell.eps_sup) / lam_max, eps_ref=cell.eps_sup,
+                  eps_ref_high=cell.eps_sub)
+    p = periodic_plan(cell, wl, kpar, opts, media_extra)
+    dt, stride = p["dt"], stride or p["stride"]
+    media = p["media"]
+    transit = shape[2] * d[2] * max(math.sqrt(abs(m.eps_inf)) for m in media)
+    max_steps = int(math.ceil((opts.run_time_fs or opts.max_time_fs) * FS / dt / stride)) * stride
+    origin = np.zeros(3) if cell.origin is None else np.asarray(cell.origin, dtype=float)
     if not oblique:
         omega = 2 * np.pi / wl
-        omega_c, omega_half, omega_max = band(wl)
-        stride = opts.dft_stride or max(1, int(2 * np.pi / (omega_c + 4 * omega_half + omega_max) / dt))
-        max_steps = int(math.ceil((opts.run_time_fs or opts.max_time_fs) * FS / dt / stride)) * stride
         phi = math.radians(phi_deg)
         ehat = np.array([-np.sin(phi), np.cos(phi), 0.0]) if pol == "s" else np.array([np.cos(phi), np.sin(phi), 0.0])
-        origin = np.zeros(3)
         r0 = origin + np.array([1, 1, cell.k_top]) * np.asarray(d)
         big = 1 << 30
         src = TFSFPlaneWave(shape=shape, origin=origin, dx=d, dt=dt, r0=r0, lo=np.array([-big, -big, cell.k_src]),
                             hi=np.array([big, big, big]), khat=np.array([0.0, 0.0, 1.0]), ehat=ehat,
-                            eps_b=cell.eps_sup, omega_c=omega_c, omega_half=omega_half,
+                            eps_b=cell.eps_sup, omega_c=p["omega_c"], omega_half=p["omega_half"],
                             max_steps=max_steps + 4 * stride, periodic=(True, True, False))
         probe = src.probe()
         bloch = (0.0, 0.0, 0.0)
     else:
         omega = np.array([2 * np.pi / wl[0]])
         k = math.hypot(*kpar)
-        gaps = [abs(omega[0] - k / math.sqrt(e)) for e in (cell.eps_sup, cell.eps_sub) if e > 0]
-        half = float(np.clip(min(gaps) / 4, 0.01 * omega[0], 0.15 * omega[0]))  # periodic.py:324-331
-        stride = opts.dft_stride or max(1, int(2 * np.pi / (2 * omega[0] + 4 * half) / dt))
-        max_steps = int(math.ceil((opts.run_time_fs or opts.max_time_fs) * FS / dt / stride)) * stride
+        # the column is all superstrate: its CPML is graded for that medium at both ends
         col_pml = PMLSpec(cells=pml.cells, kappa_max=pml.kappa_max, alpha_max=pml.alpha_max, eps_ref=cell.eps_sup)
         src = BlochPlaneSource(shape=shape, spacing=d, dt=dt, k_src=cell.k_src, k_launch=cell.k_src - 1, kpar=kpar,
-                               pol=pol, eps_b=cell.eps_sup, pml=col_pml, omega_c=float(omega[0]), omega_half=half,
-                               max_steps=max_steps + 4 * stride)
+                               pol=pol, eps_b=cell.eps_sup, pml=col_pml, omega_c=float(omega[0]),
+                               omega_half=p["omega_half"], max_steps=max_steps + 4 * stride)
         probe = src.probe(cell.k_top)
         bloch = (kpar[0], kpar[1], 0.0)
-        cos_t = math.sqrt(max(1 - k ** 2 / (cell.eps_sup * omega[0] ** 2), 0.01))
+        cos_t = math.sqrt(max(1 - k ** 2 / (cell.eps_sup * omega[0] ** 2), 0.01))  # slower transit along z
         transit = transit / cos_t
     mon_r = DFTMonitor(omega, *plane_points(shape, cell.n_cells, cell.k_r), energy=1)
     mon_t = DFTMonitor(omega, *plane_points(shape, cell.n_cells, cell.k_t), energy=1)
     mon_i = DFTMonitor(omega, *probe, source=src)
     eng = Engine(device, shape=shape, dx=d, dt=dt, single=True, inv_eps=cell.inv_eps, groups=cell.groups, pml=pml,
                  sources=[src], monitors=[mon_r, mon_t, mon_i], dft_stride=stride, periodic=(True, True, False),
-                 bloch=bloch)
+                 bloch=bloch, offdiag=cell.offdiag)
+    return PeriodicRun(eng=eng, src=src, mon_r=mon_r, mon_t=mon_t, mon_i=mon_i, pol=pol,
+                       kpar=tuple(kpar) if oblique else (0.0, 0.0), oblique=oblique, stride=stride,
+                       max_steps=max_steps, min_steps=int((src.t_passed + transit) / dt),
+                       window=math.ceil((lam_max if not oblique else float(wl[0])) / (stride * dt)),
+                       area=cell.n_cells[0] * d[0] * cell.n_cells[1] * d[1], n_cells=tuple(cell.n_cells), spacing=d,
+                       dt=dt)
+
+
+def periodic_finish(run: PeriodicRun) -> tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+    """(r, t per order (n_f, nx, ny), mx, my) of a stepped run: plane fluxes split into orders by the FFT over the
+    cell (periodic.py:347-363), over the incident power through the cell (periodic.py:421-422, 455-456)."""
+    e_i, h_i = run.mon_i.results()
+    er, hr = run.mon_r.results()
+    et, ht = run.mon_t.results()
+    if run.oblique:
+        inc = 0.5 * np.real(e_i[:, 0] * np.conj(h_i[:, 0]) - e_i[:, 1] * np.conj(h_i[:, 1])) * run.area
+    else:
+        inc = 0.5 * np.real(e_i[:, 0] * np.conj(h_i[:, 0])) * run.area
+    fr, mx, my = orders(er, hr, run.n_cells, run.spacing, run.kpar)
+    ft, _, _ = orders(et, ht, run.n_cells, run.spacing, run.kpar)
+    return -fr / inc[:, None, None], ft / inc[:, None, None], mx, my
+
+
+def solve_periodic(device, cell: PeriodicCell, wavelengths, *, pol: str = "s", kpar=(0.0, 0.0), phi_deg: float = 0.0,
+                   opts: DriverOptions | None = None) -> PeriodicResult:
+    """Reflection and transmission per diffraction order of one run (periodic.py:392-459), stepped alone under
+    ``run_until_decayed``."""
+    opts = opts or DriverOptions()
+    warnings: list[str] = []
+    run = periodic_run(device, cell, wavelengths, pol=pol, kpar=kpar, phi_deg=phi_deg, opts=opts)
     t_loop = time.perf_counter()
     try:
-        steps, level, graphs = run_until_decayed(eng, opts, max_steps=max_steps,
-                                                 min_steps=int((src.t_passed + transit) / dt),
-                                                 window=math.ceil(lam_max / (stride * dt)), warnings=warnings)
+        steps, level, graphs = run_until_decayed(run.eng, opts, max_steps=run.max_steps, min_steps=run.min_steps,
+                                                 window=run.window, warnings=warnings)
         t_loop = time.perf_counter() - t_loop
-        e_i, h_i = mon_i.results()
-        er, hr = mon_r.results()
-        et, ht = mon_t.results()
+        r, t, mx, my = periodic_finish(run)
     finally:
-        eng.close()
-    area = cell.n_cells[0] * d[0] * cell.n_cells[1] * d[1]
-    if oblique:
-        inc = 0.5 * np.real(e_i[:, 0] * np.conj(h_i[:, 0]) - e_i[:, 1] * np.conj(h_i[:, 1])) * area
-    else:
-        inc = 0.5 * np.real(e_i[:, 0] * np.conj(h_i[:, 0])) * area
-    kp = tuple(kpar) if oblique else (0.0, 0.0)
-    fr, mx, my = orders(er, hr, cell.n_cells, d, kp)
-    ft, _, _ = orders(et, ht, cell.n_cells, d, kp)
-    prov = {"polarization": pol, "kpar": list(kp), "steps": steps, "decay_level": level, "hip_graphs": graphs,
-            "dft_stride": stride, "dt_fs": dt / FS, "time_loop_s": t_loop}
-    return PeriodicResult(-fr / inc[:, None, None], ft / inc[:, None, None], mx, my, prov, warnings)
+        run.eng.close()
+    prov = {"polarization": pol, "kpar": list(run.kpar), "steps": steps, "decay_level": level, "hip_graphs": graphs,
+            "dft_stride": run.stride, "dt_fs": run.dt / FS, "time_loop_s": t_loop}
+    return PeriodicResult(r, t, mx, my, prov, warnings)
 
 
 # --- emission --------------------------------------------------------------------------------------------------

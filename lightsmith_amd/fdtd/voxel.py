@@ -112,7 +112,7 @@ def _pixel_stats(slot_fn, ctr: torch.Tensor, size: torch.Tensor, S: int, samples
 # --- voxelisation -----------------------------------------------------------------------------------------------
 
 def voxelise(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_hi, *, mode: str = "conformal",
-             samples: int = 8, device="cpu"):
+             samples: int = 8, device="cpu", periodic=(False, False, False)):
     """(inv_eps [3, *shape] float64 with 1 at D-form nodes, groups, stats, OffDiagonal or None) for the media
     slot_fn assigns (0 = the host, which must be non-dispersive); only nodes in [roi_lo, roi_hi) are examined
     (setup.py:110-273; uniform grid, no mirror planes or layered background)."""
@@ -218,7 +218,8 @@ def voxelise(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_hi, *,
         offdiag = OffDiagonal(np.concatenate(od_idx3), np.concatenate(od_comp), np.concatenate(od_coef))
     tm = None
     if mode == "conformal" and disp.any():
-        tm = vertex_tensor(slot_fn, media, shape, origin, dx3, roi_lo, roi_hi, samples=samples, device=dev)
+        tm = vertex_tensor(slot_fn, media, shape, origin, dx3, roi_lo, roi_hi, samples=samples, device=dev,
+                           periodic=periodic)
     if tm is not None:
         taken = tm.idx
         inv_flat[taken] = 1.0
@@ -240,21 +241,36 @@ def voxelise(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_hi, *,
 
 # --- conformal vertex tensor --------------------------------------------------------------------------------------
 
+def _wrap(x: np.ndarray, shape, periodic) -> tuple[np.ndarray, np.ndarray]:
+    """(real index, period count) of grid indices (n, 3) that may lie on or past a periodic axis' ghost planes:
+    along a periodic axis the real indices are 1 .. n - 2, and index i is real index 1 + (i - 1) mod (n - 2)
+    shifted by floor((i - 1) / (n - 2)) periods (setup.py:292-303)."""
+    x = np.array(x, dtype=np.int64)
+    shift = np.zeros_like(x)
+    for a in range(3):
+        if periodic[a]:
+            shift[:, a], r = np.divmod(x[:, a] - 1, shape[a] - 2)
+            x[:, a] = 1 + r
+    return x, shift
+
+
 def vertex_tensor(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_hi, *, samples: int = 8,
-                  device="cpu") -> TensorMesh | None:
-    """The TensorMesh of interfaces with a dispersive medium, or None (setup.py:325-420; uniform grid, no periodic
-    axes or mirror planes). Tensor vertices: every vertex whose cube (side dx, centred on it) holds an interface
-    with a dispersive medium, grown over the mixed vertices connected to them through shared nodes, so that each
-    node next to a tensor vertex has at its other end another tensor vertex or a single medium. Each node's two
-    slots are its vertices at the node index (slot 0) and at index + e_c (slot 1); a slot without a tensor vertex
-    takes the medium at that vertex."""
+                  device="cpu", periodic=(False, False, False)) -> TensorMesh | None:
+    """The TensorMesh of interfaces with a dispersive medium, or None (setup.py:325-420; uniform grid, no mirror
+    planes). Tensor vertices: every vertex whose cube (side dx, centred on it) holds an interface with a dispersive
+    medium, grown over the mixed vertices connected to them through shared nodes, so that each node next to a
+    tensor vertex has at its other end another tensor vertex or a single medium. Each node's two slots are its
+    vertices at the node index (slot 0) and at index + e_c (slot 1); a slot without a tensor vertex takes the
+    medium at that vertex. Along a periodic axis every real vertex is examined and indices past the cell wrap to
+    their real node, the period count kept in ``shift`` (the engine applies the Bloch phase of it)."""
     dev = torch.device(device)
     shape = tuple(int(v) for v in shape)
     S = len(media)
     disp = np.array([m.dispersive for m in media])
     dx3 = np.broadcast_to(np.asarray(dx, dtype=float), (3,)).copy()
     origin = np.asarray(origin, dtype=float)
-    rng = [np.arange(max(int(roi_lo[a]), 1), min(int(roi_hi[a]), n - 2) + 1) for a, n in enumerate(shape)]
+    rng = [np.arange(1, n - 1) if periodic[a] else np.arange(max(int(roi_lo[a]), 1), min(int(roi_hi[a]), n - 2) + 1)
+           for a, n in enumerate(shape)]
     V = np.stack(np.meshgrid(*rng, indexing="ij"), axis=-1).reshape(-1, 3)
     pos = torch.as_tensor(origin + V * dx3, device=dev)
     size = torch.as_tensor(dx3, device=dev)
@@ -283,7 +299,7 @@ def vertex_tensor(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_h
     inT = seed.copy()
     front = np.nonzero(seed)[0]
     while len(front):
-        nb = np.concatenate([Vm[front] + s * eye[a] for a in range(3) for s in (-1, 1)])
+        nb = _wrap(np.concatenate([Vm[front] + s * eye[a] for a in range(3) for s in (-1, 1)]), shape, periodic)[0]
         k = key(nb)
         j = np.clip(np.searchsorted(ks, k), 0, len(ks) - 1)
         j = np.unique(order[j][ks[j] == k])
@@ -294,7 +310,7 @@ def vertex_tensor(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_h
     # a vertex touches the E_d nodes at T - e_d (its minus side) and T (its plus side)
     raw = np.stack([np.stack([T - eye[d], T], axis=1) for d in range(3)], axis=1)  # (nV, 3, 2, 3)
     comp = np.broadcast_to(np.arange(3)[None, :, None], (nV, 3, 2)).reshape(-1)
-    real = raw.reshape(-1, 3)
+    real, shift = _wrap(raw.reshape(-1, 3), shape, periodic)
     nkey = comp * int(np.prod(shape)) + key(real)
     _, first, inv = np.unique(nkey, return_index=True, return_inverse=True)
     node_comp, node_idx3 = comp[first], real[first]
@@ -305,7 +321,7 @@ def vertex_tensor(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_h
     slot_vert = np.full((nN, 2), -1, dtype=np.int64)
     slot_medium = np.zeros((nN, 2), dtype=np.int64)
     for k in range(2):
-        vx = node_idx3 + k * eye[node_comp]
+        vx = _wrap(node_idx3 + k * eye[node_comp], shape, periodic)[0]
         kk = key(vx)
         j = np.clip(np.searchsorted(tks, kk), 0, len(tks) - 1)
         hit = tks[j] == kk
@@ -314,7 +330,7 @@ def vertex_tensor(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_h
             p = torch.as_tensor(origin + vx[~hit] * dx3, device=dev)
             slot_medium[~hit, k] = _classify(slot_fn, p).cpu().numpy()
     return TensorMesh(media=list(media), frac=frac[:, inT], normal=nhat[inT], node_comp=node_comp,
-                      node_idx3=node_idx3, vert_nodes=inv.reshape(nV, 3, 2), shift=np.zeros((nV, 3, 2, 3), np.int64),
+                      node_idx3=node_idx3, vert_nodes=inv.reshape(nV, 3, 2), shift=shift.reshape(nV, 3, 2, 3),
                       slot_vert=slot_vert, slot_medium=slot_medium, shape=shape, vert_idx3=T)
 
 

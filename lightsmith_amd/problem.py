@@ -1,7 +1,8 @@
 """Problem and result types of the FDTD front door: the subset of lightsmith's typed specification that plane-wave
-scattering by particles uses (spec.py, materials.py, results.py, backends/_fdtd/options.py of the reference), with
-the same field names, defaults and validation, so that a reference ``Simulation`` serialised to JSON parses here
-and a ``Result`` produced here has the reference's fields.
+scattering by particles, layer stacks and periodic structures use (spec.py, problems.py, materials.py, results.py,
+backends/_fdtd/options.py of the reference), with the same field names, defaults and validation, so that a
+reference problem serialised to JSON parses here and a ``Result`` / ``SpectrumResult`` produced here has the
+reference's fields.
 
 Units: nm (vacuum wavelengths), exp(-i omega t), cross sections in nm^2, energies in eV.
 """
@@ -234,6 +235,102 @@ class Simulation(_Model):
         return (3 * v / (4 * np.pi)) ** (1 / 3)
 
 
+Wavelengths = Union[list[float], WavelengthRange]
+
+
+def wavelength_array(w) -> np.ndarray:
+    """Vacuum wavelengths (nm) of a list or a WavelengthRange (problems.py:34-35)."""
+    return w.values() if isinstance(w, WavelengthRange) else np.asarray(w, dtype=float)
+
+
+# --- 2D profiles, layer stacks and periodic structures (problems.py:41-160) ------------------------------------------
+
+class Rectangle(_Model):
+    kind: Literal["rectangle"] = "rectangle"
+    center_nm: tuple[float, float] = (0.0, 0.0)
+    size_nm: tuple[float, float]
+
+
+class Circle(_Model):
+    kind: Literal["circle"] = "circle"
+    center_nm: tuple[float, float] = (0.0, 0.0)
+    radius_nm: float = Field(gt=0)
+
+
+class Ellipse(_Model):
+    kind: Literal["ellipse"] = "ellipse"
+    center_nm: tuple[float, float] = (0.0, 0.0)
+    semi_axes_nm: tuple[float, float]
+    angle_deg: float = 0.0
+
+
+class Polygon(_Model):
+    kind: Literal["polygon"] = "polygon"
+    vertices_nm: list[tuple[float, float]] = Field(min_length=3)
+
+
+Profile = Annotated[Union[Rectangle, Circle, Ellipse, Polygon], Field(discriminator="kind")]
+
+
+class Structure2D(_Model):
+    name: str | None = None
+    profile: Profile
+    material: Material
+
+
+class Layer(_Model):
+    thickness_nm: float = Field(ge=0)
+    material: Material
+    coherent: bool = True
+
+
+class LayerStackProblem(_Model):
+    """Planar multilayer: reflection, transmission and absorption against wavelength and angle. Layers are listed
+    from the superstrate side; light comes in from the superstrate and travels towards the substrate (+z)."""
+
+    kind: Literal["layer_stack"] = "layer_stack"
+    superstrate: Material
+    layers: list[Layer] = Field(default_factory=list)
+    substrate: Material
+    wavelengths: Wavelengths
+    angles_deg: list[float] = [0.0]
+    polarizations: list[Literal["s", "p"]] = ["s", "p"]
+    options: dict[str, dict[str, Any]] = Field(default_factory=dict)
+
+    def wavelength_array(self) -> np.ndarray:
+        return wavelength_array(self.wavelengths)
+
+
+class PatternedLayer(_Model):
+    thickness_nm: float = Field(gt=0)
+    background: Material
+    shapes: list[Structure2D] = Field(default_factory=list)
+
+
+class Incidence(_Model):
+    theta_deg: float = 0.0
+    phi_deg: float = 0.0
+    polarizations: list[Literal["s", "p"]] = ["s", "p"]
+
+
+class PeriodicProblem(_Model):
+    """A 1D or 2D periodic stack of patterned layers under plane-wave illumination (lattice_a2_nm None: a 1D
+    grating, invariant along y)."""
+
+    kind: Literal["periodic"] = "periodic"
+    lattice_a1_nm: tuple[float, float]
+    lattice_a2_nm: tuple[float, float] | None = None
+    superstrate: Material
+    layers: list[PatternedLayer] = Field(min_length=1)
+    substrate: Material
+    wavelengths: Wavelengths
+    incidence: Incidence = Incidence()
+    options: dict[str, dict[str, Any]] = Field(default_factory=dict)
+
+    def wavelength_array(self) -> np.ndarray:
+        return wavelength_array(self.wavelengths)
+
+
 def options_for(problem, backend_name: str, method: str) -> dict[str, Any]:
     """Method-level options overlaid with backend-specific ones (problems.py:348-350)."""
     opts = getattr(problem, "options", {}) or {}
@@ -324,17 +421,67 @@ class Result(BaseModel):
         return self._q(self.c_abs_nm2)
 
 
-def parse_problem(obj) -> Simulation:
-    """A Simulation from a model (ours, or the reference's, through its JSON), a dict or a JSON string."""
-    if isinstance(obj, Simulation):
+class DiffractionOrder(BaseModel):
+    order: tuple[int, int]
+    r: dict[str, list[list[float]]]  # reflected power fraction per polarisation, [angle][wavelength]
+    t: dict[str, list[list[float]]]
+
+
+class SpectrumResult(BaseModel):
+    """Power reflection, transmission and absorption of layer stacks and periodic structures (results.py:126-175)."""
+
+    model_config = ConfigDict(extra="forbid")
+
+    kind: Literal["spectrum"] = "spectrum"
+    method: str
+    backend: str
+    wavelengths_nm: list[float]
+    angles_deg: list[float]
+    polarizations: list[str]
+    r: dict[str, list[list[float]]]  # total reflectance per polarisation, [angle][wavelength]
+    t: dict[str, list[list[float]]]
+    a: dict[str, list[list[float]]]
+    orders: list[DiffractionOrder] | None = None
+    r_amplitude: dict[str, list[list[tuple[float, float]]]] | None = None
+    t_amplitude: dict[str, list[list[tuple[float, float]]]] | None = None
+    provenance: dict[str, Any] = Field(default_factory=dict)
+    warnings: list[str] = Field(default_factory=list)
+
+    def difference(self, other) -> dict[str, dict[str, float]]:
+        """Max and RMS absolute difference of r, t, a per polarisation (results.py:176-187; same grids)."""
+        if not (np.allclose(self.wavelengths_nm, other.wavelengths_nm)
+                and np.allclose(self.angles_deg, other.angles_deg)):
+            raise ValueError("spectra are on different wavelength/angle grids")
+        out: dict[str, dict[str, float]] = {}
+        for q in ("r", "t", "a"):
+            mine, theirs = getattr(self, q), getattr(other, q)
+            for pol in sorted(set(mine) & set(theirs)):
+                d = np.abs(np.asarray(mine[pol]) - np.asarray(theirs[pol]))
+                out[f"{q}_{pol}"] = {"max": float(d.max()), "rms": float(np.sqrt(np.mean(d ** 2)))}
+        return out
+
+
+_KINDS = {"scattering": Simulation, "layer_stack": LayerStackProblem, "periodic": PeriodicProblem}
+
+
+def parse_problem(obj):
+    """The problem (Simulation, LayerStackProblem or PeriodicProblem, by its ``kind``) from a model (ours, or the
+    reference's, through its JSON), a dict or a JSON string."""
+    if isinstance(obj, tuple(_KINDS.values())):
         return obj
     if isinstance(obj, BaseModel):
-        return Simulation.model_validate_json(obj.model_dump_json())
-    if isinstance(obj, str):
-        return Simulation.model_validate_json(obj)
-    return Simulation.model_validate(obj)
+        obj = obj.model_dump(mode="json")
+    elif isinstance(obj, str):
+        import json
+        obj = json.loads(obj)
+    kind = obj.get("kind", "scattering") if isinstance(obj, dict) else "scattering"
+    if kind not in _KINDS:
+        raise ValueError(f"problem kind {kind!r} is not on this path (scattering, layer_stack, periodic)")
+    return _KINDS[kind].model_validate(obj)
 
 
-__all__ = ["ConstantIndex", "ConstantPermittivity", "Cylinder", "Drude", "DrudeLorentz", "FDTDOptions", "LayeredSphere",
-           "LorentzPole", "Outputs", "Particle", "PlaneWave", "Result", "Shell", "Simulation", "Sphere", "Spheroid",
-           "WavelengthRange", "mesh_mode", "options_for", "parse_problem"]
+__all__ = ["Circle", "ConstantIndex", "ConstantPermittivity", "Cylinder", "DiffractionOrder", "Drude", "DrudeLorentz",
+           "Ellipse", "FDTDOptions", "Incidence", "Layer", "LayerStackProblem", "LayeredSphere", "LorentzPole",
+           "Outputs", "Particle", "PatternedLayer", "PeriodicProblem", "PlaneWave", "Polygon", "Rectangle", "Result",
+           "Shell", "Simulation", "SpectrumResult", "Sphere", "Spheroid", "Structure2D", "WavelengthRange",
+           "mesh_mode", "options_for", "parse_problem", "wavelength_array"]

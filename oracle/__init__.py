@@ -12,6 +12,9 @@ fdtd     FP64 (or FP32) NumPy restatement of ``Engine`` (Yee curls, PEC, CPML, p
 sources  Point dipole, 1D-fed TFSF plane wave, table-fed TFSF (per-step read and scatter),
          Bloch plane source, mode source (per-step injection).
 media    Poles, media, the discrete ADE permittivity, the dispersive time-step limit.
+mie      Bohren & Huffman's Mie series of a homogeneous sphere (the scattering driver's whole-solve check).
+tmm      Characteristic-matrix reflectance and transmittance of coherent layer stacks (the layer-stack and
+         periodic drivers' whole-solve check).
 cyee     A plain-C restatement of the bulk Yee + CPML + pure-ADE step (ctypes), for large
          sizes and the multi-core CPU baseline.
 

@@ -459,3 +459,100 @@ def test_mie_series_reproduces_the_worked_example_and_limits():
     ce, cs, ca = cross_sections(4.0, 60.0, [500.0], n_host=1.0)
     qe, qs, _ = mie_q(2.0, 2 * np.pi * 60.0 / 500.0)
     assert abs(ce[0] - qe * np.pi * 3600) < 1e-9 * ce[0] and abs(ca[0]) < 1e-9 * ce[0]
+
+
+def test_tmm_matches_the_airy_sum_quarter_wave_mirror_and_conserves_energy():
+    """oracle/tmm.py: a single film against the three-layer Airy sum (tests/test_drivers.py: a different method,
+    itself pinned by Fresnel and the quarter-/half-wave cases) at normal and oblique incidence, s and p, lossless
+    and Drude films; the quarter-wave mirror (HL)^N R = ((n0 - Y) / (n0 + Y))^2, Y = (n_H / n_L)^{2N} n_s;
+    R + T = 1 for a lossless multilayer at every angle; the layer order matters (a swapped pair fails the mirror)."""
+    from oracle.tmm import stack_rt
+    from tests.test_drivers import thin_film
+    wl = np.array([450.0, 600.0, 750.0])
+    for n2 in (2.0, 0.3 + 3.1j):
+        for th in (0.0, 0.52, 1.05):
+            for pol in "sp":
+                R, T = stack_rt([n2 ** 2], [37.0], 1.0, 2.25, wl, th, pol)
+                for q, lam in enumerate(wl):
+                    Ra, Ta = thin_film(1.0, n2, 1.5, 37.0, lam, th, pol)
+                    assert abs(R[q] - Ra) < 1e-12 and abs(T[q] - Ta) < 1e-12
+    nh, nl, ns, lam = 2.3, 1.38, 1.52, 550.0
+    for N in (1, 3, 6):
+        eps = [nh ** 2, nl ** 2] * N
+        d = [lam / (4 * nh), lam / (4 * nl)] * N
+        R, T = stack_rt(eps, d, 1.0, ns ** 2, [lam], 0.0, "s")
+        Y = (nh / nl) ** (2 * N) * ns
+        assert abs(R[0] - ((1 - Y) / (1 + Y)) ** 2) < 1e-12 and abs(R[0] + T[0] - 1) < 1e-12
+        Rw, _ = stack_rt(eps[::-1], d[::-1], 1.0, ns ** 2, [lam], 0.0, "s")
+        assert abs(Rw[0] - R[0]) > 1e-3
+    rng = np.random.default_rng(3)
+    eps = list(rng.uniform(1.2, 6.0, 5))
+    d = list(rng.uniform(20, 200, 5))
+    for th in (0.0, 0.4, 1.2):
+        for pol in "sp":
+            R, T = stack_rt(eps, d, 1.44, 2.1, np.linspace(400, 900, 7), th, pol)
+            assert np.all(np.abs(R + T - 1) < 1e-12)
+
+
+def test_rcwa_reduces_to_tmm_conserves_energy_and_converges():
+    """oracle/rcwa.py: a uniform layer (and a uniform two-layer stack, lossy included) is the characteristic-matrix
+    TMM for TE and TM at normal and oblique incidence, every other order empty; lossless gratings (a centred lamellar
+    one and an asymmetric two-box one) conserve energy; N = 25 and N = 50 harmonics agree within 2e-3 on the
+    lamellar grating of the reference's tests; a centred grating at normal incidence sends equal power into +n and
+    -n; specular reflection of the asymmetric grating is reciprocal."""
+    from oracle.rcwa import lamellar
+    from oracle.tmm import stack_rt
+    for pol in "sp":
+        for th in (0.0, 0.3, 0.9):
+            n, R, T = lamellar([(300.0, 4.0, [])], 1.0, 2.25, 700.0, 600.0, th, pol, 8)
+            Rt, Tt = stack_rt([4.0], [300.0], 1.0, 2.25, [600.0], th, pol)
+            assert abs(R[n == 0][0] - Rt[0]) < 1e-12 and abs(T[n == 0][0] - Tt[0]) < 1e-12
+            assert R[n != 0].max() < 1e-20 and T[n != 0].max() < 1e-20
+            n, R, T = lamellar([(40.0, -8.0 + 1.2j, []), (120.0, 2.1, [])], 1.44, 2.25, 500.0, 650.0, th, pol, 6)
+            Rt, Tt = stack_rt([-8.0 + 1.2j, 2.1], [40.0, 120.0], 1.44, 2.25, [650.0], th, pol)
+            assert abs(R[n == 0][0] - Rt[0]) < 1e-12 and abs(T[n == 0][0] - Tt[0]) < 1e-12
+    sym = [(250.0, 1.0, [(0.0, 300.0, 6.0)])]
+    asym = [(180.0, 1.5, [(-120.0, 200.0, 6.0), (190.0, 70.0, 3.0)])]
+    for pol in "sp":
+        for th in (0.0, 0.4):
+            for lay in (sym, asym):
+                n, R, T = lamellar(lay, 1.0, 2.25, 700.0, 600.0, th, pol, 20)
+                assert abs(R.sum() + T.sum() - 1) < 1e-10
+        a = lamellar(sym, 1.0, 2.25, 700.0, 600.0, 0.0, pol, 25)
+        b = lamellar(sym, 1.0, 2.25, 700.0, 600.0, 0.0, pol, 50)
+        for q in (1, 2):
+            assert np.abs(a[q] - b[q][25:76]).max() < 2e-3
+        n, R, T = a
+        assert np.abs(R - R[::-1]).max() < 1e-12 and np.abs(T - T[::-1]).max() < 1e-12
+        assert T[n == 1][0] > 0.1  # the first orders carry power
+        Rp = lamellar(asym, 1.0, 2.25, 700.0, 600.0, 0.35, pol, 20)[1]
+        Rm = lamellar(asym, 1.0, 2.25, 700.0, 600.0, -0.35, pol, 20)[1]
+        assert abs(Rp[20] - Rm[20]) < 1e-9
+        assert np.abs(Rp - Rp[::-1]).max() > 1e-3  # the asymmetric grating is not trivially symmetric
+
+
+def test_rcwa_box_coefficients_are_the_profile_fft_and_the_subwavelength_limit_is_the_effective_medium():
+    """The box Fourier coefficients equal the FFT of the profile sampled on 2^16 points (sign and centre of the
+    phase, width in the sinc); a 30 nm period grating at 1000 nm is its effective film: TE sees <eps>, TM (E across
+    the lines) 1 / <1/eps> -- the pair swapped misses by far more than the tolerance."""
+    from oracle.rcwa import box_coefficients, lamellar
+    from oracle.tmm import stack_rt
+    L, nx = 700.0, 1 << 16
+    boxes = [(-100.0, 180.0, 6.0), (220.0, 60.0, 3.0)]
+    x = -L / 2 + (np.arange(nx) + 0.5) * L / nx
+    prof = np.full(nx, 1.5)
+    for c, w, e in boxes:
+        prof[np.abs(x - c) < w / 2] = e
+    m = np.arange(-10, 11)
+    fft = (prof[None, :] * np.exp(-2j * np.pi * m[:, None] * x[None, :] / L)).mean(axis=1)
+    assert np.abs(box_coefficients(1.5, boxes, L, m) - fft).max() < 2e-4
+    inv = (1 / prof[None, :] * np.exp(-2j * np.pi * m[:, None] * x[None, :] / L)).mean(axis=1)
+    assert np.abs(box_coefficients(1.5, boxes, L, m, inverse=True) - inv).max() < 2e-4
+    f, e1, e2 = 0.4, 6.0, 1.0
+    lay = [(200.0, e2, [(0.0, f * 30.0, e1)])]
+    e_te, e_tm = f * e1 + (1 - f) * e2, 1 / (f / e1 + (1 - f) / e2)
+    for pol, e_eff, e_wrong in (("s", e_te, e_tm), ("p", e_tm, e_te)):
+        n, R, T = lamellar(lay, 1.0, 2.25, 30.0, 1000.0, 0.0, pol, 30)
+        Re = stack_rt([e_eff], [200.0], 1.0, 2.25, [1000.0], 0.0, pol)[0][0]
+        Rw = stack_rt([e_wrong], [200.0], 1.0, 2.25, [1000.0], 0.0, pol)[0][0]
+        assert abs(R[n == 0][0] - Re) < 2e-3 and abs(Rw - Re) > 0.05

@@ -106,23 +106,40 @@ def test_sphere_fill_matches_the_sampled_volume():
         assert abs(f[c].sum() - vol) / vol < 5e-3
 
 
-def test_vertex_tensor_equals_the_oracle_restatement():
+@pytest.mark.parametrize("periodic", [(False, False, False), (True, True, False)])
+def test_vertex_tensor_equals_the_oracle_restatement(periodic):
     """A Drude-Lorentz sphere in a dielectric host: the product's vertex tensor (classification in torch) and the
-    oracle's NumPy restatement give the same tensor vertices, node and slot tables, fractions and normals."""
+    oracle's NumPy restatement give the same tensor vertices, node and slot tables, fractions, normals and period
+    shifts. Periodic case: x and y periodic over the real nodes 1 .. n - 2, the sphere moved across the x and y
+    cell edges (its images wrap back in), so tensor vertices sit on both sides and nodes past the cell wrap."""
     r, pos = 11.3, np.array([0.7, -0.4, 0.9])
+    period = np.array([(n - 2) * DX for n in SHAPE])
+    lo = ORIGIN + DX  # the first real node
+    per = np.array(periodic)
+    if any(periodic):
+        pos = np.array([lo[0] + 2.0, lo[1] + period[1] - 3.0, 0.9])
     metal = drude_lorentz(1.5, 9.0, 0.07, [(1.2, 2.8, 0.4)])
     media = [Medium(1.7689), metal]
     roi = (np.array([3, 3, 3]), np.array(SHAPE) - 3)
 
     def np_slot(p):
-        return np.where(np.sum((p - pos) ** 2, axis=-1) <= r * r, 1, 0)
+        d = p - pos
+        d = np.where(per, d - period * np.round(d / period), d)
+        return np.where(np.sum(d ** 2, axis=-1) <= r * r, 1, 0)
 
     def t_slot(p):
-        return torch.where(((p - torch.as_tensor(pos)) ** 2).sum(-1) <= r * r, 1, 0)
+        d = p - torch.as_tensor(pos)
+        pt = torch.as_tensor(period)
+        d = torch.where(torch.as_tensor(per), d - pt * torch.round(d / pt), d)
+        return torch.where((d ** 2).sum(-1) <= r * r, 1, 0)
 
-    a = voxel.vertex_tensor(t_slot, media, SHAPE, ORIGIN, DX, *roi, samples=8)
-    b = otensor.vertex_tensor(np_slot, media, SHAPE, ORIGIN, DX, *roi, samples=8)
+    a = voxel.vertex_tensor(t_slot, media, SHAPE, ORIGIN, DX, *roi, samples=8, periodic=periodic)
+    b = otensor.vertex_tensor(np_slot, media, SHAPE, ORIGIN, DX, *roi, samples=8, periodic=periodic)
     assert a is not None and b is not None
+    assert np.array_equal(a.shift, b.shift)
+    if any(periodic):
+        assert np.abs(a.shift).max() == 1 and a.node_idx3[:, :2].min() >= 1
+        assert np.all(a.node_idx3[:, 0] <= SHAPE[0] - 2) and np.all(a.node_idx3[:, 1] <= SHAPE[1] - 2)
     assert np.array_equal(a.vert_idx3, b.vert_idx3)
     assert np.array_equal(a.node_comp, b.node_comp) and np.array_equal(a.node_idx3, b.node_idx3)
     assert np.array_equal(a.vert_nodes, b.vert_nodes)
