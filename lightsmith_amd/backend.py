@@ -109,7 +109,10 @@ def _unsupported(sim) -> str:
     if out.resonances is not None:
         missing.append("resonances output")
     if sim.substrate is not None:
-        missing.append("substrate")
+        from .fdtd.scattering import substrate_unsupported
+        why = substrate_unsupported(sim)
+        if why:
+            missing.append(f"this substrate ({why})")
     if sim.orientation_average:
         missing.append("orientation_average")
     if getattr(sim.illumination, "kind", "plane_wave") != "plane_wave":
@@ -192,8 +195,8 @@ class _NativeFDTD:
             return False, why
         try:
             cls._options(s)
-            from .fdtd.scattering import particle_media
-            particle_media(s.particles, s.medium_index, s.wavelength_array(), [])
+            from .fdtd.scattering import scene_media
+            scene_media(s, [])
         except ValueError as e:
             return False, str(e)
         return True, ""

@@ -8,8 +8,15 @@ reference, for a homogeneous host on a uniform grid):
     post   -> C_sca = outward flux of the scattered field / I, C_abs = -(net total-field flux - the background's
               own) / I, C_ext = C_sca + C_abs, I = n |E_inc|^2 / 2          (scattering.py:428-464)
 
-Not on this path (the backend's ``supports`` declines them): substrates, Gaussian beams, near/far fields,
-resonances, mirror symmetry (runs are full grids), graded grids, tabulated materials (no Drude-Lorentz fit).
+A non-dispersive, lossless substrate below z = interface_z_nm is meshed apart as a planar background
+(``voxel.planar_background``, the rules the source tables assume), the particles rasterised over it, and the plane
+wave fed from tables of the grid's exact layered background (incident, reflected and transmitted), from above or
+through the substrate, at any angle that does not graze the interface. The cross sections keep the same bookkeeping:
+the background's own flux through the total-field box comes from the same tables.
+
+Not on this path (the backend's ``supports`` declines them): dispersive or absorbing substrates, Gaussian beams,
+near/far fields, resonances, mirror symmetry (runs are full grids), graded grids, tabulated materials (no
+Drude-Lorentz fit).
 """
 
 from __future__ import annotations
@@ -25,7 +32,7 @@ from .engine import DFTMonitor, Engine, PMLSpec
 from .layered import OMEGA_H, ObliqueLayeredPlaneWave
 from .media import HBARC_EV_NM, Medium, Pole, time_step
 from .run import FS, flux, flux_points, run_until_decayed
-from .voxel import ParticleSlots, voxelise
+from .voxel import OnSubstrate, ParticleSlots, planar_background, voxelise
 
 
 # --- media (media.py:60-129) --------------------------------------------------------------------------------------
@@ -99,6 +106,33 @@ def particle_media(particles, medium_index: float, wl, warnings) -> tuple[list[M
     return media, slot_of
 
 
+def scene_media(sim, warnings) -> tuple[list[Medium], list[list[int]]]:
+    """The particles' media (``particle_media``) and, with a substrate, its medium as the last slot
+    (scattering.py:26-38)."""
+    media, slot_of = particle_media(sim.particles, sim.medium_index, sim.wavelength_array(), warnings)
+    if sim.substrate is not None:
+        media.append(to_medium(sim.substrate.material, sim.wavelength_array(), warnings))
+    return media, slot_of
+
+
+def substrate_unsupported(sim) -> str:
+    """Why this substrate problem is not on the HIP path ('' if it is; scattering.py:41-61)."""
+    if sim.substrate is None:
+        return ""
+    k, _ = sim.illumination.unit_vectors()
+    if abs(k[2]) < 0.1:
+        return "with a substrate, the plane wave must not graze the interface (|direction_z| >= 0.1)"
+    wl = sim.wavelength_array()
+    eps = np.asarray(sim.substrate.material.eps(wl), dtype=complex)
+    if np.any(np.abs(eps.imag) > 1e-4 * np.abs(eps)):
+        return ("an absorbing substrate is not on the HIP path (its cross sections need the particles' own "
+                "absorption and the far field)")
+    if getattr(sim.substrate.material, "kind", "") not in ("constant_index", "constant_permittivity") \
+            or np.any(eps.real < 1):
+        return "the HIP path takes a non-dispersive substrate (constant index, eps >= 1)"
+    return ""
+
+
 # --- planning ------------------------------------------------------------------------------------------------------
 
 def band(sim) -> tuple[float, float, float]:
@@ -155,6 +189,8 @@ def layout(sim, opts, dx: float) -> Layout:
     the longest wavelength in the host; CPML 10 cells by default."""
     lo = np.min([np.asarray(p.position_nm) - p.shape.bounding_half_extent() for p in sim.particles], axis=0)
     hi = np.max([np.asarray(p.position_nm) + p.shape.bounding_half_extent() for p in sim.particles], axis=0)
+    if sim.substrate is not None:  # the box reaches the interface (scattering.py:93-99)
+        lo[2], hi[2] = min(lo[2], sim.substrate.interface_z_nm), max(hi[2], sim.substrate.interface_z_nm)
     ctr = (lo + hi) / 2
     half_cells = np.ceil((hi - lo) / 2 / dx - 1e-9).astype(int) + 1
     floor = 0.03 * float(np.max(sim.wavelength_array())) / sim.medium_index
@@ -175,7 +211,7 @@ def plan(sim, opts, media=None, warnings=None) -> dict:
     """Everything about the run that is cheap to compute (scattering.py:232-280)."""
     warnings = [] if warnings is None else warnings
     if media is None:
-        media, _ = particle_media(sim.particles, sim.medium_index, sim.wavelength_array(), warnings)
+        media, _ = scene_media(sim, warnings)
     dx = spacing(sim, opts)
     lay = layout(sim, opts, dx)
     dt = time_step(dx, opts.courant, media)
@@ -194,8 +230,9 @@ def plan(sim, opts, media=None, warnings=None) -> dict:
     n_f = len(sim.wavelength_array())
     ext = lay.tfsf[1] - lay.tfsf[0]
     n_k = int(ext.max()) + 5
+    n_bg = max(math.sqrt(media[0].eps_inf), math.sqrt(media[-1].eps_inf) if sim.substrate is not None else 0.0)
     t_tab = 2 * max(5 * tau, 30 / (2 * np.pi / float(np.max(sim.wavelength_array())))) + 4 * tau + \
-        (4 * n_k + 2 * float(np.linalg.norm(ext))) * dx * sim.medium_index
+        (4 * n_k + 2 * float(np.linalg.norm(ext))) * dx * n_bg
     n_t = 1 << int(np.ceil(np.log2(t_tab / dt)))
     m = max(1, int(OMEGA_H / ((omega_c + 3 * omega_half) * dt)))
     mem = 4 * (12 * lay.cells + 4 * n_flux * n_f) + 4 * 6 * n_k * (n_t // m + 8)
@@ -228,6 +265,7 @@ class Scene:
     mode: str
     warnings: list
     voxelise_s: float
+    substrate: dict | None = None
 
     def monitors(self):
         """(scattered-field box, total-field box, incident reference) DFT monitors, energy groups 1 and 2."""
@@ -256,7 +294,7 @@ def setup(sim, opts, device) -> Scene:
     (scattering.py:301-390)."""
     warnings: list[str] = []
     wl = sim.wavelength_array()
-    media, slot_of = particle_media(sim.particles, sim.medium_index, wl, warnings)
+    media, slot_of = scene_media(sim, warnings)
     p = plan(sim, opts, media, warnings)
     lay, dx, dt, stride = p["layout"], p["dx"], p["dt"], p["stride"]
     if lay.cells > opts.max_cells:
@@ -265,10 +303,20 @@ def setup(sim, opts, device) -> Scene:
     khat, ehat = sim.illumination.unit_vectors()
     eps_b = sim.medium_index ** 2
     mode = opts.mesh_refinement or ("conformal" if opts.subpixel else "staircase")
+    slots = ParticleSlots(sim.particles, slot_of)
+    layered, sub_kw, substrate = {}, {}, None
+    if sim.substrate is not None:
+        z_int, s_sub = float(sim.substrate.interface_z_nm), len(media) - 1
+        slots = OnSubstrate(slots, z_int, s_sub)
+        layered = dict(base=planar_background(lay.shape, lay.origin, dx, z_int, media[0], media[s_sub]),
+                       background=(0, s_sub))
+        sub_kw = dict(z_int=z_int, sub=media[s_sub])
+        substrate = {"interface_z_nm": z_int, "incidence": "from above" if khat[2] < 0 else "through the substrate",
+                     "source": "layered tables", "absorbing": False}
     t0 = time.perf_counter()
-    inv_eps, groups, vstats, offdiag = voxelise(ParticleSlots(sim.particles, slot_of), media, lay.shape, lay.origin,
-                                                dx, lay.roi[0], lay.roi[1], mode=mode, samples=opts.subpixel_samples,
-                                                device=_raster_device(device))
+    inv_eps, groups, vstats, offdiag = voxelise(slots, media, lay.shape, lay.origin, dx, lay.roi[0], lay.roi[1],
+                                                mode=mode, samples=opts.subpixel_samples,
+                                                device=_raster_device(device), **layered)
     t_vox = time.perf_counter() - t0
     lam_max = float(wl.max())
     pml = PMLSpec(cells=((lay.pml, lay.pml),) * 3, kappa_max=opts.pml_kappa,
@@ -276,11 +324,11 @@ def setup(sim, opts, device) -> Scene:
     src = ObliqueLayeredPlaneWave(shape=lay.shape, origin=lay.origin, spacing=dx, dt=dt, lo=lay.tfsf[0],
                                   hi=lay.tfsf[1], khat=khat, ehat=ehat, inv_eps=inv_eps, host=media[0],
                                   omega_c=p["omega_c"], omega_half=p["omega_half"],
-                                  max_steps=p["max_steps"] + 4 * stride, omega_min=2 * np.pi / lam_max)
+                                  max_steps=p["max_steps"] + 4 * stride, omega_min=2 * np.pi / lam_max, **sub_kw)
     return Scene(media=media, layout=lay, dt=dt, stride=stride, max_steps=p["max_steps"], inv_eps=inv_eps,
                  groups=groups, offdiag=offdiag, pml=pml, source=src, omega=2 * np.pi / wl,
                  sca=flux_points(*lay.sca, lay.shape, dx), tot=flux_points(*lay.tot, lay.shape, dx), stats=vstats,
-                 mode=mode, warnings=warnings, voxelise_s=t_vox)
+                 mode=mode, warnings=warnings, voxelise_s=t_vox, substrate=substrate)
 
 
 def cross_sections(scene: Scene, e_ref, es, hs, et, ht) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
@@ -327,10 +375,11 @@ def solve(sim, opts, device) -> dict:
             "tfsf_box_nm": float((lay.tfsf[1] - lay.tfsf[0])[0] * lay.dx), "decay_level": level,
             "materials": [m.describe() for m in scene.media], **scene.stats, "symmetry": "off",
             "tfsf_table_stride": scene.source.m, "source": "table-fed plane wave",
-            "runtime_s": time.perf_counter() - t_start, "time_loop_s": t_loop, "voxelise_s": scene.voxelise_s}
+            "runtime_s": time.perf_counter() - t_start, "time_loop_s": t_loop, "voxelise_s": scene.voxelise_s,
+            **({"substrate": scene.substrate} if scene.substrate is not None else {})}
     return {"c_sca": c_sca, "c_abs": c_abs, "c_ext": c_ext, "near_field": None, "far_field": None,
             "resonances": None, "provenance": prov, "warnings": warnings}
 
 
-__all__ = ["Layout", "Scene", "band", "cross_sections", "layout", "particle_media", "plan", "setup", "solve",
-           "spacing", "to_medium"]
+__all__ = ["Layout", "Scene", "band", "cross_sections", "layout", "particle_media", "plan", "scene_media", "setup",
+           "solve", "spacing", "substrate_unsupported", "to_medium"]

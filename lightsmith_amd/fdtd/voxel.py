@@ -73,6 +73,18 @@ class ParticleSlots:
         return out
 
 
+class OnSubstrate:
+    """Points -> medium slot of particles over a substrate: where ``inner`` finds the host (slot 0) below the plane
+    z = z_int, the substrate's slot (scattering.py:83-90)."""
+
+    def __init__(self, inner, z_int: float, sub_slot: int):
+        self.inner, self.z_int, self.sub_slot = inner, float(z_int), int(sub_slot)
+
+    def __call__(self, pts: torch.Tensor) -> torch.Tensor:
+        out = self.inner(pts)
+        return torch.where((out == 0) & (pts[..., 2] < self.z_int), self.sub_slot, out)
+
+
 def _classify(slot_fn, pts: torch.Tensor, chunk: int = 1 << 22) -> torch.Tensor:
     """slot_fn over a (n, 3) point list in chunks."""
     out = torch.empty(pts.shape[0], dtype=torch.int64, device=pts.device)
@@ -109,13 +121,36 @@ def _pixel_stats(slot_fn, ctr: torch.Tensor, size: torch.Tensor, S: int, samples
     return frac, nhat
 
 
+# --- planar background ------------------------------------------------------------------------------------------
+
+def planar_background(shape, origin, dx, z_int: float, host: Medium, sub: Medium) -> np.ndarray:
+    """inv_eps [3, *shape] of the host above the plane z = z_int and a non-dispersive substrate below it, meshed as
+    the layered source tables assume (substrate.py:25-70 for non-dispersive media): with f the substrate's share of
+    an E node's cell along z, E_z takes <1/eps> = f/eps_s + (1-f)/eps_h and E_x, E_y take 1/<eps>."""
+    if host.dispersive or sub.dispersive:
+        raise ValueError("the planar background is meshed here for non-dispersive host and substrate only")
+    shape = tuple(int(v) for v in shape)
+    dz = float(np.broadcast_to(np.asarray(dx, dtype=float), (3,))[2])
+    inv_eps = np.empty((3,) + shape)
+    k = np.arange(shape[2], dtype=float)
+    e_h, e_s = host.eps_inf, sub.eps_inf
+    for c in range(3):
+        z = origin[2] + (k + (0.5 if c == 2 else 0.0)) * dz  # E_c node heights
+        f = np.clip((z_int - (z - dz / 2)) / dz, 0.0, 1.0)
+        inv_eps[c] = (f / e_s + (1 - f) / e_h if c == 2 else 1 / (f * e_s + (1 - f) * e_h))[None, None, :]
+    return inv_eps
+
+
 # --- voxelisation -----------------------------------------------------------------------------------------------
 
 def voxelise(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_hi, *, mode: str = "conformal",
-             samples: int = 8, device="cpu", periodic=(False, False, False)):
+             samples: int = 8, device="cpu", periodic=(False, False, False), base=None, background=()):
     """(inv_eps [3, *shape] float64 with 1 at D-form nodes, groups, stats, OffDiagonal or None) for the media
     slot_fn assigns (0 = the host, which must be non-dispersive); only nodes in [roi_lo, roi_hi) are examined
-    (setup.py:110-273; uniform grid, no mirror planes or layered background)."""
+    (setup.py:110-273; uniform grid, no mirror planes). ``base``: the inv_eps of a planar background meshed
+    beforehand (``planar_background``) and ``background`` the media slots it holds: a node whose cell holds
+    background media only keeps its base value, so a planar interface is meshed alike inside and outside the
+    examined region, and the vertex tensor does not grow along it."""
     if mode not in MESH_MODES:
         raise ValueError(f"mesh refinement {mode!r} is not one of {MESH_MODES}")
     if media[0].dispersive:
@@ -128,8 +163,10 @@ def voxelise(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_hi, *,
     S = len(media)
     disp = np.array([m.dispersive for m in media])
     eps_inf = np.array([m.eps_inf for m in media])
-    inv_eps = np.full((3,) + shape, 1.0 / eps_inf[0])
+    inv_eps = np.full((3,) + shape, 1.0 / eps_inf[0]) if base is None else np.array(base, dtype=float)
     inv_flat = inv_eps.reshape(-1)
+    is_bg = np.zeros(S, dtype=bool)
+    is_bg[list(background)] = True
     n_roi = roi_hi - roi_lo
     # the half-cell lattice origin + (m / 2) dx, m = 2 lo - 1 .. 2 hi: every E position of the examined nodes
     # and every corner and edge midpoint of their cells, classified once
@@ -164,7 +201,7 @@ def voxelise(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_hi, *,
         flat = flat_index(c, idx3[:, 0], idx3[:, 1], idx3[:, 2], shape)
         for s in range(1, S):
             sel = (~mixed_h) & (center_h == s)
-            if not sel.any():
+            if is_bg[s] or not sel.any():
                 continue
             if disp[s]:
                 pure_idx[s].append(flat[sel])
@@ -174,12 +211,17 @@ def voxelise(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_hi, *,
         mi = np.nonzero(mixed_h)[0]
         if not len(mi):
             continue
-        n_mixed += len(mi)
         off = np.zeros(3)
         off[c] = 0.5
         ctr = torch.as_tensor(origin + (idx3[mi] + off) * dx3, device=dev)
         frac_t, nhat_t = _pixel_stats(slot_fn, ctr, size, S, samples)
         frac, nhat = frac_t.cpu().numpy(), nhat_t.cpu().numpy()
+        if is_bg.any():  # cells of background media only keep the base
+            part = frac[~is_bg].sum(axis=0) > 0
+            mi, frac, nhat = mi[part], frac[:, part], nhat[part]
+            if not len(mi):
+                continue
+        n_mixed += len(mi)
         has_n = np.linalg.norm(nhat, axis=1) > 0
         w = np.where(has_n, nhat[:, c] ** 2, 1 / 3)
         if mode == "volume_average":
@@ -219,7 +261,7 @@ def voxelise(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_hi, *,
     tm = None
     if mode == "conformal" and disp.any():
         tm = vertex_tensor(slot_fn, media, shape, origin, dx3, roi_lo, roi_hi, samples=samples, device=dev,
-                           periodic=periodic)
+                           periodic=periodic, background=background)
     if tm is not None:
         taken = tm.idx
         inv_flat[taken] = 1.0
@@ -255,14 +297,15 @@ def _wrap(x: np.ndarray, shape, periodic) -> tuple[np.ndarray, np.ndarray]:
 
 
 def vertex_tensor(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_hi, *, samples: int = 8,
-                  device="cpu", periodic=(False, False, False)) -> TensorMesh | None:
+                  device="cpu", periodic=(False, False, False), background=()) -> TensorMesh | None:
     """The TensorMesh of interfaces with a dispersive medium, or None (setup.py:325-420; uniform grid, no mirror
     planes). Tensor vertices: every vertex whose cube (side dx, centred on it) holds an interface with a dispersive
     medium, grown over the mixed vertices connected to them through shared nodes, so that each node next to a
     tensor vertex has at its other end another tensor vertex or a single medium. Each node's two slots are its
     vertices at the node index (slot 0) and at index + e_c (slot 1); a slot without a tensor vertex takes the
     medium at that vertex. Along a periodic axis every real vertex is examined and indices past the cell wrap to
-    their real node, the period count kept in ``shift`` (the engine applies the Bloch phase of it)."""
+    their real node, the period count kept in ``shift`` (the engine applies the Bloch phase of it). A vertex whose
+    cube holds ``background`` media only (a planar background meshed apart) is not a mixed vertex."""
     dev = torch.device(device)
     shape = tuple(int(v) for v in shape)
     S = len(media)
@@ -284,6 +327,13 @@ def vertex_tensor(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_h
         return None
     frac_t, nhat_t = _pixel_stats(slot_fn, pos[torch.as_tensor(mixed, device=dev)], size, S, samples)
     frac, nhat = frac_t.cpu().numpy(), nhat_t.cpu().numpy()
+    if len(background):
+        fg = np.ones(S, dtype=bool)
+        fg[list(background)] = False
+        part = frac[fg].sum(axis=0) > 0
+        Vm, frac, nhat = Vm[part], frac[:, part], nhat[part]
+        if not len(Vm):
+            return None
     seed = (frac[disp] > 0).any(axis=0) & (frac.max(axis=0) < 1)
     if not seed.any():
         return None
@@ -334,4 +384,4 @@ def vertex_tensor(slot_fn, media: list[Medium], shape, origin, dx, roi_lo, roi_h
                       slot_vert=slot_vert, slot_medium=slot_medium, shape=shape, vert_idx3=T)
 
 
-__all__ = ["MESH_MODES", "ParticleSlots", "vertex_tensor", "voxelise"]
+__all__ = ["MESH_MODES", "OnSubstrate", "ParticleSlots", "planar_background", "vertex_tensor", "voxelise"]

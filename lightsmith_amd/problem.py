@@ -202,6 +202,13 @@ class Outputs(_Model):
     resonances: dict | None = None
 
 
+class Substrate(_Model):
+    """A semi-infinite half-space z < interface_z_nm of the given material (spec.py:142-146)."""
+
+    material: Material
+    interface_z_nm: float = 0.0
+
+
 class Simulation(_Model):
     """A plane-wave scattering problem (spec.py:251-296)."""
 
@@ -211,7 +218,7 @@ class Simulation(_Model):
     medium_index: float = Field(1.0, gt=0)
     illumination: PlaneWave = PlaneWave()
     wavelengths: list[float] | WavelengthRange
-    substrate: dict | None = None
+    substrate: Substrate | None = None
     orientation_average: bool = False
     outputs: Outputs = Outputs()
     options: dict[str, dict[str, Any]] = Field(default_factory=dict)
@@ -483,5 +490,5 @@ def parse_problem(obj):
 __all__ = ["Circle", "ConstantIndex", "ConstantPermittivity", "Cylinder", "DiffractionOrder", "Drude", "DrudeLorentz",
            "Ellipse", "FDTDOptions", "Incidence", "Layer", "LayerStackProblem", "LayeredSphere", "LorentzPole",
            "Outputs", "Particle", "PatternedLayer", "PeriodicProblem", "PlaneWave", "Polygon", "Rectangle", "Result",
-           "Shell", "Simulation", "SpectrumResult", "Sphere", "Spheroid", "Structure2D", "WavelengthRange",
+           "Shell", "Simulation", "SpectrumResult", "Sphere", "Spheroid", "Structure2D", "Substrate", "WavelengthRange",
            "mesh_mode", "options_for", "parse_problem", "wavelength_array"]

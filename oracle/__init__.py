@@ -15,6 +15,10 @@ media    Poles, media, the discrete ADE permittivity, the dispersive time-step l
 mie      Bohren & Huffman's Mie series of a homogeneous sphere (the scattering driver's whole-solve check).
 tmm      Characteristic-matrix reflectance and transmittance of coherent layer stacks (the layer-stack and
          periodic drivers' whole-solve check).
+rcwa     Rigorous coupled-wave analysis of lamellar gratings (the grating orders' whole-solve check).
+dipole_substrate
+         A small sphere above a substrate as a self-consistent point dipole with Sommerfeld reflection (the
+         scattering driver's whole-solve check over a substrate).
 cyee     A plain-C restatement of the bulk Yee + CPML + pure-ADE step (ctypes), for large
          sizes and the multi-core CPU baseline.
 

@@ -17,6 +17,30 @@ from __future__ import annotations
 import numpy as np
 
 
+def mie_ab(m: complex, x: float, n_max: int) -> tuple[np.ndarray, np.ndarray]:
+    """The scattering coefficients a_n, b_n, n = 1 .. n_max (the same recurrences as mie_q)."""
+    m, x = complex(m), float(x)
+    mx = m * x
+    n_mx = int(max(n_max, abs(mx))) + 15
+    D = np.zeros(n_mx + 1, dtype=complex)
+    for n in range(n_mx, 0, -1):
+        D[n - 1] = n / mx - 1.0 / (D[n] + n / mx)
+    psi0, psi1 = np.cos(x), np.sin(x)
+    chi0, chi1 = -np.sin(x), np.cos(x)
+    xi1 = complex(psi1, -chi1)
+    a, b = np.zeros(n_max, dtype=complex), np.zeros(n_max, dtype=complex)
+    for n in range(1, n_max + 1):
+        psi = (2.0 * n - 1.0) * psi1 / x - psi0
+        chi = (2.0 * n - 1.0) * chi1 / x - chi0
+        xi = complex(psi, -chi)
+        a[n - 1] = ((D[n] / m + n / x) * psi - psi1) / ((D[n] / m + n / x) * xi - xi1)
+        b[n - 1] = ((m * D[n] + n / x) * psi - psi1) / ((m * D[n] + n / x) * xi - xi1)
+        psi0, psi1 = psi1, psi
+        chi0, chi1 = chi1, chi
+        xi1 = complex(psi1, -chi1)
+    return a, b
+
+
 def mie_q(m: complex, x: float) -> tuple[float, float, float]:
     """(Q_ext, Q_sca, Q_back) for relative index m and size parameter x = k r (k in the host medium)."""
     m = complex(m)

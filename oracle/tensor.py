@@ -69,8 +69,9 @@ def _pixel_average(slot_fn, pos, dx, S: int, samples: int):
 
 
 def vertex_tensor(slot_fn, media, shape, origin, dx: float, roi_lo, roi_hi, samples: int = 8,
-                  periodic=(False, False, False)):
-    """The TensorMesh of interfaces with a dispersive medium, or None (setup.py:325-420, uniform grid)."""
+                  periodic=(False, False, False), background=()):
+    """The TensorMesh of interfaces with a dispersive medium, or None (setup.py:325-420, uniform grid). Vertices
+    whose cube holds only the ``background`` media are not mixed (setup.py:359-363)."""
     shape = tuple(shape)
     S = len(media)
     disp = np.array([m.dispersive for m in media])
@@ -88,6 +89,11 @@ def vertex_tensor(slot_fn, media, shape, origin, dx: float, roi_lo, roi_hi, samp
     if not len(Vm):
         return None
     frac, nhat = _pixel_average(slot_fn, pos[mixed], dx3, S, samples)
+    if len(background):
+        part = np.delete(frac, list(background), axis=0).sum(axis=0) > 0
+        Vm, frac, nhat = Vm[part], frac[:, part], nhat[part]
+        if not len(Vm):
+            return None
     seed = (frac[disp] > 0).any(axis=0) & (frac.max(axis=0) < 1)
     if not seed.any():
         return None

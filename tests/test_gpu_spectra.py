@@ -181,3 +181,26 @@ def test_lamellar_grating_matches_rcwa(gpu, theta, wl, tol, tol_order):
     print(f"lamellar {theta} deg vs RCWA: totals {worst:.4f}, orders {worst_order:.4f}; "
           f"runs {len(res.provenance['runs'])}, {res.provenance['time_loop_s']:.2f} s")
     assert worst < tol and worst_order < tol_order
+
+
+@pytest.mark.parametrize("substrate, lam0", [
+    (P.Drude(eps_inf=9.5, wp_ev=9.0, gamma_ev=0.07), 480.0),  # a Drude metal, eps -4.1 + 0.3i .. -13.6 + 0.8i
+    (P.DrudeLorentz(eps_inf=3.0, poles=[P.LorentzPole(delta_eps=1.5, omega0_ev=3.0, gamma_ev=1.5)]), 420.0),  # ~4.5 + 1i
+], ids=["metal", "lossy_dielectric"])
+def test_absorbing_substrate_transmission_is_taken_at_the_interface(gpu, substrate, lam0):
+    """80 nm of n = 1.46 on an absorbing substrate, up to 700 nm, at 0 and 30 deg: T is the flux into the substrate
+    at the interface, as TMM gives it (the monitor's depth undone per order), so a metal substrate does not report
+    T = 0. Tolerance 0.02, the reference's for its fitted Si and Au substrates (test_fdtd_periodic.py:268-279). The
+    metal's band starts at 480 nm, clear of its eps = 0 point near 420 nm (420-700 nm: 0.021)."""
+    prob = P.LayerStackProblem(superstrate=CI(n_real=1.0), layers=[P.Layer(thickness_nm=80, material=CI(n_real=1.46))],
+                               substrate=substrate, wavelengths=P.WavelengthRange(start_nm=lam0, stop_nm=700, num=5),
+                               angles_deg=[0.0, 30.0], polarizations=["s", "p"])
+    res = api.solve(prob, "fdtd", gpu)
+    ref = _tmm(prob)
+    d = _max_diff(res, ref)
+    t = np.asarray(res.t["s"])
+    per_wl = np.max([np.abs(np.asarray(getattr(res, q)[p]) - ref[p][i]).max(axis=0)
+                     for p in ref for i, q in enumerate("rt")], axis=0)
+    print(f"80 nm film on {substrate.kind}: max |dR|, |dT| vs TMM {d:.4f} (per wavelength {np.round(per_wl, 4)}); "
+          f"T_s {np.round(t[0], 4)}")
+    assert d < 0.02 and t.min() > 1e-3

@@ -556,3 +556,38 @@ def test_rcwa_box_coefficients_are_the_profile_fft_and_the_subwavelength_limit_i
         Re = stack_rt([e_eff], [200.0], 1.0, 2.25, [1000.0], 0.0, pol)[0][0]
         Rw = stack_rt([e_wrong], [200.0], 1.0, 2.25, [1000.0], 0.0, pol)[0][0]
         assert abs(R[n == 0][0] - Re) < 2e-3 and abs(Rw - Re) > 0.05
+
+
+def test_dipole_substrate_model_reduces_to_mie_the_image_dipole_and_fresnel():
+    """oracle/dipole_substrate.py: with the substrate equal to the host the sphere's C_sca is the Mie series' a_1
+    term (both lit from above and from below); over a perfect conductor (r_s = -1, r_p = 1) the Sommerfeld integrals
+    equal the image dipole's field at distance 2h in closed form (transverse image -p, axial image +p); the Fresnel
+    amplitude driving the dipole reflects |r|^2 = TMM's R for s and p; a plausible slip (r_p with the eps ratio
+    inverted) fails the last check."""
+    from oracle.dipole_substrate import background_field, fresnel, reflected_green, sphere_above_substrate
+    from oracle.mie import mie_ab
+    from oracle.tmm import stack_rt
+    wl = np.array([400.0, 600.0])
+    for below in (False, True):
+        c = sphere_above_substrate(20.0, 2.0, 1.0, 1.0, 40.0, wl, 30.0, "p", from_below=below)
+        k = 2 * np.pi / wl
+        a1 = np.array([mie_ab(2.0, kk * 20.0, 1)[0][0] for kk in k])
+        assert np.allclose(c, 6 * np.pi / k ** 2 * np.abs(a1) ** 2, rtol=1e-9)
+    k1, h = 2 * np.pi / 500.0, 60.0
+    gp, gz = reflected_green(k1, h, lambda s: -1.0 + 0j, lambda s: 1.0 + 0j)
+    d = 2 * h
+    ph = np.exp(1j * k1 * d) / (4 * np.pi * d)
+    assert abs(gp - (-ph * (k1 ** 2 + 1j * k1 / d - 1 / d ** 2))) < 1e-7 * abs(gp)
+    assert abs(gz - 2 * ph * (1 / d ** 2 - 1j * k1 / d)) < 1e-7 * abs(gz)
+    for th in (0.0, 0.5, 1.0):
+        kx = np.sin(th)
+        rs, rp = fresnel(1.0, 2.25, kx)
+        for pol, r in (("s", rs), ("p", rp)):
+            R = stack_rt([], [], 1.0, 2.25, [600.0], th, pol)[0][0]
+            assert abs(abs(r) ** 2 - R) < 1e-12
+        e, _ = background_field(1.0, 1.5, 2 * np.pi / 600.0, 2 * np.pi / 600.0 * kx, 0.0, "p", False)
+        q1, q2 = np.cos(th), np.sqrt(2.25 - kx ** 2)
+        wrong = (q1 - 2.25 * q2) / (q1 + 2.25 * q2)
+        if th > 0:
+            assert abs(abs(wrong) ** 2 - stack_rt([], [], 1.0, 2.25, [600.0], th, "p")[0][0]) > 1e-3
+        assert abs(e[0]) > 0 and abs(e[1]) == 0  # p: E in the plane of incidence
