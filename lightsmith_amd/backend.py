@@ -120,7 +120,7 @@ def _unsupported(sim) -> str:
     return "does not support " + ", ".join(missing) if missing else ""
 
 
-KINDS = ("scattering", "periodic", "layer_stack")
+KINDS = ("scattering", "periodic", "layer_stack", "emission")
 
 
 def _kind(sim) -> str:
@@ -139,6 +139,14 @@ def _result(**kw):
         return RefResult.from_arrays(**kw)
     except Exception:  # noqa: BLE001 - lightsmith absent
         return P.Result.from_arrays(**kw)
+
+
+def _emission(res: P.EmissionResult):
+    try:
+        from lightsmith.results import EmissionResult as RefEmission  # type: ignore
+        return RefEmission.model_validate(res.model_dump(mode="json"))
+    except Exception:  # noqa: BLE001 - lightsmith absent
+        return res
 
 
 def _spectrum(res: P.SpectrumResult):
@@ -175,11 +183,22 @@ class _NativeFDTD:
     @classmethod
     def supports(cls, sim) -> tuple[bool, str]:
         if _kind(sim) not in KINDS:
-            return False, "lightsmith-fdtd (HIP) solves scattering, periodic and layer_stack problems"
+            return False, "lightsmith-fdtd (HIP) solves scattering, periodic, layer_stack and emission problems"
         try:
             s = _as_problem(sim)
         except Exception as e:  # noqa: BLE001 - a reference feature the mirror schema lacks
             return False, f"problem not on the HIP path: {e}"
+        if s.kind == "emission":
+            from .fdtd import emission
+            from .fdtd.scattering import particle_media
+            try:
+                cls._options(s)
+                ok, why = emission.supports(s)
+                if ok:
+                    particle_media(s.particles, s.medium_index, s.wavelength_array(), [])
+            except ValueError as e:
+                return False, str(e)
+            return ok, why
         if s.kind != "scattering":
             from .fdtd import spectra
             try:
@@ -231,6 +250,16 @@ class _NativeFDTD:
     def estimate(cls, sim, device) -> dict[str, Any]:
         from .fdtd import scattering
         s = _as_problem(sim)
+        if s.kind == "emission":
+            from .fdtd import emission
+            p = emission.plan(s, cls._options(s))
+            lay = p["layout"]
+            return {"grid": list(lay.shape), "cells": lay.cells, "spacing_nm": p["dx"],
+                    "dt_fs": p["dt"] / scattering.FS, "steps_estimate": p["est_steps"], "steps_max": p["max_steps"],
+                    "runs": p["runs"], "dft_stride": p["stride"], "precision": "single",
+                    "memory_bytes": p["memory_bytes"], "fits": lay.cells <= cls._options(s).max_cells,
+                    "warnings": p["warnings"],
+                    "notes": "two runs, stepped as one batch: the scene, and the same dipole in the medium alone (P0)"}
         if s.kind != "scattering":
             return cls._estimate_spectrum(s)
         opts = cls._options(s)
@@ -251,6 +280,9 @@ class _NativeFDTD:
         name = _device_name(device)
         if not name.startswith("cuda"):
             raise ValueError(f"{cls.name} runs on a GPU (device 'gpu' or 'cuda:N'), got {name!r}")
+        if s.kind == "emission":
+            from .fdtd import emission
+            return _emission(emission.solve(s, cls._options(s), name))
         if s.kind != "scattering":
             from .fdtd import spectra
             return _spectrum(spectra.solve(s, cls._options(s), name))

@@ -151,9 +151,10 @@ def _n_max(materials, wl) -> tuple[float, bool]:
     return n_max, metal
 
 
-def spacing(sim, opts) -> float:
+def spacing(sim, opts, feature_nm: float | None = None) -> float:
     """Cell size: spacing_nm, else the shortest wavelength over cells_per_wavelength in the densest material,
-    at most the smallest particle half-extent over cells_per_radius (10, or 20 with a metal) (scattering.py:119-133)."""
+    at most the smallest particle half-extent over cells_per_radius (10, or 20 with a metal), and at most a quarter
+    of ``feature_nm`` when given (scattering.py:119-133)."""
     if opts.spacing_nm:
         return float(opts.spacing_nm)
     wl = sim.wavelength_array()
@@ -161,7 +162,9 @@ def spacing(sim, opts) -> float:
     n_max = max(n_max, sim.medium_index)
     cpr = opts.cells_per_radius or (20.0 if metal else 10.0)
     dx = wl.min() / (n_max * opts.cells_per_wavelength)
-    return min(dx, min(float(np.min(p.shape.bounding_half_extent())) for p in sim.particles) / cpr)
+    if sim.particles:
+        dx = min(dx, min(float(np.min(p.shape.bounding_half_extent())) for p in sim.particles) / cpr)
+    return min(dx, feature_nm / 4) if feature_nm else dx
 
 
 @dataclass
@@ -184,12 +187,15 @@ class Layout:
         return int(np.prod(self.shape))
 
 
-def layout(sim, opts, dx: float) -> Layout:
-    """setup.py:627-647: odd shape centred on the particles' bounding box; padding and gap 4 cells, at least 3 % of
-    the longest wavelength in the host; CPML 10 cells by default."""
-    lo = np.min([np.asarray(p.position_nm) - p.shape.bounding_half_extent() for p in sim.particles], axis=0)
-    hi = np.max([np.asarray(p.position_nm) + p.shape.bounding_half_extent() for p in sim.particles], axis=0)
-    if sim.substrate is not None:  # the box reaches the interface (scattering.py:93-99)
+def layout(sim, opts, dx: float, bbox=None) -> Layout:
+    """setup.py:627-647: odd shape centred on the particles' bounding box (or ``bbox`` = (lo, hi)); padding and gap
+    4 cells, at least 3 % of the longest wavelength in the host; CPML 10 cells by default."""
+    if bbox is not None:
+        lo, hi = (np.asarray(v, dtype=float).copy() for v in bbox)
+    else:
+        lo = np.min([np.asarray(p.position_nm) - p.shape.bounding_half_extent() for p in sim.particles], axis=0)
+        hi = np.max([np.asarray(p.position_nm) + p.shape.bounding_half_extent() for p in sim.particles], axis=0)
+    if getattr(sim, "substrate", None) is not None:  # the box reaches the interface (scattering.py:93-99)
         lo[2], hi[2] = min(lo[2], sim.substrate.interface_z_nm), max(hi[2], sim.substrate.interface_z_nm)
     ctr = (lo + hi) / 2
     half_cells = np.ceil((hi - lo) / 2 / dx - 1e-9).astype(int) + 1

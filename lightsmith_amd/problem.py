@@ -338,6 +338,47 @@ class PeriodicProblem(_Model):
         return wavelength_array(self.wavelengths)
 
 
+class Dipole(_Model):
+    """A point electric dipole emitter (problems.py:269-283)."""
+
+    position_nm: tuple[float, float, float] = (0.0, 0.0, 0.0)
+    orientation: tuple[float, float, float] = (0.0, 0.0, 1.0)
+
+    @model_validator(mode="after")
+    def _nonzero(self):
+        if not np.any(self.orientation):
+            raise ValueError("dipole orientation must be non-zero")
+        return self
+
+    def unit(self) -> np.ndarray:
+        o = np.asarray(self.orientation, dtype=float)
+        return o / np.linalg.norm(o)
+
+
+class RadiationPatternRequest(_Model):
+    """Radiated power per solid angle; theta from +z, phi from +x (problems.py:286-291)."""
+
+    theta_deg: list[float] = Field(min_length=1)
+    phi_deg: list[float] = Field([0.0, 90.0], min_length=1)
+    wavelengths_nm: list[float] = Field(min_length=1)
+
+
+class EmissionProblem(_Model):
+    """A point dipole near particles in a homogeneous medium: decay-rate (Purcell) and radiative enhancements and
+    the efficiency, relative to the same dipole in the medium alone (problems.py:294-309)."""
+
+    kind: Literal["emission"] = "emission"
+    particles: list[Particle] = Field(default_factory=list)
+    medium_index: float = Field(1.0, gt=0)
+    dipole: Dipole = Dipole()
+    wavelengths: Wavelengths
+    radiation_pattern: RadiationPatternRequest | None = None
+    options: dict[str, dict[str, Any]] = Field(default_factory=dict)
+
+    def wavelength_array(self) -> np.ndarray:
+        return wavelength_array(self.wavelengths)
+
+
 def options_for(problem, backend_name: str, method: str) -> dict[str, Any]:
     """Method-level options overlaid with backend-specific ones (problems.py:348-350)."""
     opts = getattr(problem, "options", {}) or {}
@@ -468,7 +509,32 @@ class SpectrumResult(BaseModel):
         return out
 
 
-_KINDS = {"scattering": Simulation, "layer_stack": LayerStackProblem, "periodic": PeriodicProblem}
+class RadiationPatternData(BaseModel):
+    wavelength_nm: float
+    theta_deg: list[float]
+    phi_deg: list[float]
+    dp_domega: list[list[float]]  # radiated power per steradian over P0, [theta][phi]
+
+
+class EmissionResult(BaseModel):
+    """Dipole emission near particles relative to the same dipole in the medium alone (results.py:144-152)."""
+
+    model_config = ConfigDict(extra="forbid")
+
+    kind: Literal["emission"] = "emission"
+    method: str
+    backend: str
+    wavelengths_nm: list[float]
+    purcell: list[float]  # total decay-rate enhancement: emitted power / P0
+    radiative: list[float]  # power reaching the far field / P0
+    efficiency: list[float]  # radiative / purcell
+    radiation_pattern: list[RadiationPatternData] | None = None
+    provenance: dict[str, Any] = Field(default_factory=dict)
+    warnings: list[str] = Field(default_factory=list)
+
+
+_KINDS = {"scattering": Simulation, "layer_stack": LayerStackProblem, "periodic": PeriodicProblem,
+          "emission": EmissionProblem}
 
 
 def parse_problem(obj):
@@ -483,12 +549,13 @@ def parse_problem(obj):
         obj = json.loads(obj)
     kind = obj.get("kind", "scattering") if isinstance(obj, dict) else "scattering"
     if kind not in _KINDS:
-        raise ValueError(f"problem kind {kind!r} is not on this path (scattering, layer_stack, periodic)")
+        raise ValueError(f"problem kind {kind!r} is not on this path (scattering, layer_stack, periodic, emission)")
     return _KINDS[kind].model_validate(obj)
 
 
-__all__ = ["Circle", "ConstantIndex", "ConstantPermittivity", "Cylinder", "DiffractionOrder", "Drude", "DrudeLorentz",
-           "Ellipse", "FDTDOptions", "Incidence", "Layer", "LayerStackProblem", "LayeredSphere", "LorentzPole",
+__all__ = ["Circle", "ConstantIndex", "ConstantPermittivity", "Cylinder", "DiffractionOrder", "Dipole", "Drude",
+           "DrudeLorentz", "Ellipse", "EmissionProblem", "EmissionResult", "RadiationPatternData",
+           "RadiationPatternRequest", "FDTDOptions", "Incidence", "Layer", "LayerStackProblem", "LayeredSphere", "LorentzPole",
            "Outputs", "Particle", "PatternedLayer", "PeriodicProblem", "PlaneWave", "Polygon", "Rectangle", "Result",
            "Shell", "Simulation", "SpectrumResult", "Sphere", "Spheroid", "Structure2D", "Substrate", "WavelengthRange",
            "mesh_mode", "options_for", "parse_problem", "wavelength_array"]

@@ -591,3 +591,26 @@ def test_dipole_substrate_model_reduces_to_mie_the_image_dipole_and_fresnel():
         if th > 0:
             assert abs(abs(wrong) ** 2 - stack_rt([], [], 1.0, 2.25, [600.0], th, "p")[0][0]) > 1e-3
         assert abs(e[0]) > 0 and abs(e[1]) == 0  # p: E in the plane of incidence
+
+
+def test_dipole_sphere_emission_obeys_the_optical_theorem_and_the_quasistatic_limit():
+    """oracle/dipole_sphere.py: next to a lossless sphere the total rate equals the radiative rate (to 1e-12, for
+    the radial and tangential forms and a mix); far from the sphere both are 1; a small lossy sphere quenches as
+    1 + c R^3 Im(beta) / (k^3 d^6), c = 6 radial and 3/2 tangential, the error falling 4x per doubling of d (the
+    test_emission.py:36-51 limits of the reference's own Mie emission)."""
+    from oracle.dipole_sphere import dipole_emission
+    for cos2 in (0.0, 0.3, 1.0):
+        t, r = dipole_emission(60.0, 2.25 + 0j, [450.0, 600.0, 800.0], 1.0, 75.0, cos2)
+        assert np.allclose(t, r, rtol=1e-12, atol=0) and np.all(np.abs(t - 1) > 0.05)
+        t, r = dipole_emission(40.0, 4.0 + 0j, [500.0, 700.0], 1.33, 50.0, cos2)
+        assert np.allclose(t, r, rtol=1e-12, atol=0)
+    t, r = dipole_emission(60.0, 2.25 + 0j, [600.0], 1.0, 1e5, 1.0)
+    assert abs(t[0] - 1) < 1e-4 and abs(r[0] - 1) < 1e-4
+    eps, R, wl = 4 + 2j, 5.0, 1e5
+    k, beta = 2 * np.pi / wl, (eps - 1) / (eps + 2)
+    for cos2, c in ((1.0, 6.0), (0.0, 1.5)):
+        err = []
+        for d in (50.0, 100.0):
+            tot, _ = dipole_emission(R, eps, [wl], 1.0, d, cos2)
+            err.append((tot[0] - 1) / (c * R ** 3 * beta.imag / (k ** 3 * d ** 6)) - 1)
+        assert abs(err[1]) < 0.01 and abs(err[0] / err[1] - 4) < 0.4
