@@ -6,7 +6,8 @@ Two runs on one grid, time step and source sampling (emission.py:120-196): the s
 alone. A point current spread over the eight E nodes around the dipole radiates a broadband pulse (``PointDipole``);
 the power it gives the field is P_total(w) = -1/2 Re sum_i E_i(w) J_i(w)* dV at those nodes, and the power reaching
 the far field P_rad(w) the outward flux through a closed box around the dipole and the particles; the reference run
-gives P0(w), and purcell = P_total / P0, radiative = P_rad / P0, efficiency = P_rad / P_total.
+gives P0(w), and purcell = P_total / P0, radiative = P_rad / P0, efficiency = P_rad / P_total. The radiation
+pattern is the box's near-to-far transform (``farfield``), per unit P0 and solid angle.
 
 The fields are float32, where the reference's emission default is float64: P_total is the small real part of a
 self-field ~1/(k dx)^3 larger, so its float32 error grows as (k dx)^-3 (the reference quotes 1-4 % for its float32
@@ -17,8 +18,6 @@ reference's float64 runs agree to 1e-4) but carries no self-field: radiative is 
 only P_total, in purcell and efficiency, keeps the source formula's float32 error. The source-side P0 and its
 difference from the box flux are reported in the provenance. The two runs are stepped as one batch, each stopping
 where it would alone.
-
-Not on this path: radiation patterns (the near-to-far transform over the box); ``supports`` says so.
 """
 
 from __future__ import annotations
@@ -29,6 +28,7 @@ import time
 import numpy as np
 
 from .. import problem as P
+from . import farfield
 from .engine import DFTMonitor, Engine, PMLSpec, TimeMonitor
 from .media import time_step
 from .run import FS, RunOptions, flux, flux_points, run_batch_until_decayed, run_until_decayed
@@ -38,11 +38,16 @@ from .voxel import ParticleSlots, voxelise
 
 
 def supports(prob) -> tuple[bool, str]:
-    """(emission.py:34-37)"""
-    if prob.radiation_pattern is not None:
-        return False, ("radiation patterns (the near-to-far transform of the box fields) are not on the HIP path; "
-                       "purcell, radiative and efficiency are")
+    """(emission.py:34-37): every emission problem whose materials the engine takes."""
     return True, ""
+
+
+def _wavelengths(prob) -> np.ndarray:
+    """The result wavelengths, then the radiation pattern's (emission.py:40-43)."""
+    wl = prob.wavelength_array()
+    if prob.radiation_pattern is not None:
+        wl = np.concatenate([wl, np.asarray(prob.radiation_pattern.wavelengths_nm, dtype=float)])
+    return wl
 
 
 def _gap(prob) -> float | None:
@@ -68,7 +73,7 @@ def _bbox(prob, dx: float):
 def plan(prob, opts: P.FDTDOptions, warnings=None, media=None) -> dict:
     """Grid, time step, DFT stride and step cap of both runs (emission.py:70-103)."""
     warnings = [] if warnings is None else warnings
-    wl = prob.wavelength_array()
+    wl = _wavelengths(prob)
     if media is None:
         media, _ = particle_media(prob.particles, prob.medium_index, wl, warnings)
     dx = spacing(prob, opts, _gap(prob))
@@ -98,7 +103,8 @@ class _Run:
     def __init__(self, device, prob, opts, p, media, slot_of, scene: bool, warnings: list):
         lay, dx, dt, stride = p["layout"], p["dx"], p["dt"], p["stride"]
         self.dx, self.dt, self.stride = dx, dt, stride
-        self.omega = 2 * np.pi / prob.wavelength_array()
+        self.omega = 2 * np.pi / _wavelengths(prob)
+        self.shape, self.origin = lay.shape, lay.origin
         self.stats = {}
         if scene and prob.particles:
             inv_eps, groups, self.stats, offdiag = voxelise(
@@ -126,6 +132,18 @@ class _Run:
         self.min_steps = int((2 * self.src.t0 + transit) / dt)
         self.window = math.ceil(2 * np.pi / float(self.omega.min()) / (stride * dt))
 
+    def pattern(self, prob, f: int, p0: float):
+        """dP/dOmega / P0 (n_theta, n_phi) at frequency index f from the box's near-to-far transform
+        (emission.py:226-240)."""
+        req = prob.radiation_pattern
+        eb, hb = self.mon_box.results()
+        surf = farfield.box_surface(self.box[0], self.box[1], self.box[3], self.shape, self.origin, self.dx)
+        u = farfield.directions(req.theta_deg, req.phi_deg)
+        nb = prob.medium_index
+        ef = farfield.far_field(surf, eb[f], hb[f], u.reshape(-1, 3), nb * self.omega[f], 1 / nb,
+                                center=prob.dipole.position_nm)
+        return (0.5 * nb * np.sum(np.abs(ef) ** 2, axis=1) / p0).reshape(u.shape[:2])
+
     def powers(self, steps: int) -> tuple[np.ndarray, np.ndarray, float]:
         """(P_total, P_rad, relative difference of P_total with the float32 DFT accumulator) per wavelength."""
         n = steps // self.stride
@@ -148,7 +166,8 @@ def solve(prob, opts: P.FDTDOptions, device, batched: bool = True) -> P.Emission
         raise ValueError(why)
     warnings: list[str] = []
     wl = prob.wavelength_array()
-    media, slot_of = particle_media(prob.particles, prob.medium_index, wl, warnings)
+    n_wl = len(wl)
+    media, slot_of = particle_media(prob.particles, prob.medium_index, _wavelengths(prob), warnings)
     p = plan(prob, opts, warnings, media)
     lay = p["layout"]
     if lay.cells > opts.max_cells:
@@ -169,9 +188,18 @@ def solve(prob, opts: P.FDTDOptions, device, batched: bool = True) -> P.Emission
                                      warnings=warnings) for r in runs]
         t_loop = time.perf_counter() - t_loop
         (pt, pr, d32), (p0, p0_rad, d32_ref) = (r.powers(q[0]) for r, q in zip(runs, res))
+        pattern = None
+        if prob.radiation_pattern is not None:  # the pattern wavelengths follow the results' in every monitor
+            req = prob.radiation_pattern
+            pattern = [P.RadiationPatternData(wavelength_nm=float(lam), theta_deg=list(req.theta_deg),
+                                              phi_deg=list(req.phi_deg),
+                                              dp_domega=runs[0].pattern(prob, n_wl + f, float(p0_rad[n_wl + f]))
+                                              .tolist())
+                       for f, lam in enumerate(req.wavelengths_nm)]
     finally:
         for r in runs:
             r.eng.close()
+    pt, pr, p0, p0_rad = pt[:n_wl], pr[:n_wl], p0[:n_wl], p0_rad[:n_wl]
     balance = float(np.max(np.abs(p0 / p0_rad - 1)))
     if balance > 0.02:
         warnings.append(f"reference run: emitted and radiated power differ by {balance:.1%} (float32 fields at "
@@ -187,7 +215,7 @@ def solve(prob, opts: P.FDTDOptions, device, batched: bool = True) -> P.Emission
             "time_loop_s": t_loop, "runtime_s": time.perf_counter() - t_start}
     return P.EmissionResult(method="fdtd", backend="lightsmith-fdtd", wavelengths_nm=wl.tolist(),
                             purcell=purcell.tolist(), radiative=radiative.tolist(),
-                            efficiency=(radiative / purcell).tolist(), provenance=prov,
+                            efficiency=(radiative / purcell).tolist(), radiation_pattern=pattern, provenance=prov,
                             warnings=list(dict.fromkeys(warnings)))
 
 

@@ -14,9 +14,11 @@ wave fed from tables of the grid's exact layered background (incident, reflected
 through the substrate, at any angle that does not graze the interface. The cross sections keep the same bookkeeping:
 the background's own flux through the total-field box comes from the same tables.
 
+The far field (dC_sca/dOmega) is the scattered-field box's near-to-far transform (``farfield``).
+
 Not on this path (the backend's ``supports`` declines them): dispersive or absorbing substrates, Gaussian beams,
-near/far fields, resonances, mirror symmetry (runs are full grids), graded grids, tabulated materials (no
-Drude-Lorentz fit).
+near fields, the far field over a substrate, resonances, mirror symmetry (runs are full grids), graded grids,
+tabulated materials (no Drude-Lorentz fit).
 """
 
 from __future__ import annotations
@@ -28,6 +30,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
+from . import farfield
 from .engine import DFTMonitor, Engine, PMLSpec
 from .layered import OMEGA_H, ObliqueLayeredPlaneWave
 from .media import HBARC_EV_NM, Medium, Pole, time_step
@@ -106,12 +109,21 @@ def particle_media(particles, medium_index: float, wl, warnings) -> tuple[list[M
     return media, slot_of
 
 
+def band_wavelengths(sim) -> np.ndarray:
+    """The wavelengths the pulse must carry: the results' and the far field's (scattering.py:135-140)."""
+    wl = sim.wavelength_array()
+    if sim.outputs.far_field is not None:
+        wl = np.concatenate([wl, np.asarray(sim.outputs.far_field.wavelengths_nm, dtype=float)])
+    return wl
+
+
 def scene_media(sim, warnings) -> tuple[list[Medium], list[list[int]]]:
-    """The particles' media (``particle_media``) and, with a substrate, its medium as the last slot
-    (scattering.py:26-38)."""
-    media, slot_of = particle_media(sim.particles, sim.medium_index, sim.wavelength_array(), warnings)
+    """The particles' media (``particle_media``) and, with a substrate, its medium as the last slot, fitted over
+    the band's wavelengths (scattering.py:26-38)."""
+    wl = band_wavelengths(sim)
+    media, slot_of = particle_media(sim.particles, sim.medium_index, wl, warnings)
     if sim.substrate is not None:
-        media.append(to_medium(sim.substrate.material, sim.wavelength_array(), warnings))
+        media.append(to_medium(sim.substrate.material, wl, warnings))
     return media, slot_of
 
 
@@ -136,8 +148,9 @@ def substrate_unsupported(sim) -> str:
 # --- planning ------------------------------------------------------------------------------------------------------
 
 def band(sim) -> tuple[float, float, float]:
-    """(omega_c, omega_half, omega_max) of the pulse covering the wavelengths (scattering.py:143-152)."""
-    om = 2 * np.pi / sim.wavelength_array()
+    """(omega_c, omega_half, omega_max) of the pulse covering the wavelengths, the far field's included
+    (scattering.py:143-152)."""
+    om = 2 * np.pi / band_wavelengths(sim)
     omega_c = 0.5 * (om.min() + om.max())
     return float(omega_c), float(max(0.5 * (om.max() - om.min()), 0.15 * omega_c)), float(om.max())
 
@@ -237,7 +250,7 @@ def plan(sim, opts, media=None, warnings=None) -> dict:
     ext = lay.tfsf[1] - lay.tfsf[0]
     n_k = int(ext.max()) + 5
     n_bg = max(math.sqrt(media[0].eps_inf), math.sqrt(media[-1].eps_inf) if sim.substrate is not None else 0.0)
-    t_tab = 2 * max(5 * tau, 30 / (2 * np.pi / float(np.max(sim.wavelength_array())))) + 4 * tau + \
+    t_tab = 2 * max(5 * tau, 30 / (2 * np.pi / float(np.max(band_wavelengths(sim))))) + 4 * tau + \
         (4 * n_k + 2 * float(np.linalg.norm(ext))) * dx * n_bg
     n_t = 1 << int(np.ceil(np.log2(t_tab / dt)))
     m = max(1, int(OMEGA_H / ((omega_c + 3 * omega_half) * dt)))
@@ -272,12 +285,18 @@ class Scene:
     warnings: list
     voxelise_s: float
     substrate: dict | None = None
+    omega_ff: np.ndarray | None = None  # the far field's frequencies, if it is asked for
 
     def monitors(self):
-        """(scattered-field box, total-field box, incident reference) DFT monitors, energy groups 1 and 2."""
-        return [DFTMonitor(self.omega, self.sca[0], self.sca[1], self.sca[2], energy=1),
-                DFTMonitor(self.omega, self.tot[0], self.tot[1], self.tot[2], energy=2),
-                DFTMonitor(self.omega, np.arange(3), source=self.source.view)]
+        """(scattered-field box, total-field box, incident reference) DFT monitors, energy groups 1 and 2, then,
+        with a far field, the scattered-field box and the incident reference at its frequencies."""
+        out = [DFTMonitor(self.omega, self.sca[0], self.sca[1], self.sca[2], energy=1),
+               DFTMonitor(self.omega, self.tot[0], self.tot[1], self.tot[2], energy=2),
+               DFTMonitor(self.omega, np.arange(3), source=self.source.view)]
+        if self.omega_ff is not None:
+            out += [DFTMonitor(self.omega_ff, self.sca[0], self.sca[1], self.sca[2]),
+                    DFTMonitor(self.omega_ff, np.arange(3), source=self.source.view)]
+        return out
 
     def engine(self, device, **kw) -> Engine:
         """The HIP engine of the scene, with fresh monitors."""
@@ -330,11 +349,14 @@ def setup(sim, opts, device) -> Scene:
     src = ObliqueLayeredPlaneWave(shape=lay.shape, origin=lay.origin, spacing=dx, dt=dt, lo=lay.tfsf[0],
                                   hi=lay.tfsf[1], khat=khat, ehat=ehat, inv_eps=inv_eps, host=media[0],
                                   omega_c=p["omega_c"], omega_half=p["omega_half"],
-                                  max_steps=p["max_steps"] + 4 * stride, omega_min=2 * np.pi / lam_max, **sub_kw)
+                                  max_steps=p["max_steps"] + 4 * stride,
+                                  omega_min=2 * np.pi / float(np.max(band_wavelengths(sim))), **sub_kw)
     return Scene(media=media, layout=lay, dt=dt, stride=stride, max_steps=p["max_steps"], inv_eps=inv_eps,
                  groups=groups, offdiag=offdiag, pml=pml, source=src, omega=2 * np.pi / wl,
                  sca=flux_points(*lay.sca, lay.shape, dx), tot=flux_points(*lay.tot, lay.shape, dx), stats=vstats,
-                 mode=mode, warnings=warnings, voxelise_s=t_vox, substrate=substrate)
+                 mode=mode, warnings=warnings, voxelise_s=t_vox, substrate=substrate,
+                 omega_ff=None if sim.outputs.far_field is None else
+                 2 * np.pi / np.asarray(sim.outputs.far_field.wavelengths_nm, dtype=float))
 
 
 def cross_sections(scene: Scene, e_ref, es, hs, et, ht) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
@@ -352,6 +374,29 @@ def cross_sections(scene: Scene, e_ref, es, hs, et, ht) -> tuple[np.ndarray, np.
     return c_sca, c_abs, c_sca + c_abs
 
 
+def far_field_data(scene: Scene, sim, e_ff, h_ff, e_inc) -> list:
+    """dC_sca/dOmega on the requested grid from the scattered-field box's near-to-far transform, per unit incident
+    intensity (scattering.py:592-625): theta from the incident direction, phi from the incident polarisation."""
+    from ..problem import FarFieldData
+    req = sim.outputs.far_field
+    khat, ehat = sim.illumination.unit_vectors()
+    th = np.radians(np.asarray(req.theta_deg, dtype=float))[:, None, None]
+    ph = np.radians(np.asarray(req.phi_deg, dtype=float))[None, :, None]
+    u = np.cos(th) * khat + np.sin(th) * (np.cos(ph) * ehat + np.sin(ph) * np.cross(khat, ehat))
+    lay = scene.layout
+    surf = farfield.box_surface(scene.sca[0], scene.sca[1], scene.sca[3], lay.shape, lay.origin, lay.dx)
+    nb = sim.medium_index
+    out = []
+    for f, lam in enumerate(req.wavelengths_nm):
+        ef = farfield.far_field(surf, e_ff[f], h_ff[f], u.reshape(-1, 3), nb * scene.omega_ff[f], 1 / nb,
+                                center=lay.origin + lay.center * lay.dx)
+        intensity = 0.5 * scene.source.n_inc * float(np.sum(np.abs(e_inc[f]) ** 2))
+        dc = (0.5 * nb * np.sum(np.abs(ef) ** 2, axis=1) / intensity).reshape(u.shape[:2])
+        out.append(FarFieldData(wavelength_nm=float(lam), theta_deg=list(req.theta_deg), phi_deg=list(req.phi_deg),
+                                dcsca_domega_nm2=dc.tolist()))
+    return out
+
+
 def solve(sim, opts, device) -> dict:
     """Cross sections of ``sim`` (c_sca, c_abs, c_ext in nm^2 per wavelength) with provenance and warnings."""
     t_start = time.perf_counter()
@@ -359,7 +404,8 @@ def solve(sim, opts, device) -> dict:
     warnings = scene.warnings
     lay, dt, stride = scene.layout, scene.dt, scene.stride
     eng = scene.engine(device)
-    mon_sca, mon_tot, mon_inc = eng.monitors
+    mon_sca, mon_tot, mon_inc = eng.monitors[:3]
+    far = None
     t_loop = time.perf_counter()
     try:
         steps, level, graphs = run_until_decayed(eng, opts, max_steps=scene.max_steps,
@@ -370,6 +416,9 @@ def solve(sim, opts, device) -> dict:
         e_ref, _ = mon_inc.results()
         es, hs = mon_sca.results()
         et, ht = mon_tot.results()
+        if scene.omega_ff is not None:
+            e_ff, h_ff = eng.monitors[3].results()
+            far = far_field_data(scene, sim, e_ff, h_ff, eng.monitors[4].results()[0])
     finally:
         eng.close()
     c_sca, c_abs, c_ext = cross_sections(scene, e_ref, es, hs, et, ht)
@@ -383,9 +432,9 @@ def solve(sim, opts, device) -> dict:
             "tfsf_table_stride": scene.source.m, "source": "table-fed plane wave",
             "runtime_s": time.perf_counter() - t_start, "time_loop_s": t_loop, "voxelise_s": scene.voxelise_s,
             **({"substrate": scene.substrate} if scene.substrate is not None else {})}
-    return {"c_sca": c_sca, "c_abs": c_abs, "c_ext": c_ext, "near_field": None, "far_field": None,
+    return {"c_sca": c_sca, "c_abs": c_abs, "c_ext": c_ext, "near_field": None, "far_field": far,
             "resonances": None, "provenance": prov, "warnings": warnings}
 
 
-__all__ = ["Layout", "Scene", "band", "cross_sections", "layout", "particle_media", "plan", "scene_media", "setup",
-           "solve", "spacing", "substrate_unsupported", "to_medium"]
+__all__ = ["Layout", "Scene", "band", "band_wavelengths", "cross_sections", "far_field_data", "layout",
+           "particle_media", "plan", "scene_media", "setup", "solve", "spacing", "substrate_unsupported", "to_medium"]

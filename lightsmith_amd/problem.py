@@ -192,13 +192,22 @@ class WavelengthRange(_Model):
         return np.linspace(self.start_nm, self.stop_nm, self.num)
 
 
+class FarFieldRequest(_Model):
+    """dC_sca/dOmega on a theta x phi grid: theta from the incident direction, phi the azimuth around it from the
+    incident polarisation (phi = 0 the E-plane, 90 the H-plane) (spec.py:216-225)."""
+
+    theta_deg: list[float] = Field(min_length=1)
+    phi_deg: list[float] = Field([0.0, 90.0], min_length=1)
+    wavelengths_nm: list[float] = Field(min_length=1)
+
+
 class Outputs(_Model):
-    """Only cross sections are produced on this path; near-/far-field and resonance requests are declined by the
-    backend's ``supports``."""
+    """Cross sections and the far field are produced on this path; near-field and resonance requests are declined
+    by the backend's ``supports``."""
 
     cross_sections: bool = True
     near_field: dict | None = None
-    far_field: dict | None = None
+    far_field: FarFieldRequest | None = None
     resonances: dict | None = None
 
 
@@ -428,6 +437,13 @@ def mesh_mode(opts: FDTDOptions) -> str:
 
 # --- result (results.py:39-88) ---------------------------------------------------------------------------------------
 
+class FarFieldData(BaseModel):
+    wavelength_nm: float
+    theta_deg: list[float]
+    phi_deg: list[float]
+    dcsca_domega_nm2: list[list[float]]  # dC_sca/dOmega in nm^2/sr, [theta][phi]
+
+
 class Result(BaseModel):
     """Cross sections of a scattering problem."""
 
@@ -442,7 +458,7 @@ class Result(BaseModel):
     c_abs_nm2: list[float]
     equivalent_radius_nm: float
     near_field: list | None = None
-    far_field: list | None = None
+    far_field: list[FarFieldData] | None = None
     resonances: list | None = None
     provenance: dict[str, Any] = Field(default_factory=dict)
     warnings: list[str] = Field(default_factory=list)
@@ -554,7 +570,7 @@ def parse_problem(obj):
 
 
 __all__ = ["Circle", "ConstantIndex", "ConstantPermittivity", "Cylinder", "DiffractionOrder", "Dipole", "Drude",
-           "DrudeLorentz", "Ellipse", "EmissionProblem", "EmissionResult", "RadiationPatternData",
+           "DrudeLorentz", "Ellipse", "EmissionProblem", "EmissionResult", "FarFieldData", "FarFieldRequest", "RadiationPatternData",
            "RadiationPatternRequest", "FDTDOptions", "Incidence", "Layer", "LayerStackProblem", "LayeredSphere", "LorentzPole",
            "Outputs", "Particle", "PatternedLayer", "PeriodicProblem", "PlaneWave", "Polygon", "Rectangle", "Result",
            "Shell", "Simulation", "SpectrumResult", "Sphere", "Spheroid", "Structure2D", "Substrate", "WavelengthRange",

@@ -84,3 +84,32 @@ def cross_sections(eps_sphere, radius: float, wavelengths, n_host: float = 1.0):
         g = np.pi * radius ** 2
         out[:, q] = (qe * g, qs * g, (qe - qs) * g)
     return out[0], out[1], out[2]
+
+
+def amplitudes(m: complex, x: float, mu) -> tuple[np.ndarray, np.ndarray]:
+    """The scattering amplitudes S1, S2 at mu = cos(theta) (Bohren & Huffman eq. 4.74): with the angular functions
+    pi_n = (2n-1)/(n-1) mu pi_{n-1} - n/(n-1) pi_{n-2} (pi_0 = 0, pi_1 = 1) and tau_n = n mu pi_n - (n+1) pi_{n-1},
+    S1 = sum (2n+1)/(n(n+1)) (a_n pi_n + b_n tau_n), S2 = sum (2n+1)/(n(n+1)) (a_n tau_n + b_n pi_n)."""
+    mu = np.atleast_1d(np.asarray(mu, dtype=float))
+    n_stop = int(x + 4.0 * x ** (1.0 / 3.0) + 2.0)
+    a, b = mie_ab(m, x, n_stop)
+    s1 = np.zeros(mu.shape, dtype=complex)
+    s2 = np.zeros(mu.shape, dtype=complex)
+    pi0, pi1 = np.zeros_like(mu), np.ones_like(mu)
+    for n in range(1, n_stop + 1):
+        tau = n * mu * pi1 - (n + 1) * pi0
+        f = (2 * n + 1) / (n * (n + 1))
+        s1 += f * (a[n - 1] * pi1 + b[n - 1] * tau)
+        s2 += f * (a[n - 1] * tau + b[n - 1] * pi1)
+        pi0, pi1 = pi1, ((2 * n + 1) * mu * pi1 - (n + 1) * pi0) / n
+    return s1, s2
+
+
+def dcsca_domega(eps_sphere, radius: float, wavelength: float, theta_deg, phi_deg, n_host: float = 1.0):
+    """dC_sca/dOmega (n_theta, n_phi) of a sphere, theta from the incident direction, phi from the incident
+    polarisation: (|S2|^2 cos^2 phi + |S1|^2 sin^2 phi) / k^2 (Bohren & Huffman eq. 3.16, 4.74)."""
+    k = 2 * np.pi * n_host / wavelength
+    s1, s2 = amplitudes(np.sqrt(complex(eps_sphere)) / n_host, k * radius, np.cos(np.radians(theta_deg)))
+    ph = np.radians(np.asarray(phi_deg, dtype=float))[None, :]
+    return (np.abs(s2[:, None]) ** 2 * np.cos(ph) ** 2 + np.abs(s1[:, None]) ** 2 * np.sin(ph) ** 2) / k ** 2
+

@@ -80,3 +80,16 @@ def test_materials_map_to_the_reference_poles():
     assert len(m.poles) == 3 and m.poles[0].omega0 == 0.0
     wl = np.linspace(400, 800, 9)
     assert np.allclose(m.eps(2 * np.pi / wl), mat.eps(wl), rtol=1e-9)
+
+
+def test_far_field_is_taken_and_declined_over_a_substrate():
+    from lightsmith_amd.fdtd import scattering
+    base = json.loads(workloads.au_sphere_sim().model_dump_json())
+    base["outputs"] = {"far_field": {"theta_deg": [0.0, 90.0], "wavelengths_nm": [550.0, 900.0]}}
+    ok, why = backend.NativeFDTD.supports(base)
+    assert ok, why
+    sim = P.parse_problem(base)
+    assert scattering.band_wavelengths(sim).max() == 900.0  # the pulse carries the far field's band too
+    base["substrate"] = {"material": {"kind": "constant_index", "n_real": 1.5}, "interface_z_nm": -60.0}
+    ok, why = backend.NativeFDTD.supports(base)
+    assert not ok and "far_field" in why

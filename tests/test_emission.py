@@ -1,5 +1,6 @@
 """Dipole emission problems on the CPU (no solve): the schema parses the reference's JSON form, the backend takes
-emission problems and declines radiation patterns with a reason, and the plan follows the reference's rules
+emission problems (radiation patterns included, their wavelengths after the results') and declines double precision
+with a reason, and the plan follows the reference's rules
 (emission.py:46-103): the layout around the particles and the dipole, a quarter of the dipole's gap as the largest
 cell, the 6 omega_half stride margin."""
 
@@ -32,14 +33,17 @@ def test_emission_problem_parses_and_dispatches():
         P.Dipole(orientation=(0.0, 0.0, 0.0))
 
 
-def test_emission_supports_and_declines_patterns():
+def test_emission_supports_patterns_and_declines_foreign_materials():
     B = backend.NativeFDTD
     assert B.supports(near_sphere())[0]
     assert B.supports(P.EmissionProblem(wavelengths=[500.0]))[0]  # a lone dipole
     pat = near_sphere().model_copy(update={"radiation_pattern": P.RadiationPatternRequest(theta_deg=[0.0],
                                                                                           wavelengths_nm=[600.0])})
-    ok, why = B.supports(pat)
-    assert not ok and "radiation pattern" in why
+    assert B.supports(pat)[0]
+    assert emission._wavelengths(pat).tolist() == [450.0, 600.0, 800.0, 600.0]
+    bad = near_sphere().model_copy(update={"options": {"fdtd": {"precision": "double"}}})
+    ok, why = B.supports(bad)
+    assert not ok and "double" in why
 
 
 def test_emission_plan_follows_the_reference_rules():

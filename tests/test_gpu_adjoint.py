@@ -27,8 +27,8 @@ def _hip_grad(case, gpu, **kw):
     acc_e, acc_h = monitor_spectra(rho, prob)
     loss = cases.adjoint_objective(acc_e, acc_h, case)
     loss.backward()
-    out = (rho.grad.detach().cpu().numpy(), float(loss), acc_e.detach().cpu().numpy(), acc_h.detach().cpu().numpy(),
-           prob)
+    out = (rho.grad.detach().cpu().numpy(), float(loss.detach()), acc_e.detach().cpu().numpy(),
+           acc_h.detach().cpu().numpy(), prob)
     return out
 
 

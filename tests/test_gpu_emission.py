@@ -3,7 +3,8 @@ outside a sphere (oracle/dipole_sphere.py), as the reference checks its FDTD emi
 (test_emission.py:70-115):
 
 * a lone dipole: the scene run is the reference run, so radiative is exactly 1; its source power over its box
-  flux (purcell) is 1 to the float32 self-field's accuracy;
+  flux (purcell) is 1 to the float32 self-field's accuracy; its radiation pattern (the box's near-to-far
+  transform) is 3/(8 pi) sin^2(theta) at the reference's tolerance;
 * a dipole tilted 45 deg, 10 nm from an n = 2, r = 60 nm sphere, 2.5 nm cells: purcell and radiative against the
   series, efficiency 1 (lossless). The reference holds its float64 run to 5e-3 and quotes 1-4 % for float32
   fields; here radiative (two box fluxes) is held to 5e-3 and purcell (the float32 source formula, whose error
@@ -25,15 +26,26 @@ from tests.test_emission import near_sphere
 pytestmark = pytest.mark.gpu
 
 
-def test_lone_dipole_is_its_own_reference(gpu):
+def test_lone_dipole_is_its_own_reference_and_radiates_the_dipole_pattern(gpu):
+    """The reference's lone-dipole test (test_emission.py:85-101): its pattern within 2e-3 of the peak of
+    3/(8 pi) sin^2(theta) at every phi, integrating to 1 within 2e-3."""
+    th = np.linspace(0, 180, 19)
     prob = P.EmissionProblem(medium_index=1.33, dipole=P.Dipole(position_nm=(0.3, 0.7, 1.1)),
-                             wavelengths=[500.0, 600.0], options={"fdtd": {"spacing_nm": 2.5}})
+                             wavelengths=[500.0, 600.0],
+                             radiation_pattern=P.RadiationPatternRequest(theta_deg=list(th), phi_deg=[0, 45, 90],
+                                                                         wavelengths_nm=[500.0, 600.0]),
+                             options={"fdtd": {"spacing_nm": 2.5}})
     res = api.solve(prob, "fdtd", gpu)
+    t = np.radians(th)
+    ref = 3 / (8 * np.pi) * np.sin(t) ** 2
+    errs = [np.abs(np.asarray(d.dp_domega) - ref[:, None]).max() / ref.max() for d in res.radiation_pattern]
+    ints = [2 * np.pi * np.trapezoid(np.asarray(d.dp_domega)[:, 0] * np.sin(t), t) for d in res.radiation_pattern]
     print(f"lone dipole: purcell {res.purcell}, radiative {res.radiative}, "
           f"balance {res.provenance['reference_balance']:.2e}, float32 accumulator "
-          f"{res.provenance['float32_accumulator_difference']:.2e}")
+          f"{res.provenance['float32_accumulator_difference']:.2e}; pattern error {errs}, integral {ints}")
     assert res.radiative == [1.0, 1.0]
     assert np.abs(np.asarray(res.purcell) - 1).max() < 2e-3
+    assert max(errs) < 2e-3 and max(abs(v - 1) for v in ints) < 2e-3
 
 
 def test_dielectric_sphere_matches_the_mie_series_and_batch_equals_sequential(gpu):

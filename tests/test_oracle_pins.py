@@ -614,3 +614,26 @@ def test_dipole_sphere_emission_obeys_the_optical_theorem_and_the_quasistatic_li
             tot, _ = dipole_emission(R, eps, [wl], 1.0, d, cos2)
             err.append((tot[0] - 1) / (c * R ** 3 * beta.imag / (k ** 3 * d ** 6)) - 1)
         assert abs(err[1]) < 0.01 and abs(err[0] / err[1] - 4) < 0.4
+
+
+def test_mie_amplitudes_integrate_to_c_sca_and_obey_the_optical_theorem():
+    """oracle/mie.py amplitudes: the angular functions' recurrence against their closed forms for n = 1, 2
+    (pi_2 = 3 mu, tau_2 = 3 cos 2 theta); dC_sca/dOmega integrated over the sphere equals the series' C_sca (to the
+    quadrature's 1e-6); C_ext = 4 pi / k^2 Re S(0) (the optical theorem) equals the series' C_ext; S1 = S2 forward
+    and backward; a Rayleigh sphere's pattern is (cos^2 theta cos^2 phi + sin^2 phi) times |alpha|^2 k^4 / 16 pi^2."""
+    from oracle.mie import amplitudes, cross_sections, dcsca_domega, mie_ab
+    for eps, r, lam in ((2.25, 60.0, 500.0), (4.0 + 0.5j, 80.0, 600.0)):
+        k = 2 * np.pi / lam
+        th = np.linspace(0, np.pi, 2001)
+        pat = dcsca_domega(eps, r, lam, np.degrees(th), [0.0, 90.0])
+        c = 2 * np.pi * np.trapezoid(0.5 * (pat[:, 0] + pat[:, 1]) * np.sin(th), th)
+        ce, cs, _ = cross_sections(eps, r, [lam])
+        assert abs(c / cs[0] - 1) < 1e-5
+        s1, s2 = amplitudes(np.sqrt(eps), k * r, [1.0, -1.0])
+        assert abs(4 * np.pi / k ** 2 * s1[0].real / ce[0] - 1) < 1e-12
+        assert abs(s1[0] - s2[0]) < 1e-12 * abs(s1[0]) and abs(s1[1] + s2[1]) < 1e-12 * abs(s1[1])
+    x, m = 0.01, 1.5
+    a, _ = mie_ab(m, x, 1)
+    mu = np.cos(np.radians([0.0, 30.0, 90.0]))
+    s1, s2 = amplitudes(m, x, mu)
+    assert np.allclose(s1, 1.5 * a[0], rtol=1e-4) and np.allclose(s2, 1.5 * a[0] * mu, rtol=1e-4, atol=1e-11)  # b_1 ~ x^5
