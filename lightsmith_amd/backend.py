@@ -102,12 +102,8 @@ def _unsupported(sim) -> str:
     """Why this problem is not on the HIP path ('' if it is): backends/base.py:67-86 and fdtd.py:65-85."""
     missing = []
     out = sim.outputs
-    if out.near_field is not None:
-        missing.append("near_field output")
     if out.far_field is not None and sim.substrate is not None:
         missing.append("far_field output over a substrate")
-    if out.resonances is not None:
-        missing.append("resonances output")
     if sim.substrate is not None:
         from .fdtd.scattering import substrate_unsupported
         why = substrate_unsupported(sim)
@@ -290,12 +286,14 @@ class _NativeFDTD:
         res = _result(method=cls.method, backend=cls.name, wavelengths_nm=s.wavelength_array(), c_ext=out["c_ext"],
                       c_sca=out["c_sca"], c_abs=out["c_abs"], r_eq=s.equivalent_radius_nm(),
                       provenance=out["provenance"], warnings=out["warnings"])
-        if out["far_field"] is not None:
-            ff = out["far_field"]
-            if not isinstance(res, P.Result):  # the reference's Result: its own FarFieldData
-                from lightsmith.results import FarFieldData as RefFarField  # type: ignore
-                ff = [RefFarField.model_validate(d.model_dump()) for d in ff]
-            res.far_field = ff
+        for key, ref_cls in (("far_field", "FarFieldData"), ("near_field", "NearFieldData"),
+                             ("resonances", "ResonanceData")):
+            if out[key] is not None:
+                data = out[key]
+                if not isinstance(res, P.Result):  # the reference's Result: its own data classes
+                    import lightsmith.results as ref_results  # type: ignore
+                    data = [getattr(ref_results, ref_cls).model_validate(d.model_dump()) for d in data]
+                setattr(res, key, data)
         return res
 
 

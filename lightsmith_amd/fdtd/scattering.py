@@ -14,10 +14,12 @@ wave fed from tables of the grid's exact layered background (incident, reflected
 through the substrate, at any angle that does not graze the interface. The cross sections keep the same bookkeeping:
 the background's own flux through the total-field box comes from the same tables.
 
-The far field (dC_sca/dOmega) is the scattered-field box's near-to-far transform (``farfield``).
+The far field (dC_sca/dOmega) is the scattered-field box's near-to-far transform (``farfield``); the near field the
+total E on a plane, interpolated from a slab of DFT nodes with the tables' background added outside the box; the
+resonances the harmonic inversion (``resonance``) of probe fields sampled once per DFT chunk after the source is off.
 
 Not on this path (the backend's ``supports`` declines them): dispersive or absorbing substrates, Gaussian beams,
-near fields, the far field over a substrate, resonances, mirror symmetry (runs are full grids), graded grids,
+the far field over a substrate, mirror symmetry (runs are full grids), graded grids,
 tabulated materials (no Drude-Lorentz fit).
 """
 
@@ -31,7 +33,7 @@ import numpy as np
 import torch
 
 from . import farfield
-from .engine import DFTMonitor, Engine, PMLSpec
+from .engine import DFTMonitor, Engine, PMLSpec, TimeMonitor, flat_index
 from .layered import OMEGA_H, ObliqueLayeredPlaneWave
 from .media import HBARC_EV_NM, Medium, Pole, time_step
 from .run import FS, flux, flux_points, run_until_decayed
@@ -110,8 +112,11 @@ def particle_media(particles, medium_index: float, wl, warnings) -> tuple[list[M
 
 
 def band_wavelengths(sim) -> np.ndarray:
-    """The wavelengths the pulse must carry: the results' and the far field's (scattering.py:135-140)."""
+    """The wavelengths the pulse must carry: the results', the near field's and the far field's
+    (scattering.py:135-140)."""
     wl = sim.wavelength_array()
+    if sim.outputs.near_field is not None:
+        wl = np.concatenate([wl, np.asarray(sim.outputs.near_field.wavelengths_nm, dtype=float)])
     if sim.outputs.far_field is not None:
         wl = np.concatenate([wl, np.asarray(sim.outputs.far_field.wavelengths_nm, dtype=float)])
     return wl
@@ -286,6 +291,8 @@ class Scene:
     voxelise_s: float
     substrate: dict | None = None
     omega_ff: np.ndarray | None = None  # the far field's frequencies, if it is asked for
+    near: dict | None = None  # the near field's slab of nodes and frequencies, if it is asked for
+    probes: np.ndarray | None = None  # E nodes sampled for the resonances, if they are asked for
 
     def monitors(self):
         """(scattered-field box, total-field box, incident reference) DFT monitors, energy groups 1 and 2, then,
@@ -296,13 +303,18 @@ class Scene:
         if self.omega_ff is not None:
             out += [DFTMonitor(self.omega_ff, self.sca[0], self.sca[1], self.sca[2]),
                     DFTMonitor(self.omega_ff, np.arange(3), source=self.source.view)]
+        if self.near is not None:
+            out += [DFTMonitor(self.near["omega"], self.near["idx"]),
+                    DFTMonitor(self.near["omega"], np.arange(3), source=self.source.view)]
         return out
 
     def engine(self, device, **kw) -> Engine:
-        """The HIP engine of the scene, with fresh monitors."""
+        """The HIP engine of the scene, with fresh monitors (and a time monitor on the resonance probes)."""
+        tm = () if self.probes is None else (TimeMonitor(self.probes, self.max_steps // self.stride + 4),)
         return Engine(device, shape=self.layout.shape, dx=self.layout.dx, dt=self.dt, single=True,
                       inv_eps=self.inv_eps, groups=self.groups, pml=self.pml, sources=[self.source],
-                      monitors=self.monitors(), dft_stride=self.stride, offdiag=self.offdiag, **kw)
+                      monitors=self.monitors(), dft_stride=self.stride, offdiag=self.offdiag, time_monitors=tm,
+                      **kw)
 
 
 def _raster_device(device):
@@ -356,7 +368,9 @@ def setup(sim, opts, device) -> Scene:
                  sca=flux_points(*lay.sca, lay.shape, dx), tot=flux_points(*lay.tot, lay.shape, dx), stats=vstats,
                  mode=mode, warnings=warnings, voxelise_s=t_vox, substrate=substrate,
                  omega_ff=None if sim.outputs.far_field is None else
-                 2 * np.pi / np.asarray(sim.outputs.far_field.wavelengths_nm, dtype=float))
+                 2 * np.pi / np.asarray(sim.outputs.far_field.wavelengths_nm, dtype=float),
+                 near=None if sim.outputs.near_field is None else near_slab(sim, lay),
+                 probes=None if sim.outputs.resonances is None else resonance_probes(sim, lay))
 
 
 def cross_sections(scene: Scene, e_ref, es, hs, et, ht) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
@@ -397,6 +411,118 @@ def far_field_data(scene: Scene, sim, e_ff, h_ff, e_inc) -> list:
     return out
 
 
+def resonance_probes(sim, lay: Layout, max_particles: int = 4) -> np.ndarray:
+    """E nodes (all three components) at each of the first particles' centre and at 14 points two cells outside
+    its bounding box, along the axes and the diagonals (scattering.py:506-521)."""
+    dirs = np.concatenate([np.eye(3), -np.eye(3), np.array(np.meshgrid(*[(1, -1)] * 3, indexing="ij")).reshape(3, -1).T
+                           / np.sqrt(3)])
+    idx = []
+    for p in sim.particles[:max_particles]:
+        c = np.asarray(p.position_nm, dtype=float)
+        h = np.asarray(p.shape.bounding_half_extent(), dtype=float)
+        for pt in [c] + [c + d * (h + 2 * lay.dx) for d in dirs]:
+            for comp in range(3):
+                f = (pt - lay.origin) / lay.dx - 0.5 * np.eye(3)[comp]
+                ijk = np.clip(np.rint(f).astype(int), 1, np.asarray(lay.shape) - 2)
+                idx.append(int(flat_index(comp, *ijk, lay.shape)))
+    return np.unique(np.asarray(idx, dtype=np.int64))
+
+
+def resonances(sim, x: np.ndarray, dt: float, warnings: list) -> list:
+    """The modes of the probes' ring-down in the band, amplitudes relative to the strongest (scattering.py:524-536)."""
+    from ..problem import ResonanceData
+    from .resonance import harmonic_inversion
+    wl = sim.wavelength_array()
+    if x.shape[1] < 64:
+        warnings.append("resonances: the field decayed too fast after the pulse for harmonic inversion; no "
+                        "resonance found (lower decay_by or set run_time_fs)")
+        return []
+    modes = harmonic_inversion(x, dt, 2 * np.pi / wl.max(), 2 * np.pi / wl.min(),
+                               max_modes=sim.outputs.resonances.max_modes)
+    if not modes:
+        return []
+    a0 = modes[0].amplitude
+    return [ResonanceData(wavelength_nm=2 * np.pi / m.omega.real, energy_ev=HBARC_EV_NM * m.omega.real, q=m.q,
+                          amplitude=m.amplitude / a0) for m in modes]
+
+
+_PLANE_AXES = {"xy": (0, 1), "xz": (0, 2), "yz": (1, 2)}
+
+
+def near_slab(sim, lay: Layout) -> dict:
+    """The E nodes (all three components) of the slab of cells around the requested plane that the near field is
+    interpolated from, one cell beyond it on every side (scattering.py:631-652)."""
+    req = sim.outputs.near_field
+    a1, a2 = _PLANE_AXES[req.plane]
+    normal = 3 - a1 - a2
+    ranges = []
+    for a in range(3):
+        half = lay.dx if a == normal else req.half_width_nm + lay.dx
+        centre = req.offset_nm if a == normal else 0.0
+        lo = max(1, int(math.floor((centre - half - lay.origin[a]) / lay.dx)) - 1)
+        hi = min(lay.shape[a] - 2, int(math.ceil((centre + half - lay.origin[a]) / lay.dx)) + 1)
+        if hi <= lo:
+            raise ValueError("near_field plane lies outside the FDTD grid; reduce half_width_nm or offset_nm")
+        ranges.append(np.arange(lo, hi + 1))
+    gi, gj, gk = np.meshgrid(*ranges, indexing="ij")
+    idx = np.concatenate([flat_index(c, gi, gj, gk, lay.shape).ravel() for c in range(3)])
+    return {"ranges": ranges, "idx": idx,
+            "omega": 2 * np.pi / np.asarray(req.wavelengths_nm, dtype=float)}
+
+
+def near_field_data(scene: Scene, sim, e_slab, e_inc, warnings=None) -> list:
+    """The total field on the requested plane (scattering.py:655-735): the slab's DFT, plus the background wave
+    from the source tables at nodes outside the total-field box (where the DFT holds the scattered field),
+    each component interpolated trilinearly from its own Yee positions to the points, over the incident amplitude
+    at the origin."""
+    from scipy.interpolate import RegularGridInterpolator
+
+    from ..problem import NearFieldData
+    req = sim.outputs.near_field
+    a1, a2 = _PLANE_AXES[req.plane]
+    normal = 3 - a1 - a2
+    lay, src, near = scene.layout, scene.source, scene.near
+    rng, om = near["ranges"], near["omega"]
+    shp = tuple(len(r) for r in rng)
+    e_all = np.asarray(e_slab).reshape(len(om), 3, *shp)
+    u = np.linspace(-req.half_width_nm, req.half_width_nm, req.points)
+    g1, g2 = np.meshgrid(u, u, indexing="ij")
+    pts = np.zeros(g1.shape + (3,))
+    pts[..., a1], pts[..., a2], pts[..., normal] = g1, g2, req.offset_nm
+    khat, ehat = sim.illumination.unit_vectors()
+    unit = src.incident_reference(om)
+    amp = np.sum(e_inc * np.conj(unit), axis=1) / np.sum(np.abs(unit) ** 2, axis=1)  # along ehat at the reference
+    inc0 = amp * np.exp(-1j * src.n_inc * om * (khat @ src.r_ref))  # the incident amplitude at the origin
+    lo, hi = lay.tfsf
+    out = []
+    for f, lam in enumerate(req.wavelengths_nm):
+        k = sim.medium_index * om[f]
+        comps = []
+        for c in range(3):
+            frac = [rng[a] + (0.5 if a == c else 0.0) for a in range(3)]  # fractional indices of E_c
+            axes = [lay.origin[a] + frac[a] * lay.dx for a in range(3)]
+            U = np.meshgrid(*frac, indexing="ij")
+            inside = np.all([(U[a] >= lo[a]) & (U[a] <= hi[a]) for a in range(3)], axis=0)
+            nodes = flat_index(c, *[np.floor(U[a] + 1e-9).astype(int).ravel() for a in range(3)], lay.shape)
+            bg = (amp[f] * src.background(om[f:f + 1], nodes, "E")[0]).reshape(U[0].shape)
+            if sim.substrate is None:  # beyond the tables: the continuum plane wave
+                P_ = np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1)
+                bg = np.where(np.isnan(bg), ehat[c] * inc0[f] * np.exp(1j * k * (P_ @ khat)), bg)
+            vals = e_all[f, c] + np.where(inside, 0.0, bg)
+            interp = RegularGridInterpolator(axes, vals, bounds_error=False, fill_value=np.nan)
+            comps.append(interp(pts))
+        e = np.stack(comps, axis=-1) / inc0[f]
+        if warnings is not None and f == 0 and np.isnan(e).any():
+            warnings.append("near_field: points outside the source box beyond the range where the background field "
+                            "is known are NaN")
+        out.append(NearFieldData(wavelength_nm=float(lam), plane=req.plane, offset_nm=req.offset_nm,
+                                 axis1_nm=u.tolist(), axis2_nm=u.tolist(),
+                                 enhancement=np.sum(np.abs(e) ** 2, axis=-1).tolist(),
+                                 e_real=np.moveaxis(e.real, -1, 0).tolist(),
+                                 e_imag=np.moveaxis(e.imag, -1, 0).tolist()))
+    return out
+
+
 def solve(sim, opts, device) -> dict:
     """Cross sections of ``sim`` (c_sca, c_abs, c_ext in nm^2 per wavelength) with provenance and warnings."""
     t_start = time.perf_counter()
@@ -405,7 +531,7 @@ def solve(sim, opts, device) -> dict:
     lay, dt, stride = scene.layout, scene.dt, scene.stride
     eng = scene.engine(device)
     mon_sca, mon_tot, mon_inc = eng.monitors[:3]
-    far = None
+    far = near = res_data = None
     t_loop = time.perf_counter()
     try:
         steps, level, graphs = run_until_decayed(eng, opts, max_steps=scene.max_steps,
@@ -416,9 +542,18 @@ def solve(sim, opts, device) -> dict:
         e_ref, _ = mon_inc.results()
         es, hs = mon_sca.results()
         et, ht = mon_tot.results()
+        nxt = 3
         if scene.omega_ff is not None:
-            e_ff, h_ff = eng.monitors[3].results()
-            far = far_field_data(scene, sim, e_ff, h_ff, eng.monitors[4].results()[0])
+            e_ff, h_ff = eng.monitors[nxt].results()
+            far = far_field_data(scene, sim, e_ff, h_ff, eng.monitors[nxt + 1].results()[0])
+            nxt += 2
+        if scene.near is not None:
+            near = near_field_data(scene, sim, eng.monitors[nxt].results()[0], eng.monitors[nxt + 1].results()[0],
+                                   warnings)
+        if scene.probes is not None:  # the ring-down once the source is off
+            n_skip = int(math.ceil(scene.source.t_off / (stride * dt)))
+            res_data = resonances(sim, eng.time_monitors[0].results(steps // stride)[:, n_skip:], stride * dt,
+                                  warnings)
     finally:
         eng.close()
     c_sca, c_abs, c_ext = cross_sections(scene, e_ref, es, hs, et, ht)
@@ -432,9 +567,10 @@ def solve(sim, opts, device) -> dict:
             "tfsf_table_stride": scene.source.m, "source": "table-fed plane wave",
             "runtime_s": time.perf_counter() - t_start, "time_loop_s": t_loop, "voxelise_s": scene.voxelise_s,
             **({"substrate": scene.substrate} if scene.substrate is not None else {})}
-    return {"c_sca": c_sca, "c_abs": c_abs, "c_ext": c_ext, "near_field": None, "far_field": far,
-            "resonances": None, "provenance": prov, "warnings": warnings}
+    return {"c_sca": c_sca, "c_abs": c_abs, "c_ext": c_ext, "near_field": near, "far_field": far,
+            "resonances": res_data, "provenance": prov, "warnings": warnings}
 
 
 __all__ = ["Layout", "Scene", "band", "band_wavelengths", "cross_sections", "far_field_data", "layout",
-           "particle_media", "plan", "scene_media", "setup", "solve", "spacing", "substrate_unsupported", "to_medium"]
+           "near_field_data", "near_slab", "particle_media", "plan", "resonance_probes", "resonances", "scene_media",
+           "setup", "solve", "spacing", "substrate_unsupported", "to_medium"]

@@ -192,6 +192,16 @@ class WavelengthRange(_Model):
         return np.linspace(self.start_nm, self.stop_nm, self.num)
 
 
+class NearFieldRequest(_Model):
+    """The total electric field on a square plane through the scene (spec.py:206-213)."""
+
+    plane: Literal["xy", "xz", "yz"] = "xz"
+    offset_nm: float = 0.0  # the plane's position along its normal
+    half_width_nm: float = Field(gt=0)
+    points: int = Field(101, ge=2)  # samples per axis
+    wavelengths_nm: list[float] = Field(min_length=1)
+
+
 class FarFieldRequest(_Model):
     """dC_sca/dOmega on a theta x phi grid: theta from the incident direction, phi the azimuth around it from the
     incident polarisation (phi = 0 the E-plane, 90 the H-plane) (spec.py:216-225)."""
@@ -201,14 +211,20 @@ class FarFieldRequest(_Model):
     wavelengths_nm: list[float] = Field(min_length=1)
 
 
+class ResonanceRequest(_Model):
+    """Resonances of the ringing field after the incident pulse has passed: wavelength, Q and relative amplitude of
+    the modes in the band (spec.py:228-233)."""
+
+    max_modes: int = Field(10, ge=1, le=100)
+
+
 class Outputs(_Model):
-    """Cross sections and the far field are produced on this path; near-field and resonance requests are declined
-    by the backend's ``supports``."""
+    """Cross sections, the near and far fields and the resonances (spec.py:236-241)."""
 
     cross_sections: bool = True
-    near_field: dict | None = None
+    near_field: NearFieldRequest | None = None
     far_field: FarFieldRequest | None = None
-    resonances: dict | None = None
+    resonances: ResonanceRequest | None = None
 
 
 class Substrate(_Model):
@@ -437,6 +453,28 @@ def mesh_mode(opts: FDTDOptions) -> str:
 
 # --- result (results.py:39-88) ---------------------------------------------------------------------------------------
 
+class NearFieldData(BaseModel):
+    """The total field on a plane, arrays indexed [axis1][axis2] (and [component] first for E) (results.py:11-21)."""
+
+    wavelength_nm: float
+    plane: str
+    offset_nm: float
+    axis1_nm: list[float]
+    axis2_nm: list[float]
+    enhancement: list[list[float]]  # |E|^2 / |E_inc|^2, NaN where undefined
+    e_real: list[list[list[float]]] | None = None
+    e_imag: list[list[list[float]]] | None = None
+
+
+class ResonanceData(BaseModel):
+    """One resonance, complex frequency omega - i gamma / 2, Q = omega / gamma (results.py:31-37)."""
+
+    wavelength_nm: float  # vacuum wavelength 2 pi c / omega
+    energy_ev: float
+    q: float
+    amplitude: float  # at the probes, relative to the strongest resonance
+
+
 class FarFieldData(BaseModel):
     wavelength_nm: float
     theta_deg: list[float]
@@ -457,9 +495,9 @@ class Result(BaseModel):
     c_sca_nm2: list[float]
     c_abs_nm2: list[float]
     equivalent_radius_nm: float
-    near_field: list | None = None
+    near_field: list[NearFieldData] | None = None
     far_field: list[FarFieldData] | None = None
-    resonances: list | None = None
+    resonances: list[ResonanceData] | None = None
     provenance: dict[str, Any] = Field(default_factory=dict)
     warnings: list[str] = Field(default_factory=list)
 
@@ -570,7 +608,7 @@ def parse_problem(obj):
 
 
 __all__ = ["Circle", "ConstantIndex", "ConstantPermittivity", "Cylinder", "DiffractionOrder", "Dipole", "Drude",
-           "DrudeLorentz", "Ellipse", "EmissionProblem", "EmissionResult", "FarFieldData", "FarFieldRequest", "RadiationPatternData",
+           "DrudeLorentz", "Ellipse", "EmissionProblem", "EmissionResult", "FarFieldData", "FarFieldRequest", "NearFieldData", "NearFieldRequest", "ResonanceData", "ResonanceRequest", "RadiationPatternData",
            "RadiationPatternRequest", "FDTDOptions", "Incidence", "Layer", "LayerStackProblem", "LayeredSphere", "LorentzPole",
            "Outputs", "Particle", "PatternedLayer", "PeriodicProblem", "PlaneWave", "Polygon", "Rectangle", "Result",
            "Shell", "Simulation", "SpectrumResult", "Sphere", "Spheroid", "Structure2D", "Substrate", "WavelengthRange",

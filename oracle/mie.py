@@ -113,3 +113,30 @@ def dcsca_domega(eps_sphere, radius: float, wavelength: float, theta_deg, phi_de
     ph = np.radians(np.asarray(phi_deg, dtype=float))[None, :]
     return (np.abs(s2[:, None]) ** 2 * np.cos(ph) ** 2 + np.abs(s1[:, None]) ** 2 * np.sin(ph) ** 2) / k ** 2
 
+
+def pole(n: int, eps_fn, radius: float, n_host: float, omega0: complex, tol: float = 1e-13) -> complex:
+    """A complex resonance frequency (rad/nm, exp(-i omega t): Im < 0) of Mie's a_n: a zero of its denominator
+    m psi_n(m x) xi_n'(x) - xi_n(x) psi_n'(m x), x = n_host omega r, m = sqrt(eps_fn(omega)) / n_host (even in m, so
+    the branch does not matter), found by the secant method from omega0."""
+    from scipy.special import spherical_jn, spherical_yn
+
+    def denom(w):
+        x = n_host * w * radius
+        m = np.sqrt(complex(eps_fn(w))) / n_host
+        z = m * x
+        jz, djz = spherical_jn(n, z), spherical_jn(n, z, True)
+        hx = spherical_jn(n, x) + 1j * spherical_yn(n, x)
+        dhx = spherical_jn(n, x, True) + 1j * spherical_yn(n, x, True)
+        psi_z, dpsi_z = z * jz, jz + z * djz
+        xi_x, dxi_x = x * hx, hx + x * dhx
+        return m * psi_z * dxi_x - xi_x * dpsi_z
+
+    w0, w1 = complex(omega0), complex(omega0) * (1 + 1e-4)
+    f0, f1 = denom(w0), denom(w1)
+    for _ in range(200):
+        w2 = w1 - f1 * (w1 - w0) / (f1 - f0)
+        if abs(w2 - w1) < tol * abs(w2):
+            return w2
+        w0, f0, w1, f1 = w1, f1, w2, denom(w2)
+    raise RuntimeError("the secant iteration for the Mie pole did not converge")
+

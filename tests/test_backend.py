@@ -35,7 +35,7 @@ def test_au_sphere_plan_is_the_reference_layout():
 
 
 @pytest.mark.parametrize("change,fragment", [
-    (dict(outputs={"near_field": {"plane": "xz"}}), "near_field"),
+    (dict(illumination={"kind": "gaussian_beam", "waist_nm": 500.0}), "illumination"),
     (dict(substrate={"material": {"kind": "constant_index", "n_real": 0.2, "n_imag": 3.0}}), "absorbing substrate"),
     (dict(orientation_average=True), "orientation_average"),
     (dict(options={"fdtd": {"precision": "double"}}), "double"),

@@ -1,7 +1,8 @@
-"""The far field of the scattering path on the MI355X: dC_sca/dOmega of a dielectric sphere from the scattered-field
+"""The far and near fields of the scattering path on the MI355X: dC_sca/dOmega of a dielectric sphere from the scattered-field
 box's near-to-far transform against Mie's amplitudes (oracle/mie.py), at the reference's tolerances
 (test_fdtd.py:304-321: median 2e-3, max 1e-2 over 37 angles in the E- and H-planes; the phi-averaged pattern
-integrating to C_sca within 5e-3); a particle of the host medium has no far field."""
+integrating to C_sca within 5e-3); a particle of the host medium has no far field, and its near field is the plane
+wave."""
 
 import numpy as np
 import pytest
@@ -43,3 +44,58 @@ def test_null_particle_has_no_far_field(gpu):
     ff = np.abs(np.asarray(res.far_field[0].dcsca_domega_nm2)).max()
     print(f"null particle: max dC/dOmega {ff:.2e} nm^2/sr, C_sca {res.c_sca_nm2}")
     assert ff < 1e-8
+
+
+def test_null_particle_near_field_is_the_plane_wave(gpu):
+    """The reference's oblique null-particle test (test_fdtd.py:182-200): off-axis incidence, a particle of the
+    medium itself; on a 9 x 9 xz plane that crosses the total-field box the near field is the plane wave
+    e_hat exp(i n k0 k_hat . r) within 1e-3 (the reference measures 3e-4, the output grid's linear
+    interpolation; the FP64 oracle run of the same scene through the same assembly: 2.7e-4)."""
+    k, e = np.array([1.0, 2.0, 3.0]), np.array([3.0, 0.0, -1.0])
+    sim = P.Simulation(particles=[P.Particle(shape=P.Sphere(radius_nm=10), material=P.ConstantIndex(n_real=1.33))],
+                       medium_index=1.33, wavelengths=[400.0, 800.0],
+                       illumination=P.PlaneWave(direction=tuple(k), polarization=tuple(e)),
+                       options={"fdtd": {"spacing_nm": 4.0, "pml_cells": 6, "padding_nm": 8}},
+                       outputs=P.Outputs(near_field=P.NearFieldRequest(plane="xz", half_width_nm=60, points=9,
+                                                                       wavelengths_nm=[600.0])))
+    res = api.solve(sim, "fdtd", gpu)
+    nf = res.near_field[0]
+    E = np.asarray(nf.e_real) + 1j * np.asarray(nf.e_imag)
+    X, Z = np.meshgrid(nf.axis1_nm, nf.axis1_nm, indexing="ij")
+    kh, eh = k / np.linalg.norm(k), np.asarray(sim.illumination.unit_vectors()[1])
+    ana = eh[:, None, None] * np.exp(1j * 1.33 * 2 * np.pi / 600 * (kh[0] * X + kh[2] * Z))
+    err = np.abs(E - ana).max()
+    print(f"null particle near field: max |E - plane wave| {err:.2e}; C_sca {res.c_sca_nm2}")
+    assert err < 1e-3 and np.abs(res.c_sca_nm2).max() < 1e-6
+
+
+def test_drude_sphere_resonances_match_the_mie_poles(gpu):
+    """The reference's resonance test (test_fdtd.py:327-339): a Drude sphere (eps_inf 9.5, 9 eV, 0.07 eV, r = 40 nm
+    in water), 440-600 nm; the dipole and quadrupole plasmons from the ring-down against the exact complex poles of
+    Mie's a_1 and a_2 (oracle/mie.py pole), at the reference's bounds: dipole within 3 nm with Q in (7, 10) and a
+    relative amplitude above 0.5, quadrupole within 2 nm with Q in (28, 40)."""
+    from oracle.mie import pole
+    drude = P.Drude(eps_inf=9.5, wp_ev=9.0, gamma_ev=0.07)
+    sim = P.Simulation(particles=[P.Particle(shape=P.Sphere(radius_nm=40), material=drude)], medium_index=1.33,
+                       wavelengths=P.WavelengthRange(start_nm=440, stop_nm=600, num=9),
+                       outputs=P.Outputs(resonances=P.ResonanceRequest(max_modes=10)))
+    res = api.solve(sim, "fdtd", gpu)
+    hbarc = 197.3269804
+    wp, g = 9.0 / hbarc, 0.07 / hbarc
+
+    def eps(w):
+        return 9.5 - wp ** 2 / (w ** 2 + 1j * g * w)
+
+    exact = []
+    for n in (1, 2):
+        target = -(n + 1) / n * 1.33 ** 2
+        w = pole(n, eps, 40.0, 1.33, (-1j * g + np.sqrt(-g ** 2 + 4 * wp ** 2 / (9.5 - target))) / 2)
+        exact.append((2 * np.pi / w.real, w.real / (-2 * w.imag)))
+    print("resonances: " + ", ".join(f"{m.wavelength_nm:.1f} nm Q {m.q:.1f} a {m.amplitude:.2f}"
+                                     for m in res.resonances) + f"; Mie poles {exact}")
+    dip = min(res.resonances, key=lambda m: abs(m.wavelength_nm - exact[0][0]))
+    quad = min(res.resonances, key=lambda m: abs(m.wavelength_nm - exact[1][0]))
+    assert abs(dip.wavelength_nm - exact[0][0]) < 3 and 7 < dip.q < 10 and dip.amplitude > 0.5
+    assert abs(quad.wavelength_nm - exact[1][0]) < 2 and 28 < quad.q < 40
+    assert max(m.amplitude for m in res.resonances) == 1.0
+    assert all(440 <= m.wavelength_nm <= 600 for m in res.resonances)

@@ -637,3 +637,24 @@ def test_mie_amplitudes_integrate_to_c_sca_and_obey_the_optical_theorem():
     mu = np.cos(np.radians([0.0, 30.0, 90.0]))
     s1, s2 = amplitudes(m, x, mu)
     assert np.allclose(s1, 1.5 * a[0], rtol=1e-4) and np.allclose(s2, 1.5 * a[0] * mu, rtol=1e-4, atol=1e-11)  # b_1 ~ x^5
+
+
+def test_mie_pole_finder_reproduces_the_quoted_poles_and_the_quasistatic_limit():
+    """oracle/mie.py pole: the complex poles of a_1 and a_2 for a Drude sphere (eps_inf 9.5, 9 eV, 0.07 eV,
+    r = 40 nm in water) at the values the reference quotes for them (test_fdtd.py:328-330: 533.8 nm, Q 8.74;
+    487.5 nm, Q 35.1); a tiny sphere's a_1 pole is the quasi-static eps(omega) = -2 eps_host (5e-6)."""
+    from oracle.mie import pole
+    hbarc = 197.3269804
+    wp, g = 9.0 / hbarc, 0.07 / hbarc
+
+    def eps(w):
+        return 9.5 - wp ** 2 / (w ** 2 + 1j * g * w)
+
+    for n, lam, q, tol in ((1, 533.8, 8.74, 0.06), (2, 487.5, 35.1, 0.06)):
+        target = -(n + 1) / n * 1.33 ** 2
+        w0 = (-1j * g + np.sqrt(-g ** 2 + 4 * wp ** 2 / (9.5 - target))) / 2
+        w = pole(n, eps, 40.0, 1.33, w0)
+        assert abs(2 * np.pi / w.real - lam) < tol and abs(w.real / (-2 * w.imag) - q) < 0.01 * q
+    w0 = (-1j * g + np.sqrt(-g ** 2 + 4 * wp ** 2 / 11.5)) / 2
+    assert abs(pole(1, eps, 0.5, 1.0, w0 * 1.001) / w0 - 1) < 1e-5
+

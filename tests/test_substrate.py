@@ -113,3 +113,19 @@ def test_substrate_plan_reaches_the_interface():
     z = s.layout.origin[2] + k * 2.5
     col = s.inv_eps[0][2, 2, :]
     assert np.allclose(col[z < -2.0], 1 / 2.25) and np.allclose(col[z > 2.0], 1.0)
+
+
+def test_near_field_slab_covers_the_plane():
+    sim = P.Simulation(particles=[P.Particle(shape=P.Sphere(radius_nm=10), material=P.ConstantIndex(n_real=2.0))],
+                       wavelengths=[500.0], options={"fdtd": {"spacing_nm": 4.0}},
+                       outputs=P.Outputs(near_field=P.NearFieldRequest(plane="yz", offset_nm=2.0, half_width_nm=30,
+                                                                       points=5, wavelengths_nm=[500.0, 600.0])))
+    lay = scattering.layout(sim, P.FDTDOptions(spacing_nm=4.0), 4.0)
+    near = scattering.near_slab(sim, lay)
+    x = lay.origin[0] + near["ranges"][0] * 4.0
+    y = lay.origin[1] + near["ranges"][1] * 4.0
+    assert x.min() <= 2.0 - 4.0 and x.max() >= 2.0 + 4.0  # a cell either side of the plane x = 2
+    assert y.min() <= -34.0 and y.max() >= 34.0
+    assert len(near["idx"]) == 3 * np.prod([len(r) for r in near["ranges"]])
+    assert np.allclose(near["omega"], 2 * np.pi / np.array([500.0, 600.0]))
+    assert scattering.band_wavelengths(sim).tolist() == [500.0, 500.0, 600.0]
