@@ -261,8 +261,10 @@ def au_sphere_leg(dev) -> dict:
     eng = scene.engine(dev)
     t_set = time.perf_counter() - t0
     # a 97-cell row fills one 64-cell wavefront row and a third of a second, or three 32-cell rows and one lane of
-    # a fourth: pick the tiling by timing a few (each engine steps alone, so any compiled shape is exact)
-    tilings = [(1, 4, 64, 8), (1, 4, 32, 8), (1, 8, 32, 8), (1, 4, 32, 4), (1, 4, 64, 4)]
+    # a fourth, and 0.91 M cells fill the GPU only with short z marches: pick the tiling by timing a few (each
+    # engine steps alone, so any compiled shape is exact)
+    tilings = [(1, 4, 64, 8), (1, 4, 32, 8), (1, 4, 32, 4), (1, 4, 64, 4), (1, 4, 64, 3), (1, 4, 64, 2),
+               (1, 4, 64, 1), (1, 8, 64, 2)]
     times = {}
     for t in tilings:
         eng.set_tiling(*t)

@@ -222,59 +222,70 @@ __global__ void k_mixed(long long n, int S, int maxp, const long long* __restric
 }
 
 // Conformal vertex tensor (engine.py:839-923). Filters of a slot r (of n slots) for a medium table sm[S][ms]:
-// harmonic y = h0 u + sum_s f_s E_s, E_s = (u - sum_p P_s,p) / eps_inf_s with each medium's poles driven by its
-// own previous E_s (engine.py:796-816); arithmetic y = (u - sum_s f_s sum_p P_s,p) inv_a with all poles driven
-// by the previous y (engine.py:819-836).
+// harmonic y_h = h0 u_h + sum_s f_s E_s, E_s = (u_h - sum_p P_s,p) / eps_inf_s with each medium's poles driven by
+// its own previous E_s (engine.py:796-816); arithmetic (when `arith`) y_a = (u_a - sum_s f_s sum_p P_s,p) inv_a
+// with all poles driven by the previous y_a (engine.py:819-836). A medium's pole pairs of both filters (four
+// poles at a time) are loaded before any of them is stored, and their addresses do not wait for f_s, so the
+// slot's state costs one memory round trip per dispersive medium; each filter's sums keep the reference's order.
 template <class T>
-__device__ __forceinline__ T tensor_harmonic(T u, long long r, long long n, int S, int maxp, const float* sm, int ms,
-                                             const float* f, float h0, T* Eh, T* Ph) {
-  T y = scl(h0, u);
+__device__ __forceinline__ void tensor_filters(T u_h, T u_a, bool arith, long long r, long long n, int S, int maxp,
+                                               const float* __restrict__ sm, int ms, const float* __restrict__ f,
+                                               float h0, float inv_a, T* __restrict__ Eh, T* __restrict__ Ph,
+                                               T* __restrict__ Ea, T* __restrict__ Pa, T& y_h, T& y_a) {
+  T yh = scl(h0, u_h);
+  T acc = u_a;
+  const T ea = arith ? Ea[r] : zero<T>();
   for (int s = 0; s < S; ++s) {
     const float* mp = sm + s * ms;
     const int np = (int)mp[1];
+    if (np == 0) continue;
     const float fs = f[s * n + r];
-    if (np == 0 || fs == 0.f) continue;
     const T eh = Eh[s * n + r];
-    T tot = zero<T>();
-    for (int p = 0; p < np; ++p) {
-      T* P0 = Ph + ((long long)(s * maxp + p) * 2) * n;
-      T* P1 = P0 + n;
-      const T pn = add(add(scl(mp[2 + 3 * p], P0[r]), scl(mp[3 + 3 * p], P1[r])), scl(mp[4 + 3 * p], eh));
-      P1[r] = P0[r];
-      P0[r] = pn;
-      tot = p == 0 ? pn : add(tot, pn);
+    T th = zero<T>(), ta = zero<T>();
+    for (int p0 = 0; p0 < np; p0 += 4) {
+      T hp[4], hq[4], ap[4], aq[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const long long o = ((long long)(s * maxp + p0 + j) * 2) * n + r;
+        if (p0 + j < np) {
+          hp[j] = Ph[o];
+          hq[j] = Ph[o + n];
+          if (arith) {
+            ap[j] = Pa[o];
+            aq[j] = Pa[o + n];
+          }
+        }
+      }
+      if (fs == 0.f) continue;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int p = p0 + j;
+        if (p >= np) break;
+        const float c1 = mp[2 + 3 * p], c2 = mp[3 + 3 * p], c3 = mp[4 + 3 * p];
+        const long long o = ((long long)(s * maxp + p) * 2) * n + r;
+        const T pn = add(add(scl(c1, hp[j]), scl(c2, hq[j])), scl(c3, eh));
+        Ph[o + n] = hp[j];
+        Ph[o] = pn;
+        th = p == 0 ? pn : add(th, pn);
+        if (arith) {
+          const T qn = add(add(scl(c1, ap[j]), scl(c2, aq[j])), scl(c3, ea));
+          Pa[o + n] = ap[j];
+          Pa[o] = qn;
+          ta = p == 0 ? qn : add(ta, qn);
+        }
+      }
     }
-    const T ehn = scl(mp[0], sub(u, tot));
+    if (fs == 0.f) continue;
+    const T ehn = scl(mp[0], sub(u_h, th));
     Eh[s * n + r] = ehn;
-    y = add(y, scl(fs, ehn));
+    yh = add(yh, scl(fs, ehn));
+    if (arith) acc = sub(acc, scl(fs, ta));
   }
-  return y;
-}
-
-template <class T>
-__device__ __forceinline__ T tensor_arithmetic(T u, long long r, long long n, int S, int maxp, const float* sm,
-                                               int ms, const float* f, float inv_a, T* Ea, T* Pa) {
-  T acc = u;
-  const T ea = Ea[r];
-  for (int s = 0; s < S; ++s) {
-    const float* mp = sm + s * ms;
-    const int np = (int)mp[1];
-    const float fs = f[s * n + r];
-    if (np == 0 || fs == 0.f) continue;
-    T tot = zero<T>();
-    for (int p = 0; p < np; ++p) {
-      T* P0 = Pa + ((long long)(s * maxp + p) * 2) * n;
-      T* P1 = P0 + n;
-      const T pn = add(add(scl(mp[2 + 3 * p], P0[r]), scl(mp[3 + 3 * p], P1[r])), scl(mp[4 + 3 * p], ea));
-      P1[r] = P0[r];
-      P0[r] = pn;
-      tot = p == 0 ? pn : add(tot, pn);
-    }
-    acc = sub(acc, scl(fs, tot));
+  y_h = yh;
+  if (arith) {
+    y_a = scl(inv_a, acc);
+    Ea[r] = y_a;
   }
-  const T y = scl(inv_a, acc);
-  Ea[r] = y;
-  return y;
 }
 
 template <class T>
@@ -316,7 +327,11 @@ template <class T>
 __device__ __forceinline__ T tensor_slot(const TensorJob<T>& J, long long q, int k, long long node) {
   const long long n = 2 * J.nN;
   const int v = J.slot_v[q];
-  T contrib;
+  const float h0 = J.h0[q];
+  const float inv_a = J.inv_a[q];
+  T u_h, u_a = zero<T>();
+  float nc = 0.f;
+  int side = 0, c = 0;
   if (v >= 0) {
     T dv[6];
 #pragma unroll
@@ -329,8 +344,8 @@ __device__ __forceinline__ T tensor_slot(const TensorJob<T>& J, long long q, int
     for (int d = 0; d < 3; ++d) dbar[d] = scl(0.5f, add(dv[2 * d], dv[2 * d + 1]));
     const float* nv = J.normal + (long long)v * 3;
     const T sv = add(add(scl(nv[0], dbar[0]), scl(nv[1], dbar[1])), scl(nv[2], dbar[2]));
-    const int c = J.comp[node];
-    const int side = 1 - k;
+    c = J.comp[node];
+    side = 1 - k;
     T dp = dv[0], db = dbar[0];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -338,15 +353,17 @@ __device__ __forceinline__ T tensor_slot(const TensorJob<T>& J, long long q, int
         dp = side ? dv[2 * d + 1] : dv[2 * d];
         db = dbar[d];
       }
-    const float nc = nv[c];
-    const T sigma = add(sv, scl(nc, sub(dp, db)));
-    const T b = tensor_arithmetic(sub(dp, scl(nc, sigma)), q, n, J.S, J.maxp, J.sm, J.ms, J.f, J.inv_a[q], J.Ea, J.Pa);
-    const T a = tensor_harmonic(sigma, q, n, J.S, J.maxp, J.sm, J.ms, J.f, J.h0[q], J.Eh, J.Ph);
-    contrib = scl(0.5f, add(b, scl(nc, a)));
-    if (J.vph) contrib = mul(contrib, conjg_t(J.vph[(long long)v * 6 + c * 2 + side]));
+    nc = nv[c];
+    u_h = add(sv, scl(nc, sub(dp, db)));  // sigma
+    u_a = sub(dp, scl(nc, u_h));
   } else {
-    contrib = scl(0.5f, tensor_harmonic(J.D[node], q, n, J.S, J.maxp, J.sm, J.ms, J.f, J.h0[q], J.Eh, J.Ph));
+    u_h = J.D[node];
   }
+  T a, b = zero<T>();
+  tensor_filters(u_h, u_a, v >= 0, q, n, J.S, J.maxp, J.sm, J.ms, J.f, h0, inv_a, J.Eh, J.Ph, J.Ea, J.Pa, a, b);
+  if (v < 0) return scl(0.5f, a);
+  T contrib = scl(0.5f, add(b, scl(nc, a)));
+  if (J.vph) contrib = mul(contrib, conjg_t(J.vph[(long long)v * 6 + c * 2 + side]));
   return contrib;
 }
 

@@ -38,7 +38,7 @@ def _torch_grad(case):
     te, th = run.run(rho, case["steps"])
     loss = cases.adjoint_objective(te, th, case)
     loss.backward()
-    return rho.grad.numpy(), float(loss), te.detach().numpy(), th.detach().numpy()
+    return rho.grad.numpy(), float(loss.detach()), te.detach().numpy(), th.detach().numpy()
 
 
 def test_adjoint_gradient_matches_autograd(gpu):
