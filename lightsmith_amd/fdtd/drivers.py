@@ -32,7 +32,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .engine import DFTMonitor, DispersiveGroup, Engine, PMLSpec, flat_index
+from .engine import DFTMonitor, DispersiveGroup, Engine, PMLSpec, filling_tiling, flat_index
 from .media import Medium, limit_dt, time_step
 from .run import FS, RunOptions, flux, flux_points, orders, plane_points, run_until_decayed
 from .sources import BlochPlaneSource, PointDipole, TFSFPlaneWave
@@ -367,6 +367,7 @@ def periodic_run(device, cell: PeriodicCell, wavelengths, *, pol: str = "s", kpa
     eng = Engine(device, shape=shape, dx=d, dt=dt, single=True, inv_eps=cell.inv_eps, groups=cell.groups, pml=pml,
                  sources=[src], monitors=[mon_r, mon_t, mon_i], dft_stride=stride, periodic=(True, True, False),
                  bloch=bloch, offdiag=cell.offdiag)
+    eng.set_tiling(*filling_tiling([shape]))
     return PeriodicRun(eng=eng, src=src, mon_r=mon_r, mon_t=mon_t, mon_i=mon_i, pol=pol,
                        kpar=tuple(kpar) if oblique else (0.0, 0.0), oblique=oblique, stride=stride,
                        max_steps=max_steps, min_steps=int((src.t_passed + transit) / dt),

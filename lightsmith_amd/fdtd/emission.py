@@ -29,7 +29,7 @@ import numpy as np
 
 from .. import problem as P
 from . import farfield
-from .engine import DFTMonitor, Engine, PMLSpec, TimeMonitor
+from .engine import DFTMonitor, Engine, PMLSpec, TimeMonitor, filling_tiling
 from .media import time_step
 from .run import FS, RunOptions, flux, flux_points, run_batch_until_decayed, run_until_decayed
 from .scattering import _raster_device, layout, particle_media, spacing
@@ -127,6 +127,7 @@ class _Run:
         self.eng = Engine(device, shape=lay.shape, dx=dx, dt=dt, single=True, inv_eps=inv_eps, groups=groups, pml=pml,
                           sources=[self.src], monitors=[self.mon_box, self.mon_src], dft_stride=stride,
                           offdiag=offdiag, time_monitors=[self.tmon])
+        self.eng.set_tiling(*filling_tiling([lay.shape]))
         transit = float(np.max(lay.sca[1] - lay.sca[0])) * dx * prob.medium_index
         self.max_steps = p["max_steps"]
         self.min_steps = int((2 * self.src.t0 + transit) / dt)

@@ -17,7 +17,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .engine import flat_index
+from .engine import filling_tiling, flat_index
 
 FS = 299.792458  # nm/c per femtosecond (setup.py:16)
 DIVERGED_GROWTH = 1e8  # monitor energy / its peak while the pulse passed beyond which a run has diverged (setup.py:17)
@@ -137,6 +137,8 @@ def run_batch_until_decayed(engines, opts, *, max_steps, min_steps, windows, war
     from .batch import Batch
 
     n = len(engines)
+    if schedule is None:
+        schedule = filling_tiling([e.shape for e in engines]) + (0,)
     trackers = [DecayTracker(opts, max_steps=max_steps[q], min_steps=min_steps[q], window=windows[q])
                 for q in range(n)]
     steps = [0] * n

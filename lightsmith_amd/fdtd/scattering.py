@@ -33,7 +33,7 @@ import numpy as np
 import torch
 
 from . import farfield
-from .engine import DFTMonitor, Engine, PMLSpec, TimeMonitor, flat_index
+from .engine import DFTMonitor, Engine, PMLSpec, TimeMonitor, filling_tiling, flat_index
 from .layered import OMEGA_H, ObliqueLayeredPlaneWave
 from .media import HBARC_EV_NM, Medium, Pole, time_step
 from .run import FS, flux, flux_points, run_until_decayed
@@ -311,10 +311,11 @@ class Scene:
     def engine(self, device, **kw) -> Engine:
         """The HIP engine of the scene, with fresh monitors (and a time monitor on the resonance probes)."""
         tm = () if self.probes is None else (TimeMonitor(self.probes, self.max_steps // self.stride + 4),)
-        return Engine(device, shape=self.layout.shape, dx=self.layout.dx, dt=self.dt, single=True,
-                      inv_eps=self.inv_eps, groups=self.groups, pml=self.pml, sources=[self.source],
-                      monitors=self.monitors(), dft_stride=self.stride, offdiag=self.offdiag, time_monitors=tm,
-                      **kw)
+        eng = Engine(device, shape=self.layout.shape, dx=self.layout.dx, dt=self.dt, single=True,
+                     inv_eps=self.inv_eps, groups=self.groups, pml=self.pml, sources=[self.source],
+                     monitors=self.monitors(), dft_stride=self.stride, offdiag=self.offdiag, time_monitors=tm, **kw)
+        eng.set_tiling(*filling_tiling([eng.shape]))
+        return eng
 
 
 def _raster_device(device):
