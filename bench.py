@@ -113,12 +113,15 @@ def batched_leg(dev, steps: int, n_inst: int = 64, tune: bool = True) -> dict:
 
     from lightsmith_amd.fdtd import workloads
     from lightsmith_amd.fdtd.batch import Batch
+    from lightsmith_amd.fdtd.engine import filling_tiling
 
     t0 = time.perf_counter()
     runs = workloads.oblique_sweep_scene(n_inst=n_inst)
     stride = workloads.common_stride(runs)
     K = workloads.steps_for(steps, stride)
     engs = [workloads.oblique_engine(r, dev, max_steps=3 * K + 8 * stride, stride=stride) for r in runs]
+    for e in engs:  # each sequential run at the tiling the periodic driver gives a grid of its size
+        e.set_tiling(*filling_tiling([e.shape]))
     torch.cuda.synchronize()
     cells = sum(e.N for e in engs)
     log(f"oblique sweep: {len(engs)} runs, {cells / 1e6:.1f} M cells, stride {stride}, set-up "
