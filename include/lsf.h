@@ -169,6 +169,13 @@ int lsf_add_tensor_group(lsf_engine* eng, int64_t n_nodes, const int64_t* dev_no
                          const float* dev_frac, const float* dev_h0, const float* dev_inv_a,
                          const float* dev_slot_media, void* dev_state);
 
+/* Launch the tensor group's D update (D += E - E_prev per node) with the E-side table correction, as one kernel:
+ * the last op of the E correction phase must be a table correction on E and the first op of the group phase a
+ * tensor group. The caller guarantees that no table target is a tensor node, so the D update reads the same E as
+ * after a separate correction launch. No counterpart in the reference (it fuses nothing); adding an E correction
+ * or group op afterwards undoes it. */
+int lsf_fuse_table_tensor(lsf_engine* eng);
+
 /* Energy proxy slots (engine.py:315 "energy"), float[2] (or a field-precision pair). */
 int lsf_set_energy(lsf_engine* eng, float* dev_energy);
 

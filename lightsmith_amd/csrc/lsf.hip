@@ -79,6 +79,10 @@ struct Op {
   // conformal vertex tensor
   lsf::TensorJob<float> tjf{};
   lsf::TensorJob<c32> tjc{};
+  // E-side table launched with the D update of the tensor op at this index of the engine's ops (-1: alone); the
+  // tensor op then skips its own D launch (lsf_fuse_table_tensor)
+  int fuse_tensor = -1;
+  int d_fused = 0;
 };
 
 struct Monitor {
@@ -449,10 +453,21 @@ void run_op(lsf_engine* e, const Op& o, hipStream_t s) {
         hipLaunchKernelGGL(lsf::k_modal<T>, dim3(nblk(o.n, 256)), dim3(256), 0, s, modal_job<T>(e, o));
       break;
     case OP_TABLE:
-      if (o.n > 0)
+      if (o.fuse_tensor >= 0) {
+        const Op& t = e->ops[o.fuse_tensor];
+        lsf::TensorJob<T> J;
+        if constexpr (sizeof(T) == sizeof(float)) J = t.tjf;
+        else J = t.tjc;
+        J.E = (T*)e->E;
+        const int tb = (int)nblk(o.n, 256);
+        hipLaunchKernelGGL(lsf::k_table_tensor_d<T>, dim3(tb + nblk(J.nN, 256)), dim3(256), 0, s, o.n, tb, o.target,
+                           o.row_ptr, F, o.table, o.m, o.cmax, o.base, o.i0, o.u0, o.coef,
+                           (const long long*)e->counter, slot, o.scale, J);
+      } else if (o.n > 0) {
         hipLaunchKernelGGL(lsf::k_table<T>, dim3(nblk(o.n, 256)), dim3(256), 0, s, o.n, o.target, o.row_ptr, F,
                            o.table, o.m, o.cmax, o.base, o.i0, o.u0, o.coef, (const long long*)e->counter, slot,
                            o.scale);
+      }
       break;
     case OP_VIEW:
       hipLaunchKernelGGL(lsf::k_table_view, dim3(1), dim3(64), 0, s, (int)o.n, o.out, o.table, o.m, o.cmax, o.base,
@@ -508,7 +523,7 @@ void run_op(lsf_engine* e, const Op& o, hipStream_t s) {
       if constexpr (sizeof(T) == sizeof(float)) J = o.tjf;
       else J = o.tjc;
       J.E = (T*)e->E;
-      hipLaunchKernelGGL(lsf::k_tensor_d<T>, dim3(nblk(J.nN, 256)), dim3(256), 0, s, J);
+      if (!o.d_fused) hipLaunchKernelGGL(lsf::k_tensor_d<T>, dim3(nblk(J.nN, 256)), dim3(256), 0, s, J);
       hipLaunchKernelGGL(lsf::k_tensor_slot_e<T>, dim3(nblk(2 * J.nN, 256)), dim3(256), 0, s, J);
       break;
     }
@@ -625,6 +640,11 @@ int add_op(lsf_engine* e, const Op& o) {
   if (o.phase < LSF_PH_CORRECT_H || o.phase > LSF_PH_BEFORE_H) return fail(LSF_EINVAL, "bad phase");
   if (o.field != 0 && o.field != 1) return fail(LSF_EINVAL, "field must be 0 (E) or 1 (H)");
   if (e->exec) return fail(LSF_ESTATE, "the engine has a captured graph; reset it before adding work");
+  if (o.phase == LSF_PH_CORRECT_E || o.phase == LSF_PH_GROUPS)
+    for (Op& q : e->ops) {  // a fused table + D launch holds only for the ops it was checked against
+      q.fuse_tensor = -1;
+      q.d_fused = 0;
+    }
   e->ops.push_back(o);
   return LSF_OK;
 }
@@ -1059,6 +1079,25 @@ int lsf_add_tensor_group(lsf_engine* e, int64_t n_nodes, const int64_t* nodes, c
   if (e->cplx) fill(o.tjc, (c32*)state);
   else fill(o.tjf, (float*)state);
   return add_op(e, o);
+}
+
+int lsf_fuse_table_tensor(lsf_engine* e) {
+  if (int rc0 = configurable(e)) return rc0;
+  if (e->exec) return fail(LSF_ESTATE, "the engine has a captured graph; reset it before fusing launches");
+  int last_ce = -1, first_gr = -1;
+  for (int i = 0; i < (int)e->ops.size(); ++i) {
+    const Op& o = e->ops[i];
+    if (o.kind == OP_VIEW) continue;
+    if (o.phase == LSF_PH_CORRECT_E) last_ce = i;
+    if (o.phase == LSF_PH_GROUPS && first_gr < 0) first_gr = i;
+  }
+  if (last_ce < 0 || e->ops[last_ce].kind != OP_TABLE || e->ops[last_ce].field != 0)
+    return fail(LSF_EINVAL, "no E-side table correction at the end of the correction phase");
+  if (first_gr < 0 || e->ops[first_gr].kind != OP_TENSOR)
+    return fail(LSF_EINVAL, "no vertex-tensor group at the start of the group phase");
+  e->ops[last_ce].fuse_tensor = first_gr;
+  e->ops[first_gr].d_fused = 1;
+  return LSF_OK;
 }
 
 int lsf_set_energy(lsf_engine* e, float* energy) {

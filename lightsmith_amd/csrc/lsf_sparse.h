@@ -110,12 +110,12 @@ __device__ __forceinline__ float quintic(const float* __restrict__ T, long long 
 }
 
 template <class T>
-__global__ void k_table(long long n_rows, const long long* __restrict__ target, const int* __restrict__ row_ptr, T* F,
-                        const float* __restrict__ tab, int m, long long cmax, const long long* __restrict__ base,
-                        const long long* __restrict__ i0, const float* __restrict__ u0, const float* __restrict__ coef,
-                        const long long* counter, int slot, const float* scale) {
-  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_rows) return;
+__device__ __forceinline__ void table_row(long long r, const long long* __restrict__ target,
+                                          const int* __restrict__ row_ptr, T* F, const float* __restrict__ tab, int m,
+                                          long long cmax, const long long* __restrict__ base,
+                                          const long long* __restrict__ i0, const float* __restrict__ u0,
+                                          const float* __restrict__ coef, const long long* counter, int slot,
+                                          const float* scale) {
   const long long t = counter[slot];
   float acc = 0.f;
   for (int p = row_ptr[r]; p < row_ptr[r + 1]; ++p) acc += coef[p] * quintic(tab, t, i0[p], u0[p], base[p], m, cmax);
@@ -123,6 +123,16 @@ __global__ void k_table(long long n_rows, const long long* __restrict__ target, 
   if constexpr (sizeof(T) == sizeof(float)) v = acc;
   else v = mk(acc, 0.f);
   apply_corr(F, target[r], v, scale, r);
+}
+
+template <class T>
+__global__ void k_table(long long n_rows, const long long* __restrict__ target, const int* __restrict__ row_ptr, T* F,
+                        const float* __restrict__ tab, int m, long long cmax, const long long* __restrict__ base,
+                        const long long* __restrict__ i0, const float* __restrict__ u0, const float* __restrict__ coef,
+                        const long long* counter, int slot, const float* scale) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_rows) return;
+  table_row(r, target, row_ptr, F, tab, m, cmax, base, i0, u0, coef, counter, slot, scale);
 }
 
 __global__ void k_table_view(int n, float* out, const float* __restrict__ tab, int m, long long cmax,
@@ -316,6 +326,25 @@ struct TensorJob {
 template <class T>
 __global__ void k_tensor_d(TensorJob<T> J) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= J.nN) return;
+  J.D[q] = add(J.D[q], sub(J.E[J.node[q]], J.Ep[q]));
+}
+
+// The E-side table correction and the tensor nodes' D update in one launch: blocks [0, table_blocks) take the
+// table rows, the rest the tensor nodes. Only valid when no table target is a tensor node (the host checks), so
+// that the D update reads the same E as after a separate correction launch.
+template <class T>
+__global__ void k_table_tensor_d(long long n_rows, int table_blocks, const long long* __restrict__ target,
+                                 const int* __restrict__ row_ptr, T* F, const float* __restrict__ tab, int m,
+                                 long long cmax, const long long* __restrict__ base, const long long* __restrict__ i0,
+                                 const float* __restrict__ u0, const float* __restrict__ coef,
+                                 const long long* counter, int slot, const float* scale, TensorJob<T> J) {
+  if ((int)blockIdx.x < table_blocks) {
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n_rows) table_row(r, target, row_ptr, F, tab, m, cmax, base, i0, u0, coef, counter, slot, scale);
+    return;
+  }
+  const long long q = (long long)(blockIdx.x - table_blocks) * blockDim.x + threadIdx.x;
   if (q >= J.nN) return;
   J.D[q] = add(J.D[q], sub(J.E[J.node[q]], J.Ep[q]));
 }

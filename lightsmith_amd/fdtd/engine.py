@@ -288,6 +288,8 @@ class Engine:
     # launch tiling (rows per lane, wavefronts per block, x cells per wavefront row, z planes per block; the E
     # kernel of an engine with pole state marches twice the z planes); see lsf_set_tiling
     tiling = (1, 4, 64, 8)
+    # launch the tensor nodes' D update with the E-side table correction when their nodes are disjoint
+    fuse_sparse = True
     # default of the `fused` argument: step eligible engines (real fields, no periodic axes, one row per lane,
     # 64-cell rows, no H-phase operations) with the single-pass kernel over a second buffer set
     single_pass = False
@@ -364,6 +366,7 @@ class Engine:
         self.tiling = tuple(int(v) for v in self.tiling)  # this engine's, whatever the class default becomes
         ry, waves, xw, zc = self.tiling
         check(self.lib.lsf_set_tiling(self._h, ry, waves, xw, zc), "lsf_set_tiling")
+        self._e_table_nodes, self._tensor_nodes = [], []  # native targets, for the fused launch below
 
         inv_eps = np.asarray(inv_eps, dtype=float)
         if inv_eps.shape != (3,) + self.shape:
@@ -394,6 +397,11 @@ class Engine:
         self.time_monitors = list(time_monitors)
         for m in self.time_monitors:
             self._bind_time_monitor(m)
+        # the tensor nodes' D update rides on the E-side table launch when no table target is a tensor node
+        self.fused_table_tensor = False
+        if self.fuse_sparse and self._tensor_nodes and self._e_table_nodes:
+            hit = np.intersect1d(np.concatenate(self._e_table_nodes), np.concatenate(self._tensor_nodes))
+            self.fused_table_tensor = not hit.size and self.lib.lsf_fuse_table_tensor(self._h) == 0
         self._graph_stream = None
         self._graphed = False
         if self._two_sets:
@@ -689,6 +697,7 @@ class Engine:
                                             ptr(arrs["f"]), ptr(arrs["h0"]), ptr(arrs["inv_a"]), ptr(arrs["sm"]),
                                             ptr(state)), "lsf_add_tensor_group")
         self.n_tensor_nodes = getattr(self, "n_tensor_nodes", 0) + nN
+        self._tensor_nodes.append(nat)
 
     # --- off-diagonal tensor terms (engine.py:421-518, uniform grid) ------------------------------
     def _build_offdiag(self, od, groups):
@@ -818,6 +827,8 @@ class Engine:
         if base.size and (i0.min() < 0 or (base + i0 // int(m)).min() < 0 or base.max() + int(cmax) + 6 > T.numel()):
             raise ValueError("table source: reads outside the tables")
         fld = 0 if field_ == "E" else 1
+        if fld == 0 and phase == PH_CORRECT_E:
+            self._e_table_nodes.append(uniq)
         sc = self.correction_scale(uniq) if fld == 0 else None
         arrs = [self.to_device(uniq), self.to_device(row_ptr), T,
                 self.to_device(np.asarray(base, np.int64)[order]), self.to_device(np.asarray(i0, np.int64)[order]),

@@ -53,6 +53,7 @@ def test_au_sphere_full_grid_matches_the_fp64_oracle(gpu):
     sim, opts, scene = workloads.au_sphere_scene(gpu)
     _advance_source(scene.source, 3200)
     g = scene.engine(gpu)
+    assert g.fused_table_tensor  # the checked path is the one the solve runs
     o = cases.oracle_scattering_engine(scene)
     chunks = 3
     for q in range(chunks):
@@ -73,6 +74,30 @@ def test_au_sphere_full_grid_matches_the_fp64_oracle(gpu):
           ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
     g.close()
     assert all(v < TOL_F32_FLOOR for v in errs.values()), errs
+
+
+def test_fused_table_and_tensor_launch_is_bit_identical(gpu):
+    """The tensor nodes' D update launched with the E-side table correction (one kernel) against the two launches:
+    every state tensor equal after 3 chunks, with the source 3200 steps into its pulse."""
+    from lightsmith_amd.fdtd import workloads
+    from lightsmith_amd.fdtd.engine import Engine
+    sim, opts, scene = workloads.au_sphere_scene(gpu)
+    _advance_source(scene.source, 3200)
+    fused = scene.engine(gpu)
+    Engine.fuse_sparse = False
+    try:
+        alone = scene.engine(gpu)
+    finally:
+        Engine.fuse_sparse = True
+    assert fused.fused_table_tensor and not alone.fused_table_tensor
+    for _ in range(3):
+        fused.chunk()
+        alone.chunk()
+    same = [torch.equal(a, b) for a, b in zip(fused.state_tensors(), alone.state_tensors())]
+    print(f"fused vs separate launches: {sum(same)}/{len(same)} state tensors equal", flush=True)
+    fused.close()
+    alone.close()
+    assert all(same)
 
 
 def test_au_sphere_4000_steps_match_the_oracle_fixture(gpu):
