@@ -184,6 +184,12 @@ int lsf_set_energy(lsf_engine* eng, float* dev_energy);
  * twice as many). No reference counterpart (a launch choice). */
 int lsf_set_tiling(lsf_engine* eng, int rows_per_lane, int waves, int x_cells, int z_chunk);
 
+/* Order of the E launch's tiles: reverse_e = 1 (the default) walks each XCD's contiguous tile range from its end,
+ * so the E kernel starts on the tiles whose data the H kernel touched last and the next H kernel on those the E
+ * kernel touched last (the 256 MB memory-side cache still holds part of them); 0 walks forward like the H launch.
+ * Results do not depend on it (tiles are independent within a launch). No counterpart in the reference. */
+int lsf_set_tile_order(lsf_engine* eng, int reverse_e);
+
 /* Run n_steps steps (Engine.step, engine.py:665-678) on `stream` (a hipStream_t; NULL = default). */
 int lsf_step(lsf_engine* eng, int n_steps, void* stream);
 /* Run n_steps steps as lsf_step does, allocating nothing. No counterpart in the reference: for callers that cannot

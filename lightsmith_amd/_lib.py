@@ -46,6 +46,7 @@ _SIGNATURES = {
     "lsf_set_energy": [c_vp, c_vp],
     "lsf_set_tiling": [c_vp, c_int, c_int, c_int, c_int],
     "lsf_fuse_table_tensor": [c_vp],
+    "lsf_set_tile_order": [c_vp, c_int],
     "lsf_step": [c_vp, c_int, c_vp],
     "lsf_step_noalloc": [c_vp, c_int, c_vp],
     "lsf_step_timed": [c_vp, c_int, c_vp, ctypes.POINTER(c_dbl)],

@@ -133,6 +133,7 @@ struct lsf_engine {
   int* mon_tab = nullptr;    // library scratch: [n_mon + 1] offsets, [n_mon] groups
   int n_partial = 0;
   int ry = 2, waves = 4, xw = 64, zc = 16;
+  int rev_e = 1;  // the E launch walks each XCD's tiles backwards (lsf_set_tile_order)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   lsf::StepArgs* dev_args = nullptr;  // library scratch: [0] E kernel, [1] H kernel arguments
@@ -415,8 +416,13 @@ void launch_h(lsf_engine* e, hipStream_t s) {
 }
 
 void launch_e(lsf_engine* e, hipStream_t s) {
-  if (e->adjoint) lsf::launch_adjoint_e(own_launch(e, 0), s);
-  else lsf::launch_update_e(own_launch(e, 0), s);
+  if (e->adjoint) {
+    lsf::launch_adjoint_e(own_launch(e, 0), s);
+  } else {
+    lsf::KLaunch L = own_launch(e, 0);
+    L.sc.rev = e->rev_e;
+    lsf::launch_update_e(L, s);
+  }
   e->launched = true;
 }
 
@@ -1176,6 +1182,13 @@ int lsf_set_tiling(lsf_engine* e, int ry, int waves, int xw, int zc) {
   e->waves = waves;
   e->xw = xw;
   e->zc = zc;
+  return LSF_OK;
+}
+
+int lsf_set_tile_order(lsf_engine* e, int reverse_e) {
+  if (int rc0 = configurable(e)) return rc0;
+  if (e->exec) return fail(LSF_ESTATE, "the engine has a captured graph; reset it before changing the tile order");
+  e->rev_e = reverse_e ? 1 : 0;
   return LSF_OK;
 }
 

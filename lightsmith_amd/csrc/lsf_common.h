@@ -125,10 +125,13 @@ __device__ __forceinline__ int vregi(int x) {
 }
 
 // Blocks are dealt round-robin to the 8 XCDs (each with its own L2). Give each XCD a contiguous range of
-// logical tiles so that tiles sharing halo planes share an L2. The grid is padded to a multiple of 8.
-__device__ __forceinline__ int xcd_tile(int b, int nblocks_padded) {
+// logical tiles so that tiles sharing halo planes share an L2. The grid is padded to a multiple of 8. With `rev`
+// each XCD walks its range from the end, so a launch that follows a forward one starts on the tiles whose data the
+// previous launch touched last.
+__device__ __forceinline__ int xcd_tile(int b, int nblocks_padded, int rev = 0) {
   const int per = nblocks_padded >> 3;
-  return (b & 7) * per + (b >> 3);
+  const int k = b >> 3;
+  return (b & 7) * per + (rev ? per - 1 - k : k);
 }
 
 }  // namespace lsf

@@ -53,6 +53,7 @@ struct Sched {
   const int* first;    // [members] first tile of each member
   int n_tiles;
   int n_pad;           // n_tiles rounded up to a multiple of 8 (the period of the XCD-aware order)
+  int rev = 0;         // each XCD walks its tile range backwards (xcd_tile)
 };
 
 // D-form media table in device memory: n_media * (2 + 3 * max_poles) floats
@@ -329,7 +330,7 @@ __device__ __forceinline__ void h_tile(const StepArgs& a, int t) {
 template <class T, int RY, int WAVES, int XW, bool PER, bool LOOP>
 __global__ void __launch_bounds__(64 * WAVES) k_update_h(const StepArgs* __restrict__ args, Sched sc) {
   for (int b = blockIdx.x; b < sc.n_pad; b += gridDim.x) {
-    const int t = xcd_tile(b, sc.n_pad);
+    const int t = xcd_tile(b, sc.n_pad, sc.rev);
     if (t < sc.n_tiles) {
       int m, lt;
       member_of(sc, t, m, lt);
@@ -711,7 +712,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_update_e(const StepArgs* __restr
   __shared__ float4 med_tab[NP > 0 ? MAX_MEDIA_VEC4 : 1];
   int cur = -1;
   for (int b = blockIdx.x; b < sc.n_pad; b += gridDim.x) {
-    const int t = xcd_tile(b, sc.n_pad);
+    const int t = xcd_tile(b, sc.n_pad, sc.rev);
     if (t >= sc.n_tiles) {
       if (!LOOP) break;
       continue;
