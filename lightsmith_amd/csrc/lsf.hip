@@ -1702,8 +1702,13 @@ void level_h_kernels(Level& L, hipStream_t s) {
 
 void level_e_kernels(Level& L, hipStream_t s) {
   for (KGroup& g : L.eg) {
-    if (g.adj) lsf::launch_adjoint_e(g.L, s);
-    else lsf::launch_update_e(g.L, s);
+    if (g.adj) {
+      lsf::launch_adjoint_e(g.L, s);
+    } else {
+      lsf::KLaunch K = g.L;
+      K.sc.rev = 1;  // each XCD from the end of its range, where the H launch ended (as a single engine's E)
+      lsf::launch_update_e(K, s);
+    }
   }
 }
 
