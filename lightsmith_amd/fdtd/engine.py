@@ -162,12 +162,13 @@ def cpml_profile(spec: PMLSpec, n_lo: int, n_hi: int, N: int, h: float, dt: floa
     return b, c, kap
 
 
-def filling_tiling(shapes, want: int = 8192) -> tuple:
+def filling_tiling(shapes, want: int = 16384) -> tuple:
     """A launch tiling (1 row per lane, 4 wavefronts per block, x cells per row, z chunk) for grids of these shapes
     (reference order: axis 0 along the rows, axis 2 marched) stepped together: 32-cell rows when no grid is wider,
     else 64, and the longest z chunk of 8, 4 or 2 planes that still gives the bulk kernels ``want`` row marches
-    (8 per SIMD on 256 CUs). Small grids fill the GPU only with short z marches: the 97^3 Au sphere steps in
-    47.5 us at chunk 2 against 52.6 us at 4 and 69.4 us at 8; a 320^3 grid keeps chunk 8."""
+    (16 per SIMD on 256 CUs). Small grids fill the GPU only with short z marches. Measured per step on the
+    throughput scene at chunks 8 / 4 / 2 (us): 96^3 54.3 / 36.7 / 31.9, 128^3 61.3 / 49.8 / 50.1, 160^3
+    87.9 / 87.3 / 88.3, 192^3 175.9 / 166.6 / 167.4, 256^3 378.9 / 373.1 / 384.6; 320^3 is fastest at 8."""
     shapes = [tuple(int(v) for v in s) for s in shapes]
     xw = 32 if max(s[0] for s in shapes) <= 32 else 64
     for zc in (8, 4, 2):
