@@ -27,7 +27,7 @@ import numpy as np
 import torch
 
 from .._lib import check, ptr
-from .engine import DFTMonitor, Engine, PMLSpec
+from .engine import DFTMonitor, Engine, PMLSpec, filling_tiling
 
 
 @dataclass
@@ -109,6 +109,9 @@ class AdjointProblem:
             raise ValueError("the adjoint transposes pure D-form groups only (no mixed or vertex-tensor groups)")
         self.fwd = Engine(device, shape=shape, dx=dx, dt=dt, single=True, inv_eps=inv_eps, groups=groups, pml=pml,
                           sources=sources, monitors=[monitor], dft_stride=self.stride, material="dense", fused=False)
+        ry, waves, _, zc = filling_tiling([self.fwd.shape])
+        tiling = (ry, waves, 64, zc)  # the adjoint kernels take 64-cell rows
+        self.fwd.set_tiling(*tiling)
         self.monitor = monitor
         dev = self.fwd._dev
         self._dev = dev
@@ -142,6 +145,7 @@ class AdjointProblem:
         # set_density updates both at the design nodes
         self.adj = Engine(device, shape=shape, dx=dx, dt=dt, single=True, inv_eps=inv_eps, pml=pml, sources=[],
                           monitors=[], dft_stride=1, material="dense", fused=False)
+        self.adj.set_tiling(*tiling)
         self._adj_mat = self.fwd._mat.clone()
         if self.fwd.n_ade:
             self._adj_mat[torch.as_tensor(self.fwd.ade_nodes, device=dev)] = float(self.fwd.cdx)
@@ -345,9 +349,11 @@ class BatchedAdjoint:
                                  "and tape mode")
         self.stride, self.K, self.n_steps, self.store_all = p0.stride, p0.K, p0.n_steps, p0.store_all
         self.use_graphs = use_graphs
-        # the members' own tile shape (one block per tile) unless told otherwise: the batch default suits the
-        # pole-free complex sweep; engines with up to 4 poles per node need one row per lane to keep registers
-        sched = schedule or (*p0.fwd.tiling, 0)
+        # one row per lane and 64-cell rows as the members' own tiling (engines with up to 4 poles per node need
+        # one row per lane to keep registers; the adjoint kernels take 64-cell rows), with the z chunk that fills
+        # the GPU over all members together, one block per tile
+        ry, waves, _, zc = filling_tiling([p.fwd.shape for p in self.problems])
+        sched = schedule or (ry, waves, 64, zc, 0)
         self.fwd = Batch([p.fwd for p in self.problems], schedule=sched)
         self.adj = Batch([p.adj for p in self.problems], schedule=sched)
         self._run = None
