@@ -416,13 +416,10 @@ void launch_h(lsf_engine* e, hipStream_t s) {
 }
 
 void launch_e(lsf_engine* e, hipStream_t s) {
-  if (e->adjoint) {
-    lsf::launch_adjoint_e(own_launch(e, 0), s);
-  } else {
-    lsf::KLaunch L = own_launch(e, 0);
-    L.sc.rev = e->rev_e;
-    lsf::launch_update_e(L, s);
-  }
+  lsf::KLaunch L = own_launch(e, 0);
+  L.sc.rev = e->rev_e;
+  if (e->adjoint) lsf::launch_adjoint_e(L, s);
+  else lsf::launch_update_e(L, s);
   e->launched = true;
 }
 
@@ -1702,13 +1699,10 @@ void level_h_kernels(Level& L, hipStream_t s) {
 
 void level_e_kernels(Level& L, hipStream_t s) {
   for (KGroup& g : L.eg) {
-    if (g.adj) {
-      lsf::launch_adjoint_e(g.L, s);
-    } else {
-      lsf::KLaunch K = g.L;
-      K.sc.rev = 1;  // each XCD from the end of its range, where the H launch ended (as a single engine's E)
-      lsf::launch_update_e(K, s);
-    }
+    lsf::KLaunch K = g.L;
+    K.sc.rev = 1;  // each XCD from the end of its range, where the H launch ended (as a single engine's E)
+    if (g.adj) lsf::launch_adjoint_e(K, s);
+    else lsf::launch_update_e(K, s);
   }
 }
 

@@ -418,7 +418,7 @@ __device__ __forceinline__ void ae_tile(const StepArgs& a, int t, const float* t
 template <int MAT, int RY, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES) k_adjoint_e(const StepArgs* __restrict__ args, Sched sc) {
   __shared__ float tab[MAT == 1 ? 256 : 1];
-  const int t = xcd_tile(blockIdx.x, sc.n_pad);
+  const int t = xcd_tile(blockIdx.x, sc.n_pad, sc.rev);
   if (t >= sc.n_tiles) return;
   int m, lt;
   member_of(sc, t, m, lt);
